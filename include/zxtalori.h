@@ -1,0 +1,102 @@
+/* zxtalori C ABI, MI355X (gfx950) build: libzxtalori.so.
+ *
+ * Drop-in for the reference's exported table (src/exports.zig, src/native_api.zig), which
+ * its Python adapter resolves at load time (python/zxtalori/api.py:29-53).  Every entry
+ * keeps the reference's signature, return convention (0 success, -1 failure with the
+ * message in zo_error; NULL from zo_create with the message in zo_create_error),
+ * ownership (returned pointers belong to the engine and stay valid until the next
+ * mutating call) and threading (one engine per thread; zo_cancel from any thread).
+ *
+ * Group ids 0..10 are C2 C3 C4 C6 D2 D3 D4 D6 T O I as in the reference
+ * (src/exports.zig:10, src/groups.zig:2); id 11 is C1 (identity only), an extension.
+ */
+#ifndef ZXTALORI_H
+#define ZXTALORI_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct zxt_engine zxt_engine;
+
+/* src/exports.zig:4-6 */
+const char* zo_version(void);
+
+/* src/native_api.zig:9-21.  The argument (the reference's wgpu-native directory) is
+ * ignored; the engine takes HIP device 0. */
+zxt_engine* zo_create(const char* wgpu_dir);
+/* src/native_api.zig:20-22 */
+const char* zo_create_error(void);
+/* src/native_api.zig:23-29 */
+void zo_destroy(zxt_engine* e);
+
+/* src/exports.zig:83-87: seeded Haar-random orientations, folded into the zone. */
+int32_t zo_random(zxt_engine* e, uint32_t group, uint32_t count, uint32_t seed);
+/* src/exports.zig:23-28: import n quaternions (scalar first, copied), folded. */
+int32_t zo_set(zxt_engine* e, uint32_t group, const float* q, uint32_t n);
+/* src/exports.zig:9-15: inverse Hopf / FCC generation.  Not part of this build's path:
+ * returns -1 with "Generation failed: NotImplemented". */
+int32_t zo_generate(zxt_engine* e, uint32_t group, uint32_t h, uint32_t target, uint32_t seed, uint32_t kind);
+
+/* src/exports.zig:29-40 */
+uint32_t zo_count(zxt_engine* e);
+uint32_t zo_order(zxt_engine* e);
+const float* zo_operations(zxt_engine* e); /* zo_order() rows of 4 floats */
+const float* zo_quaternions(zxt_engine* e); /* zo_count() rows of 4 floats */
+
+/* src/exports.zig:41-44: energy and separation (and the energy ratios below the ratio limit). */
+int32_t zo_measure(zxt_engine* e);
+/* src/exports.zig:49-52: up to 1000 BB steps. */
+int32_t zo_relax(zxt_engine* e, uint32_t steps);
+/* src/exports.zig:55-58: 0 (all pairs) or 2..1000 mean orientation radii. */
+int32_t zo_relax_neighbourhood(zxt_engine* e, float delta);
+/* src/exports.zig:60-63: each point's nearest orbit neighbour and the separation. */
+int32_t zo_spacing(zxt_engine* e);
+/* src/exports.zig:66-68: kind 0 nearest-neighbour SO(3) degrees; NULL if not measured. */
+const float* zo_point_values(zxt_engine* e, uint32_t kind);
+/* src/exports.zig:69-72: bit 0 add the identity, bit 1 hold it fixed. */
+int32_t zo_identity(zxt_engine* e, uint32_t flags);
+/* src/exports.zig:73-81 */
+const char* zo_stats(zxt_engine* e);
+const char* zo_error(zxt_engine* e);
+void zo_cancel(zxt_engine* e);
+
+/* Out of this build's path (covering radius, Delaunay mesh, exact counts), present so the
+ * reference's loader resolves every name (src/exports.zig:88-120).  They return -1 (or
+ * 0/NULL) with a "NotImplemented" message. */
+int32_t zo_cover(zxt_engine* e);
+int32_t zo_cover_optimize(zxt_engine* e, uint32_t steps, uint32_t mode, uint32_t finish, float floor_deg);
+int32_t zo_cover_kicks(zxt_engine* e, uint32_t patience, float progress, float size, float trust);
+int32_t zo_mesh(zxt_engine* e);
+const float* zo_mesh_vertices(zxt_engine* e);
+const uint32_t* zo_mesh_tets(zxt_engine* e);
+uint32_t zo_mesh_vertex_count(zxt_engine* e);
+uint32_t zo_mesh_tet_count(zxt_engine* e);
+int32_t zo_generate_exact(zxt_engine* e, uint32_t group, uint32_t target, uint32_t flags);
+
+/* Development hooks (src/native_api.zig:31-43). */
+/* Per-kernel HIP-event timing on the engine's stream; returns the new state. */
+bool zo_profile(zxt_engine* e, bool enabled);
+/* Tab-separated "kernel count total_ms" lines; returns the bytes written. */
+size_t zo_profile_report(zxt_engine* e, char* output, size_t capacity);
+/* Scales the all-pairs work split (tests shrink it: only rounding may change). */
+void zo_pair_budget(zxt_engine* e, float scale);
+
+/* Additions of this build (no reference counterpart). */
+/* Per point, the rank j*2*order + 2*k + sign of its nearest orbit point (source j,
+ * operator k, sign 0 for +q_j*g_k), from the last zo_spacing; NULL before one. */
+const uint32_t* zo_point_indices(zxt_engine* e);
+/* Runs `passes` force passes of the current neighbourhood on the current positions and
+ * waits; nothing is read back (throughput measurement). */
+int32_t zo_force_passes(zxt_engine* e, uint32_t passes);
+/* The last near-field pass's site count, shell count and cell grid dimension. */
+int32_t zo_near_info(zxt_engine* e, uint32_t* out3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

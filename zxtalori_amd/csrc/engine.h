@@ -1,0 +1,173 @@
+// Host engine of the MI355X build: one sample, its device state and the relaxation,
+// measure and spacing drivers (the roles of src/engine.zig, src/pairs.zig RelaxState and
+// src/near.zig in the reference), over the HIP runtime with one stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace zxt {
+
+// A grown-only device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release();
+  // Ensures at least `b` bytes; contents are not kept.  Throws on failure.
+  void ensure(size_t b);
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct EngineError {
+  std::string message;
+};
+
+// measure() results (pairs.zig:17-30).
+struct Measures {
+  double count_extended = NAN;
+  double riesz_1 = NAN, riesz_2 = NAN, riesz_3 = NAN, green_ratio = NAN, log_ratio = NAN;
+  double quey_energy = NAN;
+  double separation_deg = NAN;
+  double force_rms = NAN;
+};
+
+struct StepResult {
+  double max_step_deg = 0, force_rms = 0, energy = 0;
+};
+
+class Engine;
+
+// Near-field sites, cells and constants of one sample (near.zig:31-196).
+struct NearField {
+  double delta = 0;
+  double taper = 0.5;
+  uint32_t n = 0, order = 0, group = 0;
+  bool ready = false;
+  double radius = 0;
+  NearCut cut{};
+  uint32_t capacity = 0, sdim = 1, nplanes = 0;
+  float band = 0;
+  // The last pass's sites, shell and overflow flag (valid after a wait).
+  NearInfo status{};
+  DevBuf planes, info, rim, uq, ucode, urank, ucell, perm, sq, scode, srank, offsets, cursor, src_cell, src_offsets,
+      src_cursor, order_buf, scan_tmp;
+  NearBuffers view() const;
+  void setup(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius);
+  // Records sites and sorts for the positions q (setting up first when needed).
+  void build(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius);
+  // After a wait: whether the last pass overflowed (then the next pass sizes again).
+  bool overflowed(Engine& e);
+};
+
+// Relaxation state (pairs.zig:220-413): rows, BB history, step state, near field.
+struct RelaxState {
+  uint32_t n = 0;
+  uint32_t order = 0;
+  bool current = false;
+  bool has_near = false;
+  NearField near;
+  DevBuf force, energy, nearest, prev, state, partials, ap_partial, ap_pnear, nn_partial, nn_chord, nn_rank;
+  Rows rows() const { return Rows{force.as<float4>(), energy.as<double>(), nearest.as<float>()}; }
+  void release();
+  void set_near(uint32_t group, const double* delta);
+  void prepare(Engine& e, uint32_t n);
+  void ensure_rows(uint32_t n);
+  void record_forces(Engine& e, uint32_t n, uint32_t order);
+};
+
+class Engine {
+ public:
+  Engine();
+  ~Engine();
+
+  // Public operations (src/engine.zig).  They throw EngineError.
+  void set(uint32_t group, const float* values, size_t count4);
+  void generate_random(uint32_t group, uint32_t count, uint32_t seed);
+  void measure();
+  void spacing();
+  void relax(uint32_t steps);
+  void set_neighbourhood(double delta);
+  void identity(bool add, bool fixed);
+  void force_passes(uint32_t passes);
+  std::string stats();
+  const float* point_values(uint32_t kind) const;
+  const uint32_t* point_indices() const;
+
+  uint32_t count() const { return n_; }
+  uint32_t group() const { return group_; }
+  uint32_t order() const;
+  const float* operations() const;
+  const float* quaternions() const { return host_.data(); }
+  uint32_t ratio_limit() const;
+
+  std::atomic<bool> cancelled{false};
+  Prof prof;
+  double pair_budget_scale = 1.0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::string text;
+  NearInfo last_near{};
+  uint32_t last_near_sdim = 0;
+
+  const float4* ops_device() const;
+  void sync();
+  template <typename T>
+  void download(std::vector<T>& dst, const DevBuf& src, size_t count) {
+    dst.resize(count);
+    check(hipMemcpyAsync(dst.data(), src.p, count * sizeof(T), hipMemcpyDeviceToHost, stream), "download");
+  }
+  static void check(hipError_t e, const char* what);
+
+ private:
+  void set_group(uint32_t group);
+  void fold(uint32_t n);
+  void fetch();
+  void adopt(uint32_t n);
+  void reset_run();
+  long identity_index() const;
+  void sync_neighbourhood();
+  Measures relax_measure(RelaxState& rs, bool keep_separation);
+  double nearest_pass(RelaxState& rs);
+  Measures quick_measures(const std::vector<float4>& f, const std::vector<double>& energy,
+                          const std::vector<float>& nearest) const;
+  std::vector<float>& separation_out();
+
+  uint32_t group_ = 9;
+  DevBuf ops_, q_;
+  std::vector<float> host_;
+  uint32_t n_ = 0;
+  uint32_t iteration_ = 0;
+  uint32_t seed_ = 0;
+  uint32_t base_count_ = 0;
+  std::string generator_ = "none";
+  bool measured_ = false;
+  Measures measures_;
+  uint32_t measure_iteration_ = 0;
+  bool full_measures_ = false;
+  RelaxState relax_state_, spacing_state_;
+  double neighbourhood_ = 0, measure_neighbourhood_ = 0;
+  std::vector<float> point_separation_;
+  std::vector<uint32_t> point_index_;
+  long point_separation_iteration_ = -1;
+  long point_index_iteration_ = -1;
+  double step_deg_ = 0, force_rms_ = 0;
+  bool fix_identity_ = false;
+  long fixed_index_ = -1;
+};
+
+double mean_radius(uint64_t count);
+
+}  // namespace zxt

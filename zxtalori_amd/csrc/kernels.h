@@ -1,0 +1,162 @@
+// Launch interface between the host engine (engine.cpp) and the gfx950 kernels
+// (kernels.hip).  Plain structs of device pointers; every launch goes to one stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+namespace zxt {
+
+// Per-source force rows of one pass: (fx, fy, fz, curvature) in FP32, the energy sum in
+// FP64 and the nearest chord^2 (over every orbit point but the source itself).
+// The reference keeps these in a 48-byte Row (pairs.zig:11-15); the build stores only
+// the columns the relaxation path reads.
+struct Rows {
+  float4* force = nullptr;
+  double* energy = nullptr;
+  float* nearest = nullptr;
+};
+
+// Near-field constants of one sample (near.zig:95, :176; pairs.wgsl:349-361).
+struct NearCut {
+  float chord2;       // accept a site when chord^2(x, y) <= chord2
+  float taper_start;  // window starts falling at this S3 angle (0.5 r)
+  float inv_width;    // 1 / (0.5 r)
+  float c0;           // x.y >= c0 bounds every accepted site (with margin)
+  float s0;           // sqrt(1 - c0^2)
+};
+
+// Device-resident status of one near-field pass.  lo/hi are order-preserving u32 keys of
+// the sites' chart box; lower/cpu the box the cell grid uses this pass.
+struct NearInfo {
+  uint32_t sites;     // sites written (core + shell), may exceed capacity
+  uint32_t overflow;  // set when sites > capacity
+  uint32_t rim;       // rim sources listed
+  uint32_t pad0;
+  uint32_t lo[3];
+  uint32_t hi[3];
+  float lower[3];
+  float cpu;          // cells per chart unit
+};
+
+// BB step state, device-resident (relax.wgsl:17-22 keeps the same in 16 words).
+struct BBState {
+  uint32_t iteration;
+  uint32_t flags;      // 1 non-finite force, 2 duplicate orientation
+  uint32_t step_bits;  // f32 bits of the largest move this step
+  uint32_t round;
+  double sy, yy, ss, ff, energy;
+  uint32_t rank[2];    // remaining rank of the two middle ranks
+  uint32_t prefix[2];  // selected high bits so far
+  float median;
+  float alpha;
+  double alpha64;
+  uint32_t hist[512];
+};
+
+// Scratch of an all-pairs pass: partial rows of `splits` (operator batch x site chunk).
+struct AllPairsPlan {
+  uint32_t ops_per_split = 1;
+  uint32_t site_chunk = 256;
+  uint32_t site_splits = 1;
+  uint32_t op_splits = 1;
+  uint32_t splits() const { return ops_per_split ? site_splits * op_splits : 0; }
+};
+
+// Per-launch timing with HIP events on the engine's stream (zo_profile).
+class Prof {
+ public:
+  explicit Prof(hipStream_t s = nullptr) : stream_(s) {}
+  ~Prof();
+  void set_stream(hipStream_t s) { stream_ = s; }
+  bool on() const { return on_; }
+  void enable(bool v);
+  void run(const char* name, const std::function<void()>& launch);
+  // Waits for recorded launches and folds them into the totals.
+  void collect();
+  std::string report();
+  void clear();
+  // Totals in ms and launch counts, by kernel name.
+  struct Total {
+    std::string name;
+    double ms;
+    uint64_t count;
+  };
+  std::vector<Total> totals();
+
+ private:
+  struct Rec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  hipStream_t stream_;
+  bool on_ = false;
+  std::vector<Rec> pending_;
+  std::vector<hipEvent_t> pool_;
+  std::vector<Total> totals_;
+  hipEvent_t get_event();
+};
+
+// ---- launchers (kernels.hip) ----------------------------------------------------
+void launch_haar(Prof& p, hipStream_t s, float4* q, uint32_t n, uint32_t seed);
+void launch_fold(Prof& p, hipStream_t s, float4* q, uint32_t n, const float4* ops, uint32_t order, uint32_t group);
+void launch_fill_u32(Prof& p, hipStream_t s, uint32_t* dst, uint32_t value, size_t count);
+
+AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale);
+void launch_allpairs(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                     const AllPairsPlan& plan, double* partial, float* partial_nearest, Rows out);
+void launch_allpairs_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                             uint64_t* partial, float* chord_out, uint32_t* rank_out);
+
+// Near field.  Buffers are owned by the engine's NearField (engine.cpp).
+struct NearBuffers {
+  const float4* planes = nullptr;
+  uint32_t nplanes = 0;
+  float band = 0;        // sin(r + 1e-4): rim and shell margin (shell.wgsl:19-22)
+  uint32_t capacity = 0;
+  uint32_t sdim = 1;
+  NearInfo* info = nullptr;
+  uint32_t* rim = nullptr;       // n
+  float4* uq = nullptr;          // capacity, unsorted sites
+  uint32_t* ucode = nullptr;     // capacity
+  uint32_t* urank = nullptr;     // capacity
+  uint32_t* ucell = nullptr;     // capacity
+  uint32_t* perm = nullptr;      // capacity
+  float4* sq = nullptr;          // capacity, sorted by cell
+  uint32_t* scode = nullptr;     // capacity
+  uint32_t* srank = nullptr;     // capacity
+  uint32_t* offsets = nullptr;   // sdim^3 + 1
+  uint32_t* cursor = nullptr;    // sdim^3 + 1
+  uint32_t* src_cell = nullptr;  // n
+  uint32_t* src_offsets = nullptr;  // kMortonCells + 1
+  uint32_t* src_cursor = nullptr;   // kMortonCells + 1
+  uint32_t* order = nullptr;     // n, sources in Morton order
+  uint32_t* scan_tmp = nullptr;  // scan_tmp_words(max cells)
+};
+constexpr uint32_t kMortonBits = 6;
+constexpr uint32_t kMortonCells = 1u << (3 * kMortonBits);
+size_t scan_tmp_words(size_t count);
+
+// Writes the sites (core + shell) and their chart box only (sizing pass).
+void launch_near_sites(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                       const NearBuffers& b);
+// Builds the sites (core + shell), sorts them into the cell grid and the sources into
+// Morton order.  Reads nothing back.
+void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                       const NearBuffers& b);
+// Force rows (mode 0) or nearest orbit point with rank (mode 1) over the built sites.
+void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, const NearBuffers& b, const NearCut& cut,
+                       Rows out);
+void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, uint32_t order, const NearBuffers& b,
+                         const NearCut& cut, float* chord_out, uint32_t* rank_out);
+
+// BB step (relax.wgsl), resident: stats, sums, median, move.
+void launch_bb_step(Prof& p, hipStream_t s, float4* q, uint32_t n, Rows rows, float4* prev, BBState* state,
+                    double* partials, double radius, float step_scale, uint32_t fixed);
+uint32_t bb_partials(uint32_t n);
+void launch_bb_reset(Prof& p, hipStream_t s, BBState* state);
+
+}  // namespace zxt

@@ -1,0 +1,1198 @@
+// gfx950 kernels of the QVM relaxation and spacing path.
+//
+// Reference kernels they replace (shaders/*.wgsl), by subsystem:
+//   haar, fold ................. haar.wgsl:5-10, state.wgsl:31-51
+//   all-pairs rows ............. pairs.wgsl:263-326 (quey_partial, quey_reduce)
+//   near-field sites ........... shell.wgsl:66-130 (rim, shell), near.wgsl:22-82 (sorts, images)
+//   near-field rows ............ pairs.wgsl:363-427 (near_force, near_reduce)
+//   nearest orbit point ........ pairs.wgsl:428-434 and the nearest column of the rows
+//   BB step .................... relax.wgsl:29-165
+//
+// Design (MI355X: 256 CUs in 8 XCDs, wave64, 160 KB LDS per CU):
+//  * All pairs: one thread per source, 256-source workgroups, images of a 256-site tile
+//    built in LDS per operator (never stored), (operator batch x site chunk) splits in
+//    blockIdx.y so small samples still launch thousands of workgroups; partial rows in
+//    FP64, reduced in a fixed order.
+//  * Near field: one wave per source.  A source's cutoff cap is a ball in the
+//    stereographic chart (centre v/(c0+w), radius sqrt(1-c0^2)/(c0+w)); the wave lists
+//    the cell rows that ball crosses, trims each row in x, and streams the rows' site runs
+//    64 candidates at a time.  Candidates within the cutoff are compacted through an LDS
+//    ring (ballot + mbcnt), so the ~100-op pair kernel always runs on full waves; the
+//    accepted order is deterministic, and so are the sums.  Workgroups are remapped so each
+//    XCD takes a contiguous Morton range of sources and its L2 holds their sites.
+//  * Sorts are counting sorts (atomic counts, a three-kernel scan, atomic scatter, then an
+//    in-cell sort by a unique key), so every pass is deterministic.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "kernels.h"
+#include "qvm_device.h"
+
+namespace zxt {
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+
+inline uint32_t blocks_for(size_t n, uint32_t per) { return uint32_t((n + per - 1) / per); }
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint32_t mbcnt(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+// Appends the lanes with pred to a global counter; returns each such lane's slot.
+__device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return 0;
+  const int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (int(lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+  base = __shfl(base, leader);
+  return base + mbcnt(mask);
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    uint32_t o = __shfl_up(v, d);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+__device__ inline float wave_min(float v) {
+  for (int d = 32; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d));
+  return v;
+}
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+  for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
+  return v;
+}
+__device__ inline uint64_t wave_min_u64(uint64_t v) {
+  for (int d = 32; d > 0; d >>= 1) {
+    uint32_t lo = __shfl_xor(uint32_t(v), d), hi = __shfl_xor(uint32_t(v >> 32), d);
+    uint64_t o = (uint64_t(hi) << 32) | lo;
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+// Workgroup b runs on XCD b % 8: give each XCD a contiguous range of virtual blocks.
+__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t grid) {
+  const uint32_t per = grid / 8;  // grid is a multiple of 8
+  return (b % 8) * per + b / 8;
+}
+inline uint32_t xcd_grid(uint32_t blocks) { return (blocks + 7) / 8 * 8; }
+
+__device__ inline uint32_t hash32(uint32_t v) {
+  uint32_t x = v;
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  return x ^ (x >> 16);
+}
+__device__ inline float random_unit(uint32_t v) { return (float(hash32(v) >> 8) + 0.5f) / 16777216.0f; }
+
+__device__ inline float3 chart(const float4 q) {
+  const float d = 1.0f + q.x;
+  return make_float3(q.y / d, q.z / d, q.w / d);
+}
+
+}  // namespace
+
+// ---- Prof ---------------------------------------------------------------------------
+Prof::~Prof() {
+  for (auto& r : pending_) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : pool_) (void)hipEventDestroy(e);
+}
+hipEvent_t Prof::get_event() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+void Prof::enable(bool v) {
+  if (!v) collect();
+  on_ = v;
+}
+void Prof::run(const char* name, const std::function<void()>& launch) {
+  if (!on_) {
+    launch();
+    return;
+  }
+  Rec r{name, get_event(), get_event()};
+  if (r.a) (void)hipEventRecord(r.a, stream_);
+  launch();
+  if (r.b) (void)hipEventRecord(r.b, stream_);
+  pending_.push_back(r);
+  if (pending_.size() > 4096) collect();
+}
+void Prof::collect() {
+  if (pending_.empty()) return;
+  (void)hipStreamSynchronize(stream_);
+  for (auto& r : pending_) {
+    float ms = 0.0f;
+    if (r.a && r.b) (void)hipEventElapsedTime(&ms, r.a, r.b);
+    bool found = false;
+    for (auto& t : totals_) {
+      if (t.name == r.name) {
+        t.ms += ms;
+        t.count += 1;
+        found = true;
+        break;
+      }
+    }
+    if (!found) totals_.push_back({r.name, double(ms), 1});
+    if (r.a) pool_.push_back(r.a);
+    if (r.b) pool_.push_back(r.b);
+  }
+  pending_.clear();
+}
+std::vector<Prof::Total> Prof::totals() {
+  collect();
+  return totals_;
+}
+void Prof::clear() {
+  collect();
+  totals_.clear();
+}
+std::string Prof::report() {
+  collect();
+  std::string out;
+  char line[256];
+  for (auto& t : totals_) {
+    snprintf(line, sizeof(line), "%s\t%llu\t%.6f\n", t.name.c_str(), (unsigned long long)t.count, t.ms);
+    out += line;
+  }
+  return out;
+}
+
+// ---- haar (haar.wgsl:5-10) ---------------------------------------------------------
+__global__ void k_haar(float4* __restrict__ q, uint32_t n, uint32_t seed_hash) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t base = hash32(i ^ seed_hash);
+  const float u = random_unit(base);
+  const float a = 6.28318530718f * random_unit(base + 1u);
+  const float b = 6.28318530718f * random_unit(base + 2u);
+  const float su = sqrtf(u), sv = sqrtf(1.0f - u);
+  q[i] = make_float4(su * cosf(b), sv * sinf(a), sv * cosf(a), su * sinf(b));
+}
+
+void launch_haar(Prof& p, hipStream_t s, float4* q, uint32_t n, uint32_t seed) {
+  // hash(seed) on the host: the same integer arithmetic.
+  uint32_t x = seed;
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  p.run("haar", [&] { hipLaunchKernelGGL(k_haar, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, q, n, x); });
+}
+
+// ---- fold (state.wgsl:1-51) ---------------------------------------------------------
+__device__ bool in_zone(const float4 p, uint32_t group, const float4* __restrict__ ops, uint32_t order) {
+  const float ax = fabsf(p.y), ay = fabsf(p.z), az = fabsf(p.w), w = p.x;
+  switch (group) {
+    case 0: return az <= w;
+    case 1: return az <= 0.5773502691896258f * w;
+    case 2: return az <= 0.4142135623730951f * w;
+    case 3: return az <= 0.2679491924311227f * w;
+    case 4: return fmaxf(fmaxf(ax, ay), az) <= w;
+    case 5: return az <= 0.5773502691896258f * w && fmaxf(ax, 0.5f * ax + 0.8660254037844386f * ay) <= w;
+    case 6:
+      return az <= 0.4142135623730951f * w && fmaxf(fmaxf(ax, ay), 0.7071067811865476f * (ax + ay)) <= w;
+    case 7: {
+      const float m = fmaxf(ax, ay), nn = fminf(ax, ay);
+      return az <= 0.2679491924311227f * w && fmaxf(m, 0.8660254037844386f * m + 0.5f * nn) <= w;
+    }
+    case 8: return ax + ay + az <= w;
+    case 9: return fmaxf(fmaxf(ax, ay), az) <= 0.4142135623730951f * w && ax + ay + az <= w;
+    case 10: {
+      if (w > 0.9510566f) return true;
+      if (w < 0.9256147f) return false;
+      for (uint32_t k = 1; k < order; ++k) {
+        const float4 op = ops[k];
+        if (fabsf(fabsf(op.x) - 0.8090169943749475f) < 1e-5f && fabsf(dot4(p, op)) > w) return false;
+      }
+      return true;
+    }
+    default: return true;  // C1
+  }
+}
+
+__global__ void k_fold(float4* __restrict__ q, uint32_t n, const float4* __restrict__ ops, uint32_t order,
+                       uint32_t group) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = q[i];
+  const float scale = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w)));
+  const int maximum = int((__float_as_uint(scale) >> 23) & 255u);
+  float c[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t bits = __float_as_uint(c[k]);
+    const int exponent = int((bits >> 23) & 255u);
+    const int scaled = exponent + 127 - maximum;
+    const uint32_t shifted = (bits & 0x807fffffu) | (uint32_t(max(scaled, 0)) << 23);
+    const bool keep = exponent > 0 && scaled > 0;
+    c[k] = __uint_as_float(keep ? shifted : (bits & 0x80000000u));
+  }
+  float4 p = normalize4(make_float4(c[0], c[1], c[2], c[3]));
+  if (p.x < 0.0f) p = neg4(p);
+  if (in_zone(p, group, ops, order)) {
+    q[i] = p;
+    return;
+  }
+  float4 best = p;
+  float w = p.x;
+  for (uint32_t k = 0; k < order; ++k) {
+    const float4 cq = qmul(p, ops[k]);
+    if (fabsf(cq.x) > w) {
+      best = cq.x >= 0.0f ? cq : neg4(cq);
+      w = fabsf(cq.x);
+    }
+  }
+  q[i] = normalize4(best);
+}
+
+void launch_fold(Prof& p, hipStream_t s, float4* q, uint32_t n, const float4* ops, uint32_t order, uint32_t group) {
+  p.run("fold", [&] {
+    hipLaunchKernelGGL(k_fold, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, q, n, ops, order, group);
+  });
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ dst, uint32_t value, size_t count) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
+    dst[i] = value;
+}
+void launch_fill_u32(Prof& p, hipStream_t s, uint32_t* dst, uint32_t value, size_t count) {
+  const uint32_t g = uint32_t(std::min<size_t>(4096, (count + kBlock - 1) / kBlock + 1));
+  p.run("fill", [&] { hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, dst, value, count); });
+}
+
+// ---- all pairs (pairs.wgsl:263-326) -------------------------------------------------
+// Partial row layout: 5 doubles (fx, fy, fz, curvature, energy) per (split, source),
+// nearest chord^2 separately.
+__global__ __launch_bounds__(256) void k_allpairs(const float4* __restrict__ q, uint32_t n,
+                                                  const float4* __restrict__ ops, uint32_t order,
+                                                  uint32_t ops_per_split, uint32_t site_chunk, uint32_t site_splits,
+                                                  double* __restrict__ partial, float* __restrict__ pnear) {
+  __shared__ float4 tile[kBlock];
+  const uint32_t split = blockIdx.y;
+  const uint32_t k0 = (split / site_splits) * ops_per_split, k1 = min(order, k0 + ops_per_split);
+  const uint32_t j0 = (split % site_splits) * site_chunk, j1 = min(n, j0 + site_chunk);
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool live = i < n;
+  const float4 x = live ? q[i] : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  float nearest = 4.0f;
+  for (uint32_t k = k0; k < k1; ++k) {
+    const float4 op = ops[k];
+    for (uint32_t start = j0; start < j1; start += kBlock) {
+      const uint32_t j = start + threadIdx.x;
+      if (j < j1) tile[threadIdx.x] = image(q[j], op, k);
+      __syncthreads();
+      if (live) {
+        const uint32_t cnt = min(kBlock, j1 - start);
+        for (uint32_t t0 = 0; t0 < cnt; t0 += 64) {
+          PairSum ps{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+          const uint32_t t1 = min(cnt, t0 + 64);
+          for (uint32_t t = t0; t < t1; ++t) {
+            const float4 y = tile[t];
+            const float dm2 = chord2(x, y), dp2 = chord2p(x, y);
+            const uint32_t other = start + t;
+            if (other != i || k != 0) nearest = fminf(nearest, fminf(dm2, dp2));
+            if (other != i) pair_terms<false>(x, y, dm2, dp2, 0.0f, 0.0f, ps);
+          }
+          acc[0] += ps.fx;
+          acc[1] += ps.fy;
+          acc[2] += ps.fz;
+          acc[3] += ps.curvature;
+          acc[4] += ps.energy;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (live) {
+    const size_t o = size_t(split) * n + i;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) partial[5 * o + c] = acc[c];
+    pnear[o] = nearest;
+  }
+}
+
+__global__ void k_allpairs_reduce(const double* __restrict__ partial, const float* __restrict__ pnear, uint32_t n,
+                                  uint32_t splits, Rows out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  float nearest = 4.0f;
+  for (uint32_t s = 0; s < splits; ++s) {
+    const size_t o = size_t(s) * n + i;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) a[c] += partial[5 * o + c];
+    nearest = fminf(nearest, pnear[o]);
+  }
+  out.force[i] = make_float4(float(a[0]), float(a[1]), float(a[2]), float(a[3]));
+  out.energy[i] = a[4];
+  out.nearest[i] = nearest;
+}
+
+AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale) {
+  AllPairsPlan p;
+  const uint32_t sb = blocks_for(n, kBlock);
+  // Enough workgroups to fill 256 CUs several times; at most ~4096 splits of rows.
+  const uint32_t target = 2048;
+  p.ops_per_split = 1;
+  p.op_splits = order;
+  if (uint64_t(sb) * order > 4 * target) {
+    p.ops_per_split = uint32_t((uint64_t(sb) * order + 4 * target - 1) / (4 * target));
+    p.op_splits = (order + p.ops_per_split - 1) / p.ops_per_split;
+  }
+  uint32_t chunks = 1;
+  if (uint64_t(sb) * p.op_splits < target) chunks = uint32_t((target + sb * p.op_splits - 1) / (sb * p.op_splits));
+  // A smaller budget (zo_pair_budget) splits the sites finer: only the rounding changes.
+  if (budget_scale > 0.0 && budget_scale < 1.0) chunks = uint32_t(std::min<double>(1e6, chunks / budget_scale));
+  chunks = std::max(1u, std::min(chunks, blocks_for(n, kBlock)));
+  p.site_chunk = blocks_for(n, chunks * kBlock) * kBlock;
+  p.site_splits = blocks_for(n, p.site_chunk);
+  if (budget_scale > 0.0 && budget_scale < 1.0) {
+    p.ops_per_split = 1;
+    p.op_splits = order;
+  }
+  return p;
+}
+
+void launch_allpairs(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                     const AllPairsPlan& plan, double* partial, float* partial_nearest, Rows out) {
+  const uint32_t splits = plan.splits();
+  p.run("allpairs", [&] {
+    hipLaunchKernelGGL(k_allpairs, dim3(blocks_for(n, kBlock), splits), dim3(kBlock), 0, s, q, n, ops, order,
+                       plan.ops_per_split, plan.site_chunk, plan.site_splits, partial, partial_nearest);
+  });
+  p.run("allpairs_reduce", [&] {
+    hipLaunchKernelGGL(k_allpairs_reduce, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, partial, partial_nearest,
+                       n, splits, out);
+  });
+}
+
+// Nearest orbit point over all pairs with its rank j*2*order + 2*k + sign; the minimum
+// chord^2, ties to the lowest rank.  blockIdx.y is the operator.
+__global__ __launch_bounds__(256) void k_allpairs_nearest(const float4* __restrict__ q, uint32_t n,
+                                                          const float4* __restrict__ ops, uint32_t order,
+                                                          uint64_t* __restrict__ partial) {
+  __shared__ float4 tile[kBlock];
+  const uint32_t k = blockIdx.y;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool live = i < n;
+  const float4 x = live ? q[i] : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+  const float4 op = ops[k];
+  uint64_t best = ~uint64_t(0);
+  for (uint32_t start = 0; start < n; start += kBlock) {
+    const uint32_t j = start + threadIdx.x;
+    if (j < n) tile[threadIdx.x] = image(q[j], op, k);
+    __syncthreads();
+    if (live) {
+      const uint32_t cnt = min(kBlock, n - start);
+      for (uint32_t t = 0; t < cnt; ++t) {
+        const float4 y = tile[t];
+        const uint32_t jj = start + t;
+        const uint32_t r = jj * 2 * order + 2 * k;
+        const float dm2 = chord2(x, y), dp2 = chord2p(x, y);
+        const uint64_t km = (uint64_t(__float_as_uint(dm2)) << 32) | r;
+        const uint64_t kp = (uint64_t(__float_as_uint(dp2)) << 32) | (r + 1);
+        if (!(jj == i && k == 0) && km < best) best = km;
+        if (kp < best) best = kp;
+      }
+    }
+    __syncthreads();
+  }
+  if (live) partial[size_t(k) * n + i] = best;
+}
+
+__global__ void k_nearest_reduce(const uint64_t* __restrict__ partial, uint32_t n, uint32_t order,
+                                 float* __restrict__ chord_out, uint32_t* __restrict__ rank_out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t best = ~uint64_t(0);
+  for (uint32_t k = 0; k < order; ++k) {
+    const uint64_t v = partial[size_t(k) * n + i];
+    best = v < best ? v : best;
+  }
+  chord_out[i] = best == ~uint64_t(0) ? 4.0f : __uint_as_float(uint32_t(best >> 32));
+  rank_out[i] = best == ~uint64_t(0) ? 0xffffffffu : uint32_t(best);
+}
+
+void launch_allpairs_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                             uint64_t* partial, float* chord_out, uint32_t* rank_out) {
+  p.run("allpairs_nearest", [&] {
+    hipLaunchKernelGGL(k_allpairs_nearest, dim3(blocks_for(n, kBlock), order), dim3(kBlock), 0, s, q, n, ops, order,
+                       partial);
+  });
+  p.run("nearest_reduce", [&] {
+    hipLaunchKernelGGL(k_nearest_reduce, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, partial, n, order,
+                       chord_out, rank_out);
+  });
+}
+
+// ---- scan (exclusive, in place; data[count] receives the total) -----------------------
+constexpr uint32_t kScanItems = 8;
+constexpr uint32_t kScanTile = kBlock * kScanItems;
+size_t scan_tmp_words(size_t count) { return (count + kScanTile - 1) / kScanTile + 1; }
+
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& total) {
+  const uint32_t t = threadIdx.x;
+  lds[t] = v;
+  __syncthreads();
+  for (uint32_t d = 1; d < kBlock; d <<= 1) {
+    uint32_t o = t >= d ? lds[t - d] : 0u;
+    __syncthreads();
+    lds[t] += o;
+    __syncthreads();
+  }
+  total = lds[kBlock - 1];
+  const uint32_t r = lds[t] - v;
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* __restrict__ data, size_t count,
+                                                    uint32_t* __restrict__ tile_sums) {
+  __shared__ uint32_t lds[kBlock];
+  const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < count ? data[base + k] : 0u;
+    const uint32_t c = v[k];
+    v[k] = sum;
+    sum += c;
+  }
+  uint32_t total;
+  const uint32_t off = block_excl_scan(sum, lds, total);
+#pragma unroll
+  for (uint32_t k = 0; k < kScanItems; ++k)
+    if (base + k < count) data[base + k] = v[k] + off;
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_scan_sums(uint32_t* __restrict__ tile_sums, uint32_t ntiles,
+                                                   uint32_t* __restrict__ total_out) {
+  __shared__ uint32_t lds[kBlock];
+  const uint32_t per = (ntiles + kBlock - 1) / kBlock;
+  const uint32_t b0 = threadIdx.x * per, b1 = min(ntiles, b0 + per);
+  uint32_t sum = 0;
+  for (uint32_t b = b0; b < b1; ++b) sum += tile_sums[b];
+  uint32_t total;
+  uint32_t at = block_excl_scan(sum, lds, total);
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t c = tile_sums[b];
+    tile_sums[b] = at;
+    at += c;
+  }
+  if (threadIdx.x == 0) *total_out = total;
+}
+
+__global__ void k_scan_add(uint32_t* __restrict__ data, size_t count, const uint32_t* __restrict__ tile_sums) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
+    data[i] += tile_sums[i / kScanTile];
+}
+
+// Exclusive scan of data[0..count) in place, data[count] = total.  tmp: scan_tmp_words(count).
+static void launch_scan(Prof& p, hipStream_t s, uint32_t* data, size_t count, uint32_t* tmp) {
+  const uint32_t ntiles = uint32_t((count + kScanTile - 1) / kScanTile);
+  p.run("scan", [&] {
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, data, count, tmp);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, tmp, ntiles, data + count);
+    const uint32_t g = uint32_t(std::min<size_t>(8192, (count + kBlock - 1) / kBlock + 1));
+    hipLaunchKernelGGL(k_scan_add, dim3(g), dim3(kBlock), 0, s, data, count, tmp);
+  });
+}
+
+// ---- near-field sites (shell.wgsl:19-130, near.wgsl:22-82) -----------------------------
+__global__ void k_near_reset(NearInfo* info, uint32_t n) {
+  info->sites = n;
+  info->overflow = 0;
+  info->rim = 0;
+  for (int c = 0; c < 3; ++c) {
+    info->lo[c] = 0xffffffffu;
+    info->hi[c] = 0u;
+  }
+}
+
+__device__ inline void box_reduce_commit(NearInfo* info, uint32_t lo[3], uint32_t hi[3]) {
+  __shared__ uint32_t slo[3][kBlock / 64], shi[3][kBlock / 64];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    uint32_t a = lo[c], b = hi[c];
+    for (int d = 32; d > 0; d >>= 1) {
+      a = min(a, __shfl_xor(a, d));
+      b = max(b, __shfl_xor(b, d));
+    }
+    if (lane == 0) {
+      slo[c][wave] = a;
+      shi[c][wave] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int c = threadIdx.x;
+    uint32_t a = 0xffffffffu, b = 0u;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+      a = min(a, slo[c][w]);
+      b = max(b, shi[c][w]);
+    }
+    if (a != 0xffffffffu) atomicMin(&info->lo[c], a);
+    if (b != 0u) atomicMax(&info->hi[c], b);
+  }
+}
+
+__device__ inline void box_add(uint32_t lo[3], uint32_t hi[3], const float4 y) {
+  const float3 p = chart(y);
+  const uint32_t k0 = ordered_key(p.x), k1 = ordered_key(p.y), k2 = ordered_key(p.z);
+  lo[0] = min(lo[0], k0);
+  hi[0] = max(hi[0], k0);
+  lo[1] = min(lo[1], k1);
+  hi[1] = max(hi[1], k1);
+  lo[2] = min(lo[2], k2);
+  hi[2] = max(hi[2], k2);
+}
+
+// Core sites (every source's identity image, at its own index) and the rim list: the
+// sources within the band of some zone plane (shell.wgsl:68-96).
+__global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, uint32_t n,
+                                                  const float4* __restrict__ planes, uint32_t nplanes, float band,
+                                                  uint32_t two_order, NearBuffers b) {
+  uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t rounds = (n + stride - 1) / stride;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    bool rim = false;
+    if (i < n) {
+      const float4 x = q[i];
+      b.uq[i] = x;
+      b.ucode[i] = i | 0x80000000u;
+      b.urank[i] = i * two_order;
+      box_add(lo, hi, x);
+      for (uint32_t k = 0; k < nplanes; ++k) {
+        if (dot4(x, planes[k]) <= band + 2e-6f) {
+          rim = true;
+          break;
+        }
+      }
+    }
+    const uint32_t slot = wave_append(&b.info->rim, rim);
+    if (rim) b.rim[slot] = i;
+  }
+  box_reduce_commit(b.info, lo, hi);
+}
+
+// Images (k, sign) != (identity, +) of the rim sources within the band of the zone's
+// closed halfspaces, appended as shell sites (shell.wgsl:19-39, :97-130).  The append
+// order varies; the cell sort's in-cell key order makes the result deterministic.
+__global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uint32_t n,
+                                               const float4* __restrict__ ops, uint32_t order,
+                                               const float4* __restrict__ planes, uint32_t nplanes, float band,
+                                               NearBuffers b) {
+  uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+  const uint32_t m = b.info->rim;
+  const uint32_t per = 2 * order - 1;
+  const uint64_t total = uint64_t(m) * per;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  const uint64_t rounds = (total + stride - 1) / stride;
+  for (uint64_t r = 0; r < rounds; ++r) {
+    const uint64_t t = r * stride + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool keep = false;
+    float4 y = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t src = 0, img = 0;
+    if (t < total) {
+      src = b.rim[t / per];
+      img = uint32_t(t % per) + 1u;
+      const uint32_t k = img >> 1;
+      y = image(q[src], ops[k], k);
+      if (img & 1u) y = neg4(y);
+      keep = y.x > -0.9999999f;
+      for (uint32_t pk = 0; keep && pk < nplanes; ++pk)
+        if (dot4(y, planes[pk]) < -band - 2e-6f) keep = false;
+    }
+    const uint32_t slot = wave_append(&b.info->sites, keep);
+    if (keep) {
+      if (slot < b.capacity) {
+        b.uq[slot] = y;
+        b.ucode[slot] = src;
+        b.urank[slot] = src * (2 * order) + img;
+        box_add(lo, hi, y);
+      } else {
+        b.info->overflow = 1u;
+      }
+    }
+  }
+  box_reduce_commit(b.info, lo, hi);
+}
+
+__global__ void k_box_finalize(NearInfo* info, uint32_t sdim) {
+  float lo[3], hi[3];
+  float width = 1e-6f;
+  for (int c = 0; c < 3; ++c) {
+    lo[c] = unordered_key(info->lo[c]);
+    hi[c] = unordered_key(info->hi[c]);
+    width = fmaxf(width, hi[c] - lo[c]);
+  }
+  width *= 1.0001f;
+  for (int c = 0; c < 3; ++c) info->lower[c] = lo[c] - 1e-5f * width;
+  info->cpu = float(sdim) / (width * 1.0001f);
+}
+
+__device__ inline uint32_t cell_axis(float v, float lower, float cpu, uint32_t dim) {
+  const float c = floorf((v - lower) * cpu);
+  return uint32_t(fminf(fmaxf(c, 0.0f), float(dim - 1)));
+}
+
+__global__ void k_site_count(NearBuffers b) {
+  const uint32_t S = min(b.info->sites, b.capacity);
+  const float lx = b.info->lower[0], ly = b.info->lower[1], lz = b.info->lower[2], cpu = b.info->cpu;
+  const uint32_t d = b.sdim;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
+    const float3 p = chart(b.uq[s]);
+    const uint32_t c = cell_axis(p.x, lx, cpu, d) + d * (cell_axis(p.y, ly, cpu, d) + d * cell_axis(p.z, lz, cpu, d));
+    b.ucell[s] = c;
+    atomicAdd(&b.offsets[c], 1u);
+  }
+}
+
+__global__ void k_copy_u32(const uint32_t* __restrict__ a, uint32_t* __restrict__ b, size_t count) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
+    b[i] = a[i];
+}
+
+__global__ void k_site_scatter(NearBuffers b) {
+  const uint32_t S = min(b.info->sites, b.capacity);
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x)
+    b.perm[atomicAdd(&b.cursor[b.ucell[s]], 1u)] = s;
+}
+
+// In-cell order by a unique key (the site's rank, or the item itself), so sums run in
+// the same order every pass (near.wgsl:70-82).  Crowded cells beyond 512 keep the
+// scatter's order.
+__global__ void k_cell_sort(const uint32_t* __restrict__ offsets, uint32_t cells, uint32_t* __restrict__ perm,
+                            const uint32_t* __restrict__ key) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  const uint32_t begin = offsets[c], end = offsets[c + 1];
+  if (end - begin < 2 || end - begin > 512) return;
+  for (uint32_t at = begin + 1; at < end; ++at) {
+    const uint32_t v = perm[at];
+    const uint32_t kv = key ? key[v] : v;
+    uint32_t before = at;
+    while (before > begin) {
+      const uint32_t u = perm[before - 1];
+      if ((key ? key[u] : u) <= kv) break;
+      perm[before] = u;
+      --before;
+    }
+    perm[before] = v;
+  }
+}
+
+__global__ void k_site_gather(NearBuffers b) {
+  const uint32_t S = min(b.info->sites, b.capacity);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x) {
+    const uint32_t s = b.perm[k];
+    b.sq[k] = b.uq[s];
+    b.scode[k] = b.ucode[s];
+    b.srank[k] = b.urank[s];
+  }
+}
+
+__device__ inline uint32_t spread6(uint32_t v) {
+  uint32_t x = v & 0x3fu;
+  x = (x | (x << 8)) & 0x0000f00fu;
+  x = (x | (x << 4)) & 0x000c30c3u;
+  x = (x | (x << 2)) & 0x00249249u;
+  return x;
+}
+
+__global__ void k_src_count(const float4* __restrict__ q, uint32_t n, NearBuffers b) {
+  const float lx = b.info->lower[0], ly = b.info->lower[1], lz = b.info->lower[2];
+  const float cpu = b.info->cpu * float(1u << kMortonBits) / float(b.sdim);
+  const uint32_t d = 1u << kMortonBits;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float3 p = chart(q[i]);
+    const uint32_t c = spread6(cell_axis(p.x, lx, cpu, d)) | (spread6(cell_axis(p.y, ly, cpu, d)) << 1) |
+                       (spread6(cell_axis(p.z, lz, cpu, d)) << 2);
+    b.src_cell[i] = c;
+    atomicAdd(&b.src_offsets[c], 1u);
+  }
+}
+
+__global__ void k_src_scatter(uint32_t n, NearBuffers b) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    b.order[atomicAdd(&b.src_cursor[b.src_cell[i]], 1u)] = i;
+}
+
+static uint32_t stride_grid(size_t work) { return uint32_t(std::min<size_t>(8192, (work + kBlock - 1) / kBlock + 1)); }
+
+void launch_near_sites(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                       const NearBuffers& b) {
+  p.run("near_reset", [&] { hipLaunchKernelGGL(k_near_reset, dim3(1), dim3(1), 0, s, b.info, n); });
+  p.run("core_rim", [&] {
+    hipLaunchKernelGGL(k_core_rim, dim3(std::min<uint32_t>(2048, blocks_for(n, kBlock))), dim3(kBlock), 0, s, q, n,
+                       b.planes, b.nplanes, b.band, 2 * order, b);
+  });
+  p.run("shell", [&] {
+    hipLaunchKernelGGL(k_shell, dim3(2048), dim3(kBlock), 0, s, q, n, ops, order, b.planes, b.nplanes, b.band, b);
+  });
+  p.run("box", [&] { hipLaunchKernelGGL(k_box_finalize, dim3(1), dim3(1), 0, s, b.info, b.sdim); });
+}
+
+void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                       const NearBuffers& b) {
+  const size_t cells = size_t(b.sdim) * b.sdim * b.sdim;
+  launch_fill_u32(p, s, b.offsets, 0u, cells + 1);
+  launch_fill_u32(p, s, b.src_offsets, 0u, kMortonCells + 1);
+  launch_near_sites(p, s, q, n, ops, order, b);
+  const uint32_t g = stride_grid(b.capacity);
+  p.run("site_count", [&] { hipLaunchKernelGGL(k_site_count, dim3(g), dim3(kBlock), 0, s, b); });
+  launch_scan(p, s, b.offsets, cells, b.scan_tmp);
+  p.run("copy", [&] {
+    hipLaunchKernelGGL(k_copy_u32, dim3(stride_grid(cells)), dim3(kBlock), 0, s, b.offsets, b.cursor, cells);
+  });
+  p.run("site_scatter", [&] { hipLaunchKernelGGL(k_site_scatter, dim3(g), dim3(kBlock), 0, s, b); });
+  p.run("site_sort", [&] {
+    hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(cells, kBlock)), dim3(kBlock), 0, s, b.offsets, uint32_t(cells),
+                       b.perm, b.urank);
+  });
+  p.run("site_gather", [&] { hipLaunchKernelGGL(k_site_gather, dim3(g), dim3(kBlock), 0, s, b); });
+  const uint32_t gn = stride_grid(n);
+  p.run("src_count", [&] { hipLaunchKernelGGL(k_src_count, dim3(gn), dim3(kBlock), 0, s, q, n, b); });
+  launch_scan(p, s, b.src_offsets, kMortonCells, b.scan_tmp);
+  p.run("copy", [&] {
+    hipLaunchKernelGGL(k_copy_u32, dim3(stride_grid(kMortonCells)), dim3(kBlock), 0, s, b.src_offsets, b.src_cursor,
+                       size_t(kMortonCells));
+  });
+  p.run("src_scatter", [&] { hipLaunchKernelGGL(k_src_scatter, dim3(gn), dim3(kBlock), 0, s, n, b); });
+  p.run("src_sort", [&] {
+    hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(kMortonCells, kBlock)), dim3(kBlock), 0, s, b.src_offsets,
+                       kMortonCells, b.order, nullptr);
+  });
+}
+
+// ---- near-field rows (pairs.wgsl:363-427) -------------------------------------------
+struct WaveLds {
+  float4 qy[128];
+  uint32_t qc[128];
+  uint32_t rstart[64];
+  uint32_t rexcl[64];
+};
+
+__device__ inline float row_gap(float c, int idx, uint32_t dim) {
+  // Distance (cell units) from c to cell interval [idx, idx+1]; edge cells are unbounded
+  // outward, so sites clamped into them are never pruned.
+  const float lo = idx == 0 ? -3e38f : float(idx);
+  const float hi = idx == int(dim) - 1 ? 3e38f : float(idx + 1);
+  return fmaxf(0.0f, fmaxf(lo - c, c - hi));
+}
+
+// MODE 0: force rows.  MODE 1: nearest orbit point (chord^2, rank), the identity image
+// of the source excluded.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint32_t n,
+                                              const uint32_t* __restrict__ order, const float4* __restrict__ sq,
+                                              const uint32_t* __restrict__ scode, const uint32_t* __restrict__ srank,
+                                              const uint32_t* __restrict__ offsets, const NearInfo* __restrict__ info,
+                                              uint32_t sdim, NearCut cut, uint32_t two_order, uint32_t real_blocks,
+                                              Rows rows, float* __restrict__ chord_out,
+                                              uint32_t* __restrict__ rank_out) {
+  __shared__ WaveLds lds[kBlock / 64];
+  const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
+  if (vb >= real_blocks) return;
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t slot = vb * (kBlock / 64) + wave;
+  if (slot >= n) return;  // the whole wave leaves; no workgroup barrier follows
+  WaveLds& L = lds[wave];
+  const uint32_t src = order[slot];
+  const float4 x = q[src];
+  const float cpu = info->cpu;
+  const float fdim = float(sdim);
+  const float denom = cut.c0 + x.x;
+  const bool full = !(denom > 1e-3f);
+  float cx = 0.5f * fdim, cy = 0.5f * fdim, cz = 0.5f * fdim, rc = 3e18f;
+  if (!full) {
+    const float inv = 1.0f / denom;
+    cx = (x.y * inv - info->lower[0]) * cpu;
+    cy = (x.z * inv - info->lower[1]) * cpu;
+    cz = (x.w * inv - info->lower[2]) * cpu;
+    rc = (cut.s0 * inv * 1.001f + 1e-6f) * cpu + 1e-3f;
+  }
+  auto clampc = [&](float v) -> int { return int(fminf(fmaxf(floorf(v), 0.0f), fdim - 1.0f)); };
+  const int zlo = clampc(cz - rc), zhi = clampc(cz + rc), ylo = clampc(cy - rc), yhi = clampc(cy + rc);
+  const uint32_t ny = uint32_t(yhi - ylo + 1), nrows = ny * uint32_t(zhi - zlo + 1);
+  const float rc2 = rc * rc;
+  const uint32_t self_code = src | 0x80000000u, self_rank = src * two_order;
+
+  PairSum ps{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  float nearest = 4.0f;
+  uint64_t best = ~uint64_t(0);
+  uint32_t qhead = 0, qcount = 0, rounds = 0;
+
+  auto process = [&](uint32_t e) {
+    const float4 y = L.qy[e];
+    const uint32_t code = L.qc[e];
+    const float dm2 = chord2(x, y), dp2 = chord2p(x, y);
+    if (code != self_code) nearest = fminf(nearest, dm2);
+    if ((code & 0x7fffffffu) != src) pair_terms<true>(x, y, dm2, dp2, cut.taper_start, cut.inv_width, ps);
+  };
+  auto flush = [&]() {
+    acc[0] += ps.fx;
+    acc[1] += ps.fy;
+    acc[2] += ps.fz;
+    acc[3] += ps.curvature;
+    acc[4] += ps.energy;
+    ps = PairSum{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  };
+
+  for (uint32_t rb = 0; rb < nrows; rb += 64) {
+    const uint32_t r = rb + lane;
+    uint32_t start = 0, len = 0;
+    if (r < nrows) {
+      const int yy = ylo + int(r % ny), zz = zlo + int(r / ny);
+      const float gy = row_gap(cy, yy, sdim), gz = row_gap(cz, zz, sdim);
+      const float rest = rc2 - gy * gy - gz * gz;
+      if (rest >= 0.0f) {
+        const float span = full ? 3e18f : sqrtf(rest);
+        const int x0 = clampc(cx - span), x1 = clampc(cx + span);
+        const uint32_t row = sdim * (uint32_t(yy) + sdim * uint32_t(zz));
+        start = offsets[row + x0];
+        len = offsets[row + x1 + 1] - start;
+      }
+    }
+    const uint32_t incl = wave_incl_scan(len);
+    const uint32_t total = __shfl(incl, 63);
+    wave_sync();
+    L.rstart[lane] = start;
+    L.rexcl[lane] = incl - len;
+    wave_sync();
+    for (uint32_t cb = 0; cb < total; cb += 64) {
+      const uint32_t v = cb + lane;
+      const bool valid = v < total;
+      uint32_t site = 0;
+      if (valid) {
+        uint32_t rr = 0;
+#pragma unroll
+        for (uint32_t step = 32; step > 0; step >>= 1)
+          if (L.rexcl[rr + step] <= v) rr += step;
+        site = L.rstart[rr] + (v - L.rexcl[rr]);
+      }
+      if (MODE == 0) {
+        float4 y = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t code = 0;
+        bool accept = false;
+        if (valid) {
+          y = sq[site];
+          code = scode[site];
+          accept = chord2(x, y) <= cut.chord2;
+        }
+        const uint64_t mask = __ballot(accept);
+        if (mask) {
+          if (accept) {
+            const uint32_t e = (qhead + qcount + mbcnt(mask)) & 127u;
+            L.qy[e] = y;
+            L.qc[e] = code;
+          }
+          qcount += uint32_t(__popcll(mask));
+          wave_sync();
+          if (qcount >= 64) {
+            process((qhead + lane) & 127u);
+            qhead = (qhead + 64) & 127u;
+            qcount -= 64;
+            if ((++rounds & 15u) == 0) flush();
+            wave_sync();
+          }
+        }
+      } else {
+        if (valid) {
+          const float4 y = sq[site];
+          const uint32_t rank = srank[site];
+          const float dm2 = chord2(x, y);
+          if (dm2 <= cut.chord2 && rank != self_rank) {
+            const uint64_t key = (uint64_t(__float_as_uint(dm2)) << 32) | rank;
+            best = key < best ? key : best;
+          }
+        }
+      }
+    }
+  }
+  if (MODE == 0) {
+    if (lane < qcount) process((qhead + lane) & 127u);
+    flush();
+    double f0 = wave_sum(acc[0]), f1 = wave_sum(acc[1]), f2 = wave_sum(acc[2]), f3 = wave_sum(acc[3]);
+    double e = wave_sum(acc[4]);
+    nearest = wave_min(nearest);
+    if (lane == 0) {
+      rows.force[src] = make_float4(float(f0), float(f1), float(f2), float(f3));
+      rows.energy[src] = e;
+      rows.nearest[src] = nearest;
+    }
+  } else {
+    best = wave_min_u64(best);
+    if (lane == 0) {
+      chord_out[src] = best == ~uint64_t(0) ? 4.0f : __uint_as_float(uint32_t(best >> 32));
+      rank_out[src] = best == ~uint64_t(0) ? 0xffffffffu : uint32_t(best);
+    }
+  }
+}
+
+void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, const NearBuffers& b, const NearCut& cut,
+                       Rows out) {
+  const uint32_t real = blocks_for(n, kBlock / 64);
+  p.run("near_force", [&] {
+    hipLaunchKernelGGL(k_near<0>, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.srank,
+                       b.offsets, b.info, b.sdim, cut, 0u, real, out, nullptr, nullptr);
+  });
+}
+
+void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, uint32_t order, const NearBuffers& b,
+                         const NearCut& cut, float* chord_out, uint32_t* rank_out) {
+  const uint32_t real = blocks_for(n, kBlock / 64);
+  p.run("near_nearest", [&] {
+    hipLaunchKernelGGL(k_near<1>, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.srank,
+                       b.offsets, b.info, b.sdim, cut, 2 * order, real, Rows{}, chord_out, rank_out);
+  });
+}
+
+// ---- BB step (relax.wgsl:29-165) -----------------------------------------------------
+uint32_t bb_partials(uint32_t n) { return blocks_for(n, kBlock); }
+
+__global__ void k_bb_reset(BBState* st) {
+  const uint32_t t = threadIdx.x;
+  if (t == 0) {
+    st->iteration = 0;
+    st->flags = 0;
+    st->step_bits = 0;
+    st->round = 0;
+    st->sy = st->yy = st->ss = st->ff = st->energy = 0.0;
+    st->rank[0] = st->rank[1] = 0;
+    st->prefix[0] = st->prefix[1] = 0;
+    st->median = 0.0f;
+    st->alpha = 0.0f;
+    st->alpha64 = 0.0;
+  }
+  for (uint32_t k = t; k < 512; k += blockDim.x) st->hist[k] = 0;
+}
+void launch_bb_reset(Prof& p, hipStream_t s, BBState* state) {
+  p.run("bb_reset", [&] { hipLaunchKernelGGL(k_bb_reset, dim3(1), dim3(kBlock), 0, s, state); });
+}
+
+__device__ inline void block_sum5(double v[5], double* out) {
+  __shared__ double sh[5][kBlock / 64];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    const double s = wave_sum(v[c]);
+    if (lane == 0) sh[c][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    double s = 0.0;
+    for (uint32_t w = 0; w < kBlock / 64; ++w) s += sh[threadIdx.x][w];
+    out[threadIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bb_stats(Rows rows, const float4* __restrict__ prev, uint32_t n,
+                                                  uint32_t fixed, BBState* __restrict__ st,
+                                                  double* __restrict__ partials) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool first = st->iteration == 0;
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  uint32_t fl = 0;
+  if (i < n) {
+    const float4 f = rows.force[i];
+    const bool finite = (f.x == f.x) && (f.y == f.y) && (f.z == f.z) && fabsf(f.x) <= 3.4e38f &&
+                        fabsf(f.y) <= 3.4e38f && fabsf(f.z) <= 3.4e38f;
+    if (!finite) fl |= 1u;
+    if (!(rows.nearest[i] > 0.0f)) fl |= 2u;
+    const double fx = f.x, fy = f.y, fz = f.z;
+    v[3] = fx * fx + fy * fy + fz * fz;
+    v[4] = rows.energy[i];
+    if (!first && i + 1 != fixed) {
+      const float4 pf = prev[2 * size_t(i)], pd = prev[2 * size_t(i) + 1];
+      const double sx = pd.x, sy_ = pd.y, sz = pd.z;
+      const double yx = fx - double(pf.x), yy_ = fy - double(pf.y), yz = fz - double(pf.z);
+      v[0] = sx * yx + sy_ * yy_ + sz * yz;
+      v[1] = yx * yx + yy_ * yy_ + yz * yz;
+      v[2] = sx * sx + sy_ * sy_ + sz * sz;
+    }
+  }
+  if (fl) atomicOr(&st->flags, fl);
+  block_sum5(v, partials + 5 * size_t(blockIdx.x));
+}
+
+// One workgroup: partial sums in a fixed order, then the median's rank bookkeeping.
+__global__ __launch_bounds__(256) void k_bb_sums(const double* __restrict__ partials, uint32_t nparts, uint32_t n,
+                                                 BBState* __restrict__ st) {
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (uint32_t b = threadIdx.x; b < nparts; b += kBlock)
+    for (int c = 0; c < 5; ++c) v[c] += partials[5 * size_t(b) + c];
+  __shared__ double out[5];
+  block_sum5(v, out);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st->sy = out[0];
+    st->yy = out[1];
+    st->ss = out[2];
+    st->ff = out[3];
+    st->energy = out[4];
+    st->rank[0] = (n - 1) / 2;
+    st->rank[1] = n / 2;
+    st->prefix[0] = st->prefix[1] = 0;
+    st->step_bits = 0;
+  }
+  for (uint32_t k = threadIdx.x; k < 512; k += kBlock) st->hist[k] = 0;
+}
+
+__device__ inline uint32_t curvature_bits(const float4 f) { return __float_as_uint(f.w > 0.0f ? f.w : 0.0f); }
+
+__global__ __launch_bounds__(256) void k_median_count(const float4* __restrict__ force, uint32_t n, uint32_t round,
+                                                      BBState* __restrict__ st) {
+  __shared__ uint32_t local[512];
+  local[threadIdx.x] = 0;
+  local[threadIdx.x + 256] = 0;
+  __syncthreads();
+  const uint32_t shift = 24 - 8 * round;
+  const uint32_t high = round > 0 ? (0xffffffffu << (32 - 8 * round)) : 0u;
+  const uint32_t p0 = st->prefix[0], p1 = st->prefix[1];
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const uint32_t bits = curvature_bits(force[i]);
+    const uint32_t digit = (bits >> shift) & 255u;
+    if ((bits & high) == p0) atomicAdd(&local[digit], 1u);
+    if ((bits & high) == p1) atomicAdd(&local[256 + digit], 1u);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < 512; k += kBlock)
+    if (local[k]) atomicAdd(&st->hist[k], local[k]);
+}
+
+__global__ __launch_bounds__(256) void k_median_pick(uint32_t round, double radius, BBState* __restrict__ st) {
+  if (threadIdx.x == 0) {
+    const uint32_t shift = 24 - 8 * round;
+    for (int r = 0; r < 2; ++r) {
+      uint32_t remaining = st->rank[r];
+      uint32_t digit = 255;
+      for (uint32_t d = 0; d < 256; ++d) {
+        const uint32_t c = st->hist[256 * r + d];
+        if (remaining < c) {
+          digit = d;
+          break;
+        }
+        remaining -= c;
+      }
+      st->rank[r] = remaining;
+      st->prefix[r] |= digit << shift;
+    }
+    if (round == 3) {
+      const float median =
+          fmaxf(1e-12f, 0.5f * (__uint_as_float(st->prefix[0]) + __uint_as_float(st->prefix[1])));
+      st->median = median;
+      const uint32_t it = st->iteration;
+      const double alpha0 = (0.8 / 3.4) * radius * radius * radius;
+      double alpha = alpha0;
+      if (it > 0) {
+        const double a1 = fabs(st->yy) > 1e-20 ? fabs(st->sy / st->yy) : alpha0;
+        const double a2 = fabs(st->sy) > 1e-20 ? fabs(st->ss / st->sy) : alpha0;
+        alpha = fmin(fmax(it % 2 == 1 ? a1 : a2, 1e-12), fmax(1.0, 10.0 * radius));
+      }
+      st->alpha64 = alpha;
+      st->alpha = float(alpha);
+      st->iteration = it + 1;
+    }
+  }
+  __syncthreads();
+  st->hist[threadIdx.x] = 0;
+  st->hist[threadIdx.x + 256] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_bb_move(float4* __restrict__ q, const float4* __restrict__ force,
+                                                 float4* __restrict__ prev, uint32_t n, uint32_t fixed, float radius,
+                                                 float step_scale, BBState* __restrict__ st) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (st->flags != 0) return;
+  float moved_size = 0.0f;
+  if (i < n) {
+    if (i + 1 == fixed) {
+      prev[2 * size_t(i)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      prev[2 * size_t(i) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      q[i] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+      const float alpha = st->alpha, median = st->median;
+      const float4 f = force[i];
+      const float scale = alpha * step_scale / fmaxf(1e-6f, f.w / median + 1e-2f);
+      const float size = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z) * scale;
+      const float clip = fminf(1.0f, 0.5f * radius / fmaxf(size, 1e-12f));
+      const float3 d = make_float3(f.x * (scale * clip), f.y * (scale * clip), f.z * (scale * clip));
+      prev[2 * size_t(i)] = make_float4(f.x, f.y, f.z, 0.0f);
+      prev[2 * size_t(i) + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+      moved_size = size * clip;
+      const float angle = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+      const float sinc = angle > 1e-4f ? sinf(angle) / angle : 1.0f - angle * angle / 6.0f;
+      const float4 rot = make_float4(cosf(angle), d.x * sinc, d.y * sinc, d.z * sinc);
+      float4 m = normalize4(qmul(rot, q[i]));
+      if (m.x < 0.0f) m = neg4(m);
+      q[i] = m;
+    }
+  }
+  const uint32_t mx = wave_max_u32(__float_as_uint(moved_size));
+  if ((threadIdx.x & 63) == 0 && mx) atomicMax(&st->step_bits, mx);
+}
+
+void launch_bb_step(Prof& p, hipStream_t s, float4* q, uint32_t n, Rows rows, float4* prev, BBState* state,
+                    double* partials, double radius, float step_scale, uint32_t fixed) {
+  const uint32_t nb = bb_partials(n);
+  p.run("bb_stats", [&] {
+    hipLaunchKernelGGL(k_bb_stats, dim3(nb), dim3(kBlock), 0, s, rows, prev, n, fixed, state, partials);
+  });
+  p.run("bb_sums", [&] { hipLaunchKernelGGL(k_bb_sums, dim3(1), dim3(kBlock), 0, s, partials, nb, n, state); });
+  const uint32_t mg = std::max(1u, std::min(nb, 512u));
+  for (uint32_t round = 0; round < 4; ++round) {
+    p.run("median_count", [&] {
+      hipLaunchKernelGGL(k_median_count, dim3(mg), dim3(kBlock), 0, s, rows.force, n, round, state);
+    });
+    p.run("median_pick",
+          [&] { hipLaunchKernelGGL(k_median_pick, dim3(1), dim3(kBlock), 0, s, round, radius, state); });
+  }
+  p.run("bb_move", [&] {
+    hipLaunchKernelGGL(k_bb_move, dim3(nb), dim3(kBlock), 0, s, q, rows.force, prev, n, fixed, float(radius),
+                       step_scale, state);
+  });
+}
+
+}  // namespace zxt
