@@ -1,0 +1,287 @@
+"""Near-field pair-evaluation throughput of the QVM force pass on one MI355X.
+
+Workload (the configuration the metric is quoted on): group I, the first 1,000,000 of
+zo.haar("I", 10_000_000, seed 193), neighbourhood 20 (about 8,000 orbit points per
+source).  A step is one near-field energy/force pass over the whole sample: shell of
+images near the zone, cell sort of the sites, Morton sort of the sources, the pair
+kernel.  Positions are resident in HBM before the timed region.
+
+value = accepted pair evaluations per second over the timed passes (wall clock around
+zo_force_passes, which ends with a device wait).  roofline = the pair kernel against the
+VALU + transcendental roofline: the per-pair algorithmic work is counted by running the
+reference's per-pair near-field function (shaders/pairs.wgsl:246-262, :349-361, plus the
+candidate chord and the row sums) on a counting scalar type, and priced at the MI355X
+peaks (7.86e13 FP32 lane-ops/s, 9.83e12 transcendentals/s).  Expressed in TFLOP/s-
+equivalent against 157.3 TFLOP/s so that frac = pairs/s x (VALU/7.86e13 + TRANS/9.83e12).
+cpu_baseline = the CPU restatement in oracle/ (the reference engine cannot run here) on a
+bounded sample of the same workload, on all host cores.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+METRIC = "near-field pair-evals/s at N=1e6 (I) vs roofline; batched p50/p99 vs plain"
+PEAK_VALU = 7.86e13       # FP32 lane-ops/s (256 CU x 128 lanes x 2.4 GHz, packed)
+PEAK_TRANS = 9.83e12      # transcendental lane-ops/s (quarter rate)
+PEAK_FP32_TFLOPS = 157.3  # published dense FP32 vector peak
+
+
+# ---- the counting scalar type ----------------------------------------------------
+class Count:
+    valu = 0
+    trans = 0
+
+    @classmethod
+    def reset(cls):
+        cls.valu = cls.trans = 0
+
+
+class F:
+    """A float that counts the VALU and transcendental operations applied to it."""
+    __slots__ = ("v",)
+
+    def __init__(self, v):
+        self.v = float(v.v if isinstance(v, F) else v)
+
+    @staticmethod
+    def _val(o):
+        return o.v if isinstance(o, F) else float(o)
+
+    def _op(self, o, fn, n=1):
+        Count.valu += n
+        return F(fn(self.v, F._val(o)))
+
+    def __add__(self, o): return self._op(o, lambda a, b: a + b)
+    def __radd__(self, o): return self._op(o, lambda a, b: b + a)
+    def __sub__(self, o): return self._op(o, lambda a, b: a - b)
+    def __rsub__(self, o): return self._op(o, lambda a, b: b - a)
+    def __mul__(self, o): return self._op(o, lambda a, b: a * b)
+    def __rmul__(self, o): return self._op(o, lambda a, b: b * a)
+
+    def __truediv__(self, o):  # v_rcp + v_mul
+        Count.trans += 1
+        return self._op(o, lambda a, b: a / b)
+
+    def __rtruediv__(self, o):
+        Count.trans += 1
+        return self._op(o, lambda a, b: b / a)
+
+    def __neg__(self): return F(-self.v)  # a source modifier, free
+
+    def __le__(self, o):
+        Count.valu += 1
+        return self.v <= F._val(o)
+
+    def __gt__(self, o):
+        Count.valu += 1
+        return self.v > F._val(o)
+
+
+def fma(a, b, c):
+    Count.valu += 1
+    return F(F._val(a) * F._val(b) + F._val(c))
+
+
+def fmin(a, b):
+    Count.valu += 1
+    return F(min(F._val(a), F._val(b)))
+
+
+def fmax(a, b):
+    Count.valu += 1
+    return F(max(F._val(a), F._val(b)))
+
+
+def clamp01(a):  # v_med3
+    Count.valu += 1
+    return F(min(max(F._val(a), 0.0), 1.0))
+
+
+def select(cond, a, b):
+    Count.valu += 1
+    return F(F._val(a) if cond else F._val(b))
+
+
+def fsqrt(a):
+    Count.trans += 1
+    return F(math.sqrt(F._val(a)))
+
+
+def fsin(a):  # v_sin takes revolutions: one scale, one transcendental
+    Count.valu += 1
+    Count.trans += 1
+    return F(math.sin(F._val(a)))
+
+
+def fatan01(t):
+    """The build's atan on [0, 1] (t*P(t^2), degree 8): what the pair kernel executes."""
+    z = t * t
+    p = fma(z, 0.0028498897, -0.01606863)
+    for c in (0.04269152, -0.07504295, 0.10640934, -0.14203644, 0.1999262, -0.33333072, 1.0):
+        p = fma(z, p, c)
+    return t * p
+
+
+def dot4(a, b):
+    return fma(a[3], b[3], fma(a[2], b[2], fma(a[1], b[1], a[0] * b[0])))
+
+
+def canonical_near_pair(x, y, taper_start, inv_width):
+    """The reference's near-field pair (candidate chord, quey_pair, near_pair, row sums),
+    operation for operation, for counting."""
+    m = [x[i] - y[i] for i in range(4)]
+    dm2 = dot4(m, m)
+    p = [x[i] + y[i] for i in range(4)]
+    dp2 = dot4(p, p)
+    near = fmax(fmin(dm2, dp2), 1e-30)
+    far = fmax(fmax(dm2, dp2), 1e-30)
+    t = fsqrt(near / far)
+    a = fmax(2.0 * fatan01(t), 1e-12)
+    b = math.pi - a
+    a2, b2 = fma(a, a, 1e-9), fma(b, b, 1e-9)
+    a3, b3 = fma(a * a, a, 1e-9), fma(b * b, b, 1e-9)
+    sine = (2.0 * t) / fma(t, t, 1.0) + 1e-9
+    sign = select(dm2 <= dp2, 1.0, -1.0)
+    d = [fma(x[0], y[1], -(y[0] * x[1])) + fma(x[2], y[3], -(x[3] * y[2])),
+         fma(x[0], y[2], -(y[0] * x[2])) + fma(x[3], y[1], -(x[1] * y[3])),
+         fma(x[0], y[3], -(y[0] * x[3])) + fma(x[1], y[2], -(x[2] * y[1]))]
+    along = sign * (b2 - a2) / (a2 * b2 * sine)
+    curvature = (a3 + b3) / (a3 * b3)
+    energy = math.pi / (a * b)
+    force = [-along * c for c in d]
+    s = clamp01((a - taper_start) * inv_width)
+    if s > 0:
+        window = 1.0 - s * s * fma(-2.0, s, 3.0)
+        slope = (-6.0 * inv_width) * (s * (1.0 - s))
+        extra = sign * energy * slope / fmax(fsin(a), 1e-9)
+        force = [fma(window, f, extra * c) for f, c in zip(force, d)]
+        curvature = curvature * window
+        energy = energy * window
+    acc = [F(0)] * 5
+    acc = [acc[0] + force[0], acc[1] + force[1], acc[2] + force[2], acc[3] + curvature, acc[4] + energy]
+    return acc
+
+
+def pair_op_counts():
+    """(VALU, transcendental) per accepted near-field pair in the windowed band."""
+    Count.reset()
+    x = [F(v) for v in (0.9, 0.1, -0.3, 0.3)]
+    y = [F(v) for v in (0.89, 0.12, -0.31, 0.31)]
+    canonical_near_pair(x, y, F(0.0), F(1.0))  # taper from 0: the window branch runs
+    return Count.valu, Count.trans
+
+
+# ---- measurement ------------------------------------------------------------------
+def cpu_baseline(q, ops, cut, radius, sample):
+    import oracle
+    cores = oracle.threads(0)
+    oracle.near_grid(q, ops, cut, radius, 0, 64)  # warm-up
+    t0 = time.perf_counter()
+    _, stats = oracle.near_grid(q, ops, cut, radius, 0, sample)
+    dt = time.perf_counter() - t0
+    return {"value": stats["accepted"] / dt, "unit": "pair-evals/s", "cores": cores, "kind": "port",
+            "sample": f"CPU restatement of the reference algorithm (oracle/qvm_oracle.cpp orc_near_grid): "
+                      f"sites, shell and cell grid of all 1e6 sources, then the rows of the first {sample} "
+                      f"sources; {stats['accepted']:.3e} pairs in {dt:.2f} s"}
+
+
+def traffic_from_profiles():
+    path = ROOT / "profiles" / "near_force_i1e6_traffic.json"
+    if path.is_file():
+        data = json.loads(path.read_text())
+        return data.get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--group", default="I")
+    ap.add_argument("--delta", type=float, default=20.0)
+    ap.add_argument("--cpu-sample", type=int, default=int(os.environ.get("ZXT_CPU_SAMPLE", 30000)))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: the near-field pass is not sharded "
+                                   "across GPUs here (see DESIGN.md for how it would shard)"}))
+        sys.exit(2)
+
+    import zxtalori_amd as zo
+    from oracle import dense_f64 as dense
+
+    print(f"[bench] sample {args.group} N={args.n} delta={args.delta}", file=sys.stderr, flush=True)
+    with zo.haar(args.group, 10_000_000, 193) as big:
+        q = big.quaternions[:args.n].copy()
+    with zo.from_quaternions(q, args.group) as grid:
+        q = grid.quaternions
+        ops = grid.operations
+        grid.relax(0, neighbourhood=args.delta)
+        grid.force_passes(max(1, args.warmup))
+        print(f"[bench] warm-up done {grid.near_info()}", file=sys.stderr, flush=True)
+        grid.profile(True)
+        grid.profile_report()  # clears nothing; totals start with this run
+        t0 = time.perf_counter()
+        grid.force_passes(args.steps)
+        dt = time.perf_counter() - t0
+        report = grid.profile_report()
+        grid.profile(False)
+        info = grid.near_info()
+    pairs = info["pairs"]
+    ms_per_step = dt * 1e3 / args.steps
+    value = pairs / (dt / args.steps)
+    valu, trans = pair_op_counts()
+    kernel_count, kernel_ms = report.get("near_force", (0, float("nan")))
+    kernel_s = kernel_ms / max(kernel_count, 1) / 1e3
+    flop_equiv = 2.0 * (valu + trans * PEAK_VALU / PEAK_TRANS)
+    achieved = pairs * flop_equiv / kernel_s / 1e12
+    breakdown = {k: round(ms / c, 4) for k, (c, ms) in sorted(report.items(), key=lambda kv: -kv[1][1])}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Haar orientations)",
+        "config": {"workload": f"near-field force pass, group {args.group}, N={args.n} "
+                               f"(first of zo.haar('{args.group}', 1e7, 193)), neighbourhood {args.delta:g}",
+                   "group": args.group, "n": args.n, "neighbourhood": args.delta, "seed": 193},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profiles(),
+                     "kernel": "near_force", "kernel_ms": kernel_s * 1e3, "pairs_per_launch": pairs,
+                     "alg_valu_per_pair": valu, "alg_trans_per_pair": trans,
+                     "candidates_per_pair": info["candidates"] / max(pairs, 1)},
+        "breakdown_ms": breakdown,
+        "sites": info["sites"],
+        "shell": info["shell"],
+        "sdim": info["sdim"],
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cut, radius = dense.near_cut(len(q), len(ops), args.delta)
+        out["cpu_baseline"] = cpu_baseline(q, ops, cut, radius, args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -93,6 +93,13 @@ const uint32_t* zo_point_indices(zxt_engine* e);
 /* Runs `passes` force passes of the current neighbourhood on the current positions and
  * waits; nothing is read back (throughput measurement). */
 int32_t zo_force_passes(zxt_engine* e, uint32_t passes);
+/* One force pass of the current neighbourhood on the current positions, its rows copied
+ * out: per point (fx, fy, fz, curvature) in force4, the energy sum and the nearest
+ * chord^2 (the reference's Row columns force and extra.y/extra.z, pairs.zig:11-15). */
+int32_t zo_force_rows(zxt_engine* e, float* force4, double* energy, float* nearest);
+/* The last near-field force pass's accepted pair evaluations (out2[0]: source and orbit
+ * point within the cutoff, not of the source's own orbit) and sites tested (out2[1]). */
+int32_t zo_near_pairs(zxt_engine* e, uint64_t* out2);
 /* The last near-field pass's site count, shell count and cell grid dimension. */
 int32_t zo_near_info(zxt_engine* e, uint32_t* out3);
 

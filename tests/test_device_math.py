@@ -1,0 +1,156 @@
+"""CPU pins of the reformulated pair arithmetic the gfx950 kernels use
+(zxtalori_amd/csrc/qvm_device.h pair_terms), against the reference's quey_pair and
+near_pair formulas (shaders/pairs.wgsl:246-262, :349-361) evaluated in FP64.
+
+The kernel replaces sqrt, atan and four divisions by one rsq and one rcp:
+t = near*rsq(near*far), sin a = t*far/2 (near+far = 4 for unit quaternions),
+1/sin a = (near+far)/(2 sqrt(near far)), atan(t) = t*P(t^2), and the reciprocals of
+a2*b2*sine, a3*b3, a*b from one reciprocal of their product.  Evaluated here in FP64
+with the kernel's coefficients, the rewrite must agree with the reference form to the
+polynomial's bound; the kernel's FP32 rounding is checked on the GPU against the oracle.
+"""
+import math
+import re
+
+import numpy as np
+
+from tests.conftest import ROOT
+
+SRC = (ROOT / "zxtalori_amd" / "csrc" / "qvm_device.h").read_text()
+
+
+def poly_coefficients():
+    body = SRC[SRC.index("atan01"):SRC.index("return t * p;")]
+    return [float(x) for x in re.findall(r"(-?\d+\.\d+)f", body)]
+
+
+def atan_poly(t):
+    c = poly_coefficients()  # c[0] is the leading coefficient, then fma steps in order
+    z = t * t
+    p = c[0] * z + c[1]
+    for k in c[2:]:
+        p = p * z + k
+    return t * p
+
+
+def test_atan_polynomial_bound():
+    t = np.linspace(0, 1, 200001)
+    approx = atan_poly(np.float32(t).astype(np.float64))
+    exact = np.arctan(np.float32(t).astype(np.float64))
+    rel = np.abs(approx - exact) / np.maximum(exact, 1e-30)
+    assert rel[1:].max() < 6e-8  # Remez bound 1.5e-8 plus the f32 rounding of the coefficients
+    assert approx[0] == 0.0
+
+
+def reference_pair(x, y, dm2, dp2, taper_start=None, inv_width=None, atan=math.atan, exact_sin=False):
+    near = max(min(dm2, dp2), 1e-30)
+    far = max(max(dm2, dp2), 1e-30)
+    t = math.sqrt(near / far)
+    a = max(2 * atan(t), 1e-12)
+    b = math.pi - a
+    a2, b2, a3, b3 = a * a + 1e-9, b * b + 1e-9, a ** 3 + 1e-9, b ** 3 + 1e-9
+    sine = 2 * t / (1 + t * t) + 1e-9
+    sign = 1.0 if dm2 <= dp2 else -1.0
+    d = x[0] * y[1:] - y[0] * x[1:] + np.cross(x[1:], y[1:])
+    along = sign * (b2 - a2) / (a2 * b2 * sine)
+    force = -along * d
+    curv = (a3 + b3) / (a3 * b3)
+    energy = math.pi / (a * b)
+    if taper_start is not None:
+        s = min(max((a - taper_start) * inv_width, 0.0), 1.0)
+        if s > 0:
+            w = 1 - s * s * (3 - 2 * s)
+            slope = -6 * s * (1 - s) * inv_width
+            # exact_sin: sin(2 atan t) = 2t/(1+t^2) with the exact t, as the kernel takes it.
+            sin_a = 2 * t / (1 + t * t) if exact_sin else math.sin(a)
+            extra = sign * energy * slope / max(sin_a, 1e-9)
+            force = w * force + extra * d
+            curv *= w
+            energy *= w
+    return force, curv, energy
+
+
+def kernel_pair(x, y, dm2, dp2, taper_start=None, inv_width=None):
+    nr = max(min(dm2, dp2), 1e-30)
+    fr = max(max(dm2, dp2), 1e-30)
+    R = 1 / math.sqrt(nr * fr)
+    t = nr * R
+    a = max(2 * atan_poly(t), 1e-12)
+    b = math.pi - a
+    aa, bb = a * a, b * b
+    a2, b2 = aa + 1e-9, bb + 1e-9
+    a3, b3 = aa * a + 1e-9, bb * b + 1e-9
+    sine = 0.5 * t * fr + 1e-9
+    P1, P2, P3 = a2 * b2 * sine, a3 * b3, a * b
+    D = 1 / (P1 * P2 * P3)
+    iP1, iP2, iP3 = D * P2 * P3, D * P1 * P3, D * P1 * P2
+    diff = (b2 - a2) if dm2 <= dp2 else (a2 - b2)
+    coef = -diff * iP1
+    curv = (a3 + b3) * iP2
+    energy = math.pi * iP3
+    if taper_start is not None:
+        u = min(max((a - taper_start) * inv_width, 0.0), 1.0)
+        w = (u * u) * (2 * u - 3) + 1
+        slope = -6 * inv_width * (u * (1 - u))
+        inv_sin = 0.5 * (nr + fr) * R
+        e_slope = (energy if dm2 <= dp2 else -energy) * slope
+        coef = w * coef + e_slope * inv_sin
+        curv *= w
+        energy *= w
+    d = np.array([x[0] * y[1] - y[0] * x[1] + (x[2] * y[3] - x[3] * y[2]),
+                  x[0] * y[2] - y[0] * x[2] + (x[3] * y[1] - x[1] * y[3]),
+                  x[0] * y[3] - y[0] * x[3] + (x[1] * y[2] - x[2] * y[1])])
+    return coef * d, curv, energy
+
+
+def sample_pairs(count, seed=3):
+    rng = np.random.default_rng(seed)
+    for trial in range(count):
+        x = rng.normal(size=4)
+        x /= np.linalg.norm(x)
+        # S3 angles from 1e-4 to pi: both signs of the nearer image.
+        ang = 10 ** rng.uniform(-4, math.log10(math.pi))
+        axis = rng.normal(size=4)
+        axis -= axis.dot(x) * x
+        axis /= np.linalg.norm(axis)
+        y = math.cos(ang) * x + math.sin(ang) * axis
+        y /= np.linalg.norm(y)
+        window = (0.05, 1 / 0.05) if trial % 2 else (None, None)
+        yield x, y, ((x - y) ** 2).sum(), ((x + y) ** 2).sum(), window
+
+
+def test_pair_rewrite_is_the_reference_algebra():
+    """With the same arctangent, the rewrite is the reference formula up to FP64 rounding."""
+    for x, y, dm2, dp2, window in sample_pairs(4000):
+        fr, cr, er = reference_pair(x, y, dm2, dp2, *window, atan=atan_poly, exact_sin=True)
+        fk, ck, ek = kernel_pair(x, y, dm2, dp2, *window)
+        # Scales: the unwindowed terms, so zeros of the window or of the force do not
+        # turn rounding into a large relative error.
+        f0, c0, e0 = reference_pair(x, y, dm2, dp2, atan=atan_poly)
+        scale_f = abs(f0).max() + abs(e0) * 20 * np.abs(x).sum() * 2
+        assert abs(ek - er) <= 1e-12 * e0
+        assert abs(ck - cr) <= 1e-12 * c0
+        assert np.abs(fk - fr).max() <= 1e-11 * scale_f
+
+
+def test_pair_rewrite_error_against_exact_atan():
+    """End to end, the polynomial's error moves energy and curvature by at most a few
+    parts in 1e8 of their scale, and forces by the matching derivative."""
+    for x, y, dm2, dp2, window in sample_pairs(4000):
+        fr, cr, er = reference_pair(x, y, dm2, dp2, *window)
+        fk, ck, ek = kernel_pair(x, y, dm2, dp2, *window)
+        f0, c0, e0 = reference_pair(x, y, dm2, dp2)
+        a = 2 * math.atan(math.sqrt(min(dm2, dp2) / max(dm2, dp2)))
+        assert abs(ek - er) <= 1e-7 * e0
+        assert abs(ck - cr) <= 2e-7 * c0
+        # d(force)/da ~ 2/a^3 (+ window terms ~ E W'/sin a): a relative error of 3e-8 in a.
+        scale_f = 2 / max(a, 1e-12) ** 3 * 3e-8 * a + abs(e0) * 20 * 20 * 3e-8 * a / max(math.sin(a), 1e-9)
+        assert np.abs(fk - fr).max() <= 4 * scale_f + 1e-12 * np.abs(f0).max()
+
+
+def test_duplicate_pair_stays_finite():
+    x = np.array([0.9, 0.1, -0.3, 0.3])
+    x /= np.linalg.norm(x)
+    f, c, e = kernel_pair(x, x, 0.0, 4.0)
+    assert np.isfinite(f).all() and np.isfinite(c) and np.isfinite(e)
+    assert (f == 0).all()

@@ -1,0 +1,273 @@
+"""GPU parity: the gfx950 kernels through the C ABI against the CPU oracle.
+
+Bars: bit-exact for the fold, every nearest chord^2 and every spacing index (the image
+and chord arithmetic is exact-order on both sides); FP32 pair terms with FP64 sums within
+the stated relative tolerances for energies, curvatures and forces.  Reference tests
+restated: tests/test_pair_kernels.py, tests/test_near_field.py, tests/test_measures.py,
+tests/test_point_values.py, tests/test_workbench.py.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import zxtalori_amd as zo
+from oracle import dense_f64 as dense
+from tests.conftest import GROUPS, ops32
+
+pytestmark = pytest.mark.gpu
+GROUP_ID = {g: i for i, g in enumerate(GROUPS)}
+
+
+def note(*args):
+    print(*args, file=sys.stderr, flush=True)
+
+
+def random_unit(n, seed):
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=(n, 4))
+    return (q / np.linalg.norm(q, axis=-1, keepdims=True)).astype(np.float32)
+
+
+def check_rows(force, energy, nearest, ref, e_rtol=2e-6, f_rtol=2e-5):
+    np.testing.assert_array_equal(nearest, ref[:, 5].astype(np.float32))
+    np.testing.assert_allclose(energy, ref[:, 4], rtol=e_rtol)
+    np.testing.assert_allclose(force[:, 3], ref[:, 3], rtol=5 * e_rtol)
+    err = np.linalg.norm(force[:, :3] - ref[:, :3], axis=1)
+    assert err.max() <= f_rtol * np.abs(ref[:, :3]).max() * 10
+    assert np.linalg.norm(force[:, :3] - ref[:, :3]) <= f_rtol * np.linalg.norm(ref[:, :3])
+
+
+def test_fold_is_bit_exact(operator_fixture):
+    note("fold 11 groups x 4000")
+    q = np.random.default_rng(5).normal(size=(4000, 4)).astype(np.float32)
+    q[:100] *= np.geomspace(1e-8, 3e38, 100)[:, None].astype(np.float32)
+    with zo.from_quaternions(q, "C2") as grid:
+        for group in GROUPS:
+            grid.set(q, group)
+            expected = oracle.fold(q, ops32(operator_fixture, group), GROUP_ID[group])
+            np.testing.assert_array_equal(grid.quaternions, expected)
+
+
+def test_haar_matches_oracle(operator_fixture):
+    note("haar O/I/C2 x 100000")
+    for group, seed in (("O", 193), ("I", 7), ("C2", 48)):
+        with zo.haar(group, 100_000, seed) as grid:
+            q = grid.quaternions
+            with zo.haar(group, 100_000, seed) as again:
+                np.testing.assert_array_equal(again.quaternions, q)  # repeatable
+            raw = oracle.haar(100_000, seed)
+            folded = oracle.fold(raw, ops32(operator_fixture, group), GROUP_ID[group])
+            # sin/cos differ by ulps between the GPU and libm; the fold may then pick
+            # the other copy of a point within rounding of a zone face.
+            close = np.abs(q - folded).max(1) < 1e-6
+            assert close.mean() > 0.9999, group
+
+
+@pytest.mark.parametrize("group", GROUPS)
+def test_allpairs_rows_small(operator_fixture, group):
+    q = random_unit(37, 752)
+    with zo.from_quaternions(q, group) as grid:
+        force, energy, nearest = grid.force_rows()
+        ref = oracle.allpairs(grid.quaternions, ops32(operator_fixture, group))
+    check_rows(force, energy, nearest, ref)
+
+
+@pytest.mark.parametrize("group,count", [("O", 1000), ("I", 600), ("D4", 3000)])
+def test_allpairs_rows(operator_fixture, group, count):
+    note("allpairs", group, count)
+    with zo.haar(group, count, 3) as grid:
+        force, energy, nearest = grid.force_rows()
+        ref = oracle.allpairs(grid.quaternions, ops32(operator_fixture, group))
+        grid.pair_budget(1e-5)  # finer site splits: only rounding may change
+        f2, e2, n2 = grid.force_rows()
+    check_rows(force, energy, nearest, ref)
+    np.testing.assert_array_equal(n2, nearest)
+    np.testing.assert_allclose(e2, energy, rtol=1e-6)
+
+
+@pytest.mark.parametrize("group,count,delta", [("O", 1000, 20), ("D6", 1000, 20), ("C2", 800, 20),
+                                               ("I", 3000, 20), ("T", 2000, 4), ("D3", 5000, 8)])
+def test_near_rows(operator_fixture, group, count, delta):
+    note("near rows", group, count, delta)
+    with zo.haar(group, count, 11) as grid:
+        grid.relax(0, neighbourhood=delta)
+        force, energy, nearest = grid.force_rows()
+        q = grid.quaternions
+        info = grid.near_info()
+    ops = ops32(operator_fixture, group)
+    cut, _ = dense.near_cut(count, len(ops), delta)
+    ref = oracle.near_brute(q, ops, cut)
+    assert info["sites"] >= count
+    check_rows(force, energy, nearest, ref)
+
+
+def test_near_sums_match_dense_reference(operator_fixture):
+    """test_near_field.py:43-54."""
+    for group, count in (("O", 1000), ("D6", 1000), ("C2", 800)):
+        note("near dense", group, count)
+        with zo.haar(group, count, 11) as grid:
+            grid.relax(20)
+            grid.relax(1, neighbourhood=20)
+            stats = grid.stats
+            assert stats["energy_neighbourhood"] == 20 and stats["relax_neighbourhood"] == 20
+            energy, separation = dense.near_energy(grid.quaternions, grid.operations, 20)
+            assert stats["energy"] == pytest.approx(energy, rel=2e-6)
+            assert stats["separation_deg"] == pytest.approx(separation, rel=1e-4)
+            grid.relax(1, neighbourhood=0)
+            assert grid.stats["energy_neighbourhood"] is None
+
+
+def test_relax_matches_oracle(operator_fixture):
+    """30 BB steps of the small all-pairs workload: O, N=1000 normals (seed 193)."""
+    rng = np.random.default_rng(193)
+    q0 = rng.standard_normal((1000, 4))
+    q0 /= np.linalg.norm(q0, axis=1, keepdims=True)
+    ops = ops32(operator_fixture, "O")
+    with zo.from_quaternions(q0.astype(np.float32), "O") as grid:
+        start = grid.quaternions
+        grid.relax(30)
+        stats = grid.stats
+        q_gpu = grid.quaternions
+    q_cpu, rows, last = oracle.relax(start, ops, GROUP_ID["O"], 30)
+    n, order = len(q_cpu), len(ops)
+    e_cpu = rows[:, 4].sum() / (n * order * (n - 1))
+    assert stats["iteration"] == 30
+    assert stats["energy"] == pytest.approx(e_cpu, rel=1e-5)
+    sep_cpu = 4 * math.degrees(math.asin(math.sqrt(rows[:, 5].min()) / 2))
+    assert stats["separation_deg"] == pytest.approx(sep_cpu, rel=1e-3)
+    # Same representatives; positions agree to the BB amplification of FP32 rounding.
+    diff = np.minimum(np.abs(q_gpu - q_cpu).max(1), np.abs(q_gpu + q_cpu).max(1))
+    assert np.median(diff) < 1e-4
+    np.testing.assert_allclose(np.linalg.norm(q_gpu, axis=1), 1, atol=3e-7)
+
+
+def test_near_relaxation_approaches_all_pairs():
+    """test_near_field.py:57-65."""
+    outcomes = {}
+    for delta in (0, 20):
+        note("near vs all pairs", delta)
+        with zo.haar("O", 1000, 7) as grid:
+            grid.relax(200, neighbourhood=delta)
+            grid.relax(0, neighbourhood=0)
+            grid.measure()
+            outcomes[delta] = grid.stats
+    assert outcomes[20]["energy"] == pytest.approx(outcomes[0]["energy"], rel=1e-5)
+    assert outcomes[20]["separation_deg"] > 0.95 * outcomes[0]["separation_deg"]
+
+
+@pytest.mark.parametrize("group,count", [("O", 2000), ("I", 600), ("C2", 5000), ("D6", 3000)])
+def test_spacing_indices_all_pairs(operator_fixture, group, count):
+    with zo.haar(group, count, 9) as grid:
+        stats = grid.spacing()
+        src, op, sign = grid.nearest_indices()
+        values = grid.point_values("separation")
+        q = grid.quaternions
+    rank_ref, chord_ref = oracle.nearest(q, ops32(operator_fixture, group))
+    order = len(ops32(operator_fixture, group))
+    np.testing.assert_array_equal(src * 2 * order + op * 2 + sign, rank_ref)
+    np.testing.assert_allclose(values, oracle.degrees(chord_ref), rtol=2e-6, atol=1e-6)
+    assert stats["separation_deg"] == pytest.approx(float(values.min()))
+
+
+@pytest.mark.parametrize("group,count", [("C2", 25000), ("D2", 22000)])
+def test_spacing_indices_near_field(operator_fixture, group, count):
+    """Above 20,000 orientations spacing() scans a near field of 4 mean radii."""
+    note("spacing near", group, count)
+    ops = ops32(operator_fixture, group)
+    with zo.haar(group, count, 5) as grid:
+        grid.spacing()
+        src, op, sign = grid.nearest_indices()
+        q = grid.quaternions
+    cut, _ = dense.near_cut(count, len(ops), 4)
+    rank_ref, chord_ref = oracle.nearest(q, ops, float(cut[0]))
+    order = len(ops)
+    np.testing.assert_array_equal(src * 2 * order + op * 2 + sign, rank_ref)
+
+
+@pytest.mark.parametrize("group", GROUPS)
+def test_measure_matches_dense(group):
+    """test_measures.py:46-92: Quey energy, separation and force RMS of 29 points."""
+    q = random_unit(29, 312).astype(np.float64)
+    with zo.from_quaternions(q, group) as grid:
+        actual = grid.measure()
+        expected = dense.measures(grid.quaternions, grid.operations)
+        for key in ("quey_energy", "separation_deg", "force_rms"):
+            assert actual[key] == pytest.approx(expected[key], rel=1e-5), key
+        signed = grid.quaternions
+        signed[::2] *= -1
+        grid.set(signed, group)
+        again = grid.measure()
+        for key in ("quey_energy", "separation_deg", "force_rms"):
+            assert again[key] == pytest.approx(actual[key], rel=1e-5), key
+
+
+def test_close_pair_resolved_by_chords():
+    """test_measures.py:79-85: a pair 5e-5 rad apart."""
+    near = np.array([[1, 0, 0, 0], [math.cos(5e-5), math.sin(5e-5), 0, 0]], np.float64)
+    with zo.from_quaternions(near, "C2") as grid:
+        actual = grid.measure()
+        expected = dense.measures(grid.quaternions, grid.operations)
+        assert actual["separation_deg"] == pytest.approx(expected["separation_deg"], rel=1e-5)
+
+
+def test_error_paths():
+    with zo.from_quaternions(np.array([[1, 0, 0, 0], [1, 0, 0, 0]], np.float32), "C2") as grid:
+        with pytest.raises(RuntimeError, match="DuplicateOrientations"):
+            grid.measure()
+        with pytest.raises(RuntimeError, match="DuplicateOrientations"):
+            grid.relax(2)
+    with zo.haar("O", 200, 3) as grid:
+        with pytest.raises(RuntimeError):
+            grid.relax(1, neighbourhood=1)
+        with pytest.raises(RuntimeError, match="NonFiniteQuaternion"):
+            grid.set(np.array([[np.nan, 0, 0, 0]], np.float32))
+        with pytest.raises(RuntimeError, match="ZeroQuaternion"):
+            grid.set(np.zeros((1, 4), np.float32))
+        assert len(grid) == 200  # a failed import keeps the sample
+    with zo.from_quaternions(np.array([[1, 0, 0, 0]], np.float32), "O") as grid:
+        with pytest.raises(RuntimeError, match="TooFewOrientations"):
+            grid.relax(1)
+    with pytest.raises(RuntimeError, match="NotImplemented"):
+        zo.generate("O", count=100)
+
+
+def test_fixed_identity_stays_exact(operator_fixture):
+    """test_workbench.py:8-28: the held identity stays [1,0,0,0] and the zone holds."""
+    with zo.haar("O", 500, 4) as grid:
+        grid.identity(add=True, fixed=True)
+        index = grid.stats["identity_index"]
+        assert index is not None
+        before = grid.quaternions
+        grid.relax(12)
+        q = grid.quaternions.astype(np.float64)
+        ops = grid.operations.astype(np.float64)
+        np.testing.assert_array_equal(q[index], [1, 0, 0, 0])
+        assert np.max(np.abs(q @ ops.T) - q[:, :1]) < 1e-6
+        assert np.max(np.abs(q - before)) > 1e-5
+        grid.identity(fixed=False)
+        assert not grid.stats["fixed_identity"]
+
+
+def test_near_field_large_sample(operator_fixture):
+    """Group I, N=1,000,000, neighbourhood 20 (the throughput workload): deterministic
+    passes, and the rows of a 2,000-source sample equal the oracle's grid restatement."""
+    note("near I 1e6")
+    ops = ops32(operator_fixture, "I")
+    with zo.haar("I", 10_000_000, 193) as big:
+        q = big.quaternions[:1_000_000].copy()
+    with zo.from_quaternions(q, "I") as grid:
+        grid.relax(0, neighbourhood=20)
+        f1, e1, n1 = grid.force_rows()
+        f2, e2, n2 = grid.force_rows()
+        np.testing.assert_array_equal(f1, f2)
+        np.testing.assert_array_equal(e1, e2)
+        q = grid.quaternions
+    cut, r = dense.near_cut(len(q), len(ops), 20)
+    first = 123_456  # Haar indices carry no spatial order: a contiguous block is a random sample
+    ref, stats = oracle.near_grid(q, ops, cut, r, first, 2000)
+    check_rows(f1[first:first + 2000], e1[first:first + 2000], n1[first:first + 2000], ref)
+    assert (n1 > 0).all()
+    assert 7000 < stats["accepted"] / 2000 < 9000

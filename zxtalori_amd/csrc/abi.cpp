@@ -1,0 +1,182 @@
+// The zo_* C ABI (include/zxtalori.h): the reference's export table
+// (src/exports.zig:4-120, src/native_api.zig:9-55) over the HIP engine.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/zxtalori.h"
+#include "engine.h"
+#include "groups_table.h"
+
+struct zxt_engine {
+  zxt::Engine engine;
+};
+
+namespace {
+
+char g_create_error[1024] = {0};
+
+// Runs f; a thrown EngineError becomes -1 with "prefix: message" in zo_error.
+template <typename F>
+int32_t guarded(zxt_engine* h, const char* prefix, F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const zxt::EngineError& err) {
+    h->engine.err = std::string(prefix) + ": " + err.message;
+  } catch (const std::bad_alloc&) {
+    h->engine.err = std::string(prefix) + ": OutOfMemory";
+  } catch (...) {
+    h->engine.err = std::string(prefix) + ": Unexpected";
+  }
+  return -1;
+}
+
+int32_t not_implemented(zxt_engine* h, const char* prefix) {
+  h->engine.err = std::string(prefix) + ": NotImplemented";
+  return -1;
+}
+
+bool valid_group(uint32_t id) { return id < zxt::kGroupCount; }
+
+}  // namespace
+
+extern "C" {
+
+const char* zo_version(void) { return "0.3.0-mi355x"; }
+
+zxt_engine* zo_create(const char* /*wgpu_dir*/) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+    (void)hipGetLastError();
+    snprintf(g_create_error, sizeof(g_create_error), "GPU init: NoHipDevice");
+    return nullptr;
+  }
+  try {
+    return new zxt_engine();
+  } catch (const zxt::EngineError& err) {
+    snprintf(g_create_error, sizeof(g_create_error), "Engine init: %s", err.message.c_str());
+  } catch (...) {
+    snprintf(g_create_error, sizeof(g_create_error), "Engine init: Unexpected");
+  }
+  return nullptr;
+}
+
+const char* zo_create_error(void) { return g_create_error; }
+
+void zo_destroy(zxt_engine* e) { delete e; }
+
+int32_t zo_random(zxt_engine* e, uint32_t group, uint32_t count, uint32_t seed) {
+  if (!valid_group(group)) {
+    e->engine.err = "Unknown group";
+    return -1;
+  }
+  return guarded(e, "Haar initialization", [&] { e->engine.generate_random(group, count, seed); });
+}
+
+int32_t zo_set(zxt_engine* e, uint32_t group, const float* q, uint32_t n) {
+  if (!valid_group(group)) {
+    e->engine.err = "Unknown group";
+    return -1;
+  }
+  return guarded(e, "Import failed", [&] { e->engine.set(group, q, size_t(n) * 4); });
+}
+
+int32_t zo_generate(zxt_engine* e, uint32_t group, uint32_t, uint32_t, uint32_t, uint32_t kind) {
+  if (!valid_group(group)) {
+    e->engine.err = "Unknown group";
+    return -1;
+  }
+  if (kind > 2) {
+    e->engine.err = "Unknown grid kind";
+    return -1;
+  }
+  return not_implemented(e, "Generation failed");
+}
+
+uint32_t zo_count(zxt_engine* e) { return e->engine.count(); }
+uint32_t zo_order(zxt_engine* e) { return e->engine.order(); }
+const float* zo_operations(zxt_engine* e) { return e->engine.operations(); }
+const float* zo_quaternions(zxt_engine* e) { return e->engine.quaternions(); }
+
+int32_t zo_measure(zxt_engine* e) {
+  return guarded(e, "Measure failed", [&] { e->engine.measure(); });
+}
+int32_t zo_relax(zxt_engine* e, uint32_t steps) {
+  return guarded(e, "Relaxation failed", [&] { e->engine.relax(steps); });
+}
+int32_t zo_relax_neighbourhood(zxt_engine* e, float delta) {
+  return guarded(e, "Relaxation neighbourhood", [&] { e->engine.set_neighbourhood(double(delta)); });
+}
+int32_t zo_spacing(zxt_engine* e) {
+  return guarded(e, "Spacing failed", [&] { e->engine.spacing(); });
+}
+const float* zo_point_values(zxt_engine* e, uint32_t kind) { return e->engine.point_values(kind); }
+int32_t zo_identity(zxt_engine* e, uint32_t flags) {
+  return guarded(e, "Identity constraint", [&] { e->engine.identity((flags & 1u) != 0, (flags & 2u) != 0); });
+}
+const char* zo_stats(zxt_engine* e) {
+  try {
+    e->engine.stats();
+  } catch (...) {
+    e->engine.text = "{}";
+  }
+  return e->engine.text.c_str();
+}
+const char* zo_error(zxt_engine* e) { return e->engine.err.c_str(); }
+void zo_cancel(zxt_engine* e) { e->engine.cancelled.store(true); }
+
+int32_t zo_cover(zxt_engine* e) { return not_implemented(e, "Covering radius"); }
+int32_t zo_cover_optimize(zxt_engine* e, uint32_t, uint32_t, uint32_t, float) {
+  return not_implemented(e, "Covering optimization");
+}
+int32_t zo_cover_kicks(zxt_engine* e, uint32_t, float, float, float) { return not_implemented(e, "Covering kicks"); }
+int32_t zo_mesh(zxt_engine* e) { return not_implemented(e, "DT display"); }
+const float* zo_mesh_vertices(zxt_engine*) { return nullptr; }
+const uint32_t* zo_mesh_tets(zxt_engine*) { return nullptr; }
+uint32_t zo_mesh_vertex_count(zxt_engine*) { return 0; }
+uint32_t zo_mesh_tet_count(zxt_engine*) { return 0; }
+int32_t zo_generate_exact(zxt_engine* e, uint32_t group, uint32_t, uint32_t) {
+  if (!valid_group(group)) {
+    e->engine.err = "Unknown group";
+    return -1;
+  }
+  return not_implemented(e, "Exact-count generation");
+}
+
+bool zo_profile(zxt_engine* e, bool enabled) {
+  e->engine.prof.enable(enabled);
+  return e->engine.prof.on();
+}
+size_t zo_profile_report(zxt_engine* e, char* output, size_t capacity) {
+  const std::string r = e->engine.prof.report();
+  if (!output || capacity == 0) return r.size();
+  const size_t n = std::min(capacity - 1, r.size());
+  std::memcpy(output, r.data(), n);
+  output[n] = 0;
+  return n;
+}
+void zo_pair_budget(zxt_engine* e, float scale) { e->engine.pair_budget_scale = double(scale); }
+
+const uint32_t* zo_point_indices(zxt_engine* e) { return e->engine.point_indices(); }
+int32_t zo_force_passes(zxt_engine* e, uint32_t passes) {
+  return guarded(e, "Force passes", [&] { e->engine.force_passes(passes); });
+}
+int32_t zo_force_rows(zxt_engine* e, float* force4, double* energy, float* nearest) {
+  return guarded(e, "Force rows", [&] { e->engine.force_rows(force4, energy, nearest); });
+}
+int32_t zo_near_pairs(zxt_engine* e, uint64_t* out2) {
+  out2[0] = e->engine.last_near_counts[0];
+  out2[1] = e->engine.last_near_counts[1];
+  return 0;
+}
+int32_t zo_near_info(zxt_engine* e, uint32_t* out3) {
+  out3[0] = e->engine.last_near.sites;
+  out3[1] = e->engine.last_near.sites >= e->engine.count() ? e->engine.last_near.sites - e->engine.count() : 0;
+  out3[2] = e->engine.last_near_sdim;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,780 @@
+// Host engine: sample state, relaxation, measure and spacing drivers.
+//
+// Follows the reference's control flow (src/engine.zig:233-376, :886-1020;
+// src/pairs.zig:220-413; src/near.zig:90-196) over HIP: every step records kernels on
+// one stream, and the host waits only where the reference does (end of a relax call,
+// measure, spacing, the near field's one-time sizing pass).
+#include "engine.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+
+#include "groups_table.h"
+
+namespace zxt {
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+constexpr uint32_t kMaxCount = 10000000;      // hopf.max_count (src/hopf.zig:23)
+constexpr double kFullMeasurePairs = 5e10;    // engine.zig:1123
+constexpr uint32_t kSpacingAllPairs = 20000;  // engine.zig:1126
+constexpr double kSpacingDelta = 4;           // engine.zig:1127
+const char* kVersion = "0.3.0-mi355x";
+
+[[noreturn]] void fail(const std::string& what) { throw EngineError{what}; }
+
+double nearest_degrees(double chord2) { return 4.0 * std::asin(std::min(1.0, std::sqrt(chord2) / 2.0)) * 180.0 / kPi; }
+}  // namespace
+
+double mean_radius(uint64_t count) {
+  const double target = kPi * kPi / double(count);
+  double lo = 0.0, hi = kPi;
+  for (int k = 0; k < 60; ++k) {
+    const double m = 0.5 * (lo + hi);
+    if (kPi * (m - std::sin(m)) < target)
+      lo = m;
+    else
+      hi = m;
+  }
+  return 0.25 * (lo + hi);
+}
+
+// ---- DevBuf ----------------------------------------------------------------------
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+void DevBuf::ensure(size_t b) {
+  if (p && b <= bytes) return;
+  release();
+  const size_t want = std::max<size_t>(b, 256);
+  if (hipMalloc(&p, want) != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    fail("OutOfDeviceMemory");
+  }
+  bytes = want;
+}
+
+void Engine::check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(std::string("GpuError: ") + what + ": " + hipGetErrorString(e));
+}
+
+// ---- NearField -------------------------------------------------------------------
+NearBuffers NearField::view() const {
+  NearBuffers b;
+  b.planes = planes.as<float4>();
+  b.nplanes = nplanes;
+  b.band = band;
+  b.capacity = capacity;
+  b.sdim = sdim;
+  b.info = info.as<NearInfo>();
+  b.rim = rim.as<uint32_t>();
+  b.uq = uq.as<float4>();
+  b.ucode = ucode.as<uint32_t>();
+  b.urank = urank.as<uint32_t>();
+  b.ucell = ucell.as<uint32_t>();
+  b.perm = perm.as<uint32_t>();
+  b.sq = sq.as<float4>();
+  b.scode = scode.as<uint32_t>();
+  b.srank = srank.as<uint32_t>();
+  b.offsets = offsets.as<uint32_t>();
+  b.cursor = cursor.as<uint32_t>();
+  b.src_cell = src_cell.as<uint32_t>();
+  b.src_offsets = src_offsets.as<uint32_t>();
+  b.src_cursor = src_cursor.as<uint32_t>();
+  b.order = order_buf.as<uint32_t>();
+  b.scan_tmp = scan_tmp.as<uint32_t>();
+  b.counters = counters.as<unsigned long long>();
+  return b;
+}
+
+static void release_near(NearField& f) {
+  for (DevBuf* b : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
+                    &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor, &f.order_buf,
+                    &f.scan_tmp, &f.counters})
+    b->release();
+  f.ready = false;
+}
+
+// Sizes the buffers from one sites pass (one wait), as near.zig:90-152 does.
+void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, uint32_t group_,
+                      double mean_radius_) {
+  release_near(*this);
+  n = n_;
+  order = order_;
+  group = group_;
+  radius = std::min(1.5, delta * mean_radius_);
+  const double chord = 2.0 * std::sin(radius / 2.0);
+  cut.chord2 = float(chord * chord);
+  cut.taper_start = float(taper * radius);
+  cut.inv_width = float(1.0 / std::max(1e-9, (1.0 - taper) * radius));
+  // Every site the FP32 test chord^2 <= chord2 can accept lies within this cap.
+  const double margin = double(cut.chord2) * (1.0 + 1e-5) + 1e-6;
+  const double c0 = 1.0 - margin / 2.0;
+  cut.c0 = float(c0);
+  cut.s0 = float(std::sqrt(std::max(0.0, 1.0 - c0 * c0)));
+  band = float(std::sin(radius + 1e-4));
+  // Zone planes: the bisectors of the identity and every other signed image
+  // (covering.zig:382-397), normalised in FP64.
+  std::vector<float4> pl;
+  const uint32_t off = kGroupOffset[group];
+  for (uint32_t i = 0; i < order; ++i) {
+    for (double s : {1.0, -1.0}) {
+      if (i == 0 && s == 1.0) continue;
+      const float* op = kOperators[off + i];
+      double v[4] = {1.0 - s * op[0], -s * op[1], -s * op[2], -s * op[3]};
+      const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+      pl.push_back(make_float4(float(v[0] / len), float(v[1] / len), float(v[2] / len), float(v[3] / len)));
+    }
+  }
+  nplanes = uint32_t(pl.size());
+  planes.ensure(pl.size() * sizeof(float4));
+  Engine::check(hipMemcpyAsync(planes.p, pl.data(), pl.size() * sizeof(float4), hipMemcpyHostToDevice, e.stream),
+                "planes");
+  info.ensure(sizeof(NearInfo));
+  rim.ensure(size_t(n) * 4);
+  uint64_t cap = std::max<uint64_t>(uint64_t(n) + 1024, uint64_t(n) * 4);
+  bool sized = false;
+  for (int attempt = 0; attempt < 5; ++attempt) {
+    capacity = uint32_t(std::min<uint64_t>(cap, 0xfff00000ull));
+    uq.ensure(size_t(capacity) * 16);
+    ucode.ensure(size_t(capacity) * 4);
+    urank.ensure(size_t(capacity) * 4);
+    sdim = 1;
+    launch_near_sites(e.prof, e.stream, q, n, e.ops_device(), order, view());
+    Engine::check(hipMemcpyAsync(&status, info.p, sizeof(NearInfo), hipMemcpyDeviceToHost, e.stream), "info");
+    e.sync();
+    if (!status.overflow) {
+      sized = true;
+      break;
+    }
+    cap = std::max<uint64_t>(cap * 2, uint64_t(status.sites) + status.sites / 4);
+  }
+  if (!sized) fail("NearShellTooLarge");
+  const uint64_t count = status.sites;
+  // Room for the shell to change as the sources move.
+  capacity = uint32_t(std::min<uint64_t>(std::max<uint64_t>(capacity, count + count / 4 + 1024), 0xfff00000ull));
+  sdim = uint32_t(std::min(320.0, std::max(1.0, std::round(std::cbrt(double(count) / 2.0)))));
+  const size_t cells = size_t(sdim) * sdim * sdim;
+  uq.ensure(size_t(capacity) * 16);
+  ucode.ensure(size_t(capacity) * 4);
+  urank.ensure(size_t(capacity) * 4);
+  ucell.ensure(size_t(capacity) * 4);
+  perm.ensure(size_t(capacity) * 4);
+  sq.ensure(size_t(capacity) * 16);
+  scode.ensure(size_t(capacity) * 4);
+  srank.ensure(size_t(capacity) * 4);
+  offsets.ensure((cells + 1) * 4);
+  cursor.ensure((cells + 1) * 4);
+  src_cell.ensure(size_t(n) * 4);
+  src_offsets.ensure((size_t(kMortonCells) + 1) * 4);
+  src_cursor.ensure((size_t(kMortonCells) + 1) * 4);
+  order_buf.ensure(size_t(n) * 4);
+  scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, kMortonCells)) * 4);
+  counters.ensure(4 * sizeof(unsigned long long));
+  ready = true;
+}
+
+void NearField::build(Engine& e, const float4* q, uint32_t n_, uint32_t order_, uint32_t group_,
+                      double mean_radius_) {
+  if (!ready || n != n_ || order != order_ || group != group_) setup(e, q, n_, order_, group_, mean_radius_);
+  launch_near_build(e.prof, e.stream, q, n, e.ops_device(), order, view());
+}
+
+bool NearField::overflowed(Engine& e) {
+  Engine::check(hipMemcpyAsync(&status, info.p, sizeof(NearInfo), hipMemcpyDeviceToHost, e.stream), "info");
+  Engine::check(hipMemcpyAsync(e.last_near_counts, counters.p, sizeof(e.last_near_counts), hipMemcpyDeviceToHost,
+                               e.stream),
+                "counters");
+  e.sync();
+  e.last_near = status;
+  e.last_near_sdim = sdim;
+  if (!status.overflow) return false;
+  ready = false;
+  return true;
+}
+
+// ---- RelaxState --------------------------------------------------------------------
+void RelaxState::release() {
+  for (DevBuf* b : {&force, &energy, &nearest, &prev, &state, &partials, &ap_partial, &ap_pnear, &nn_partial,
+                    &nn_chord, &nn_rank})
+    b->release();
+  release_near(near);
+  n = 0;
+  current = false;
+}
+
+void RelaxState::set_near(uint32_t group, const double* delta) {
+  if (has_near) {
+    if (delta && near.group == group && near.delta == *delta) return;
+    release_near(near);
+    has_near = false;
+    current = false;
+  }
+  if (delta) {
+    has_near = true;
+    near.delta = *delta;
+    near.group = group;
+    near.ready = false;
+    current = false;
+  }
+}
+
+void RelaxState::prepare(Engine& e, uint32_t n_) {
+  if (n == n_ && state.p) return;
+  const bool keep_near = has_near;
+  const double delta = near.delta;
+  const uint32_t group = near.group;
+  release();
+  has_near = keep_near;
+  near.delta = delta;
+  near.group = group;
+  n = n_;
+  prev.ensure(size_t(n) * 32);
+  launch_fill_u32(e.prof, e.stream, prev.as<uint32_t>(), 0u, size_t(n) * 8);
+  state.ensure(sizeof(BBState));
+  launch_bb_reset(e.prof, e.stream, state.as<BBState>());
+  partials.ensure(size_t(bb_partials(n)) * 5 * sizeof(double));
+}
+
+void RelaxState::ensure_rows(uint32_t n_) {
+  force.ensure(size_t(n_) * 16);
+  energy.ensure(size_t(n_) * 8);
+  nearest.ensure(size_t(n_) * 4);
+}
+
+void RelaxState::record_forces(Engine& e, uint32_t n_, uint32_t order_) {
+  ensure_rows(n_);
+  float4* q = static_cast<float4*>(e.q_device());
+  if (has_near) {
+    near.build(e, q, n_, order_, e.group(), mean_radius(uint64_t(n_) * order_));
+    launch_near_force(e.prof, e.stream, q, n_, near.view(), near.cut, rows());
+  } else {
+    const AllPairsPlan plan = plan_allpairs(n_, order_, e.pair_budget_scale);
+    ap_partial.ensure(size_t(plan.splits()) * n_ * 5 * sizeof(double));
+    ap_pnear.ensure(size_t(plan.splits()) * n_ * sizeof(float));
+    launch_allpairs(e.prof, e.stream, q, n_, e.ops_device(), order_, plan, ap_partial.as<double>(),
+                    ap_pnear.as<float>(), rows());
+  }
+}
+
+// ---- Engine ------------------------------------------------------------------------
+Engine::Engine() {
+  check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+  prof.set_stream(stream);
+  ops_.ensure(sizeof(kOperators));
+  check(hipMemcpy(ops_.p, kOperators, sizeof(kOperators), hipMemcpyHostToDevice), "operators");
+  set_group(9);
+}
+
+Engine::~Engine() {
+  (void)hipStreamSynchronize(stream);
+  relax_state_.release();
+  spacing_state_.release();
+  measure_state_.release();
+  q_.release();
+  ops_.release();
+  (void)hipStreamDestroy(stream);
+}
+
+void Engine::sync() { check(hipStreamSynchronize(stream), "wait"); }
+
+void* Engine::q_device() const { return q_.p; }
+uint32_t Engine::order() const { return kGroupOrder[group_]; }
+const float* Engine::operations() const { return kOperators[kGroupOffset[group_]]; }
+const float4* Engine::ops_device() const { return ops_.as<float4>() + kGroupOffset[group_]; }
+uint32_t Engine::ratio_limit() const { return uint32_t(std::floor(std::sqrt(kFullMeasurePairs / order()))); }
+
+void Engine::set_group(uint32_t group) {
+  if (group >= kGroupCount) fail("UnknownGroup");
+  group_ = group;
+}
+
+void Engine::fold(uint32_t n) { launch_fold(prof, stream, q_.as<float4>(), n, ops_device(), order(), group_); }
+
+void Engine::fetch() {
+  host_.resize(size_t(n_) * 4);
+  check(hipMemcpyAsync(host_.data(), q_.p, size_t(n_) * 16, hipMemcpyDeviceToHost, stream), "fetch");
+  sync();
+}
+
+void Engine::reset_run() {
+  point_separation_iteration_ = -1;
+  point_index_iteration_ = -1;
+  relax_state_.release();
+  spacing_state_.release();
+  measure_state_.release();
+  iteration_ = 0;
+  step_deg_ = 0;
+  force_rms_ = 0;
+  measured_ = false;
+  full_measures_ = false;
+  cancelled.store(false);
+  err.clear();
+}
+
+void Engine::adopt(uint32_t n) {
+  n_ = n;
+  fetch();
+  fixed_index_ = fix_identity_ ? identity_index() : -1;
+  reset_run();
+}
+
+long Engine::identity_index() const {
+  for (uint32_t i = 0; i < n_; ++i) {
+    const float* a = &host_[size_t(i) * 4];
+    if (std::fabs(a[1]) + std::fabs(a[2]) + std::fabs(a[3]) < 1e-6f) return long(i);
+  }
+  return -1;
+}
+
+void Engine::set(uint32_t group, const float* values, size_t count4) {
+  if (count4 == 0 || count4 % 4 != 0 || count4 > size_t(4) * kMaxCount) fail("InvalidCount");
+  for (size_t i = 0; i < count4 / 4; ++i) {
+    double norm = 0.0;
+    for (int c = 0; c < 4; ++c) {
+      const float x = values[4 * i + c];
+      if (!std::isfinite(x)) fail("NonFiniteQuaternion");
+      norm += double(x) * x;
+    }
+    if (norm < 1e-20) fail("ZeroQuaternion");
+  }
+  const uint32_t old = group_;
+  set_group(group);
+  try {
+    const uint32_t n = uint32_t(count4 / 4);
+    q_.ensure(size_t(n) * 16);
+    check(hipMemcpyAsync(q_.p, values, size_t(n) * 16, hipMemcpyHostToDevice, stream), "upload");
+    fold(n);
+    adopt(n);
+  } catch (...) {
+    group_ = old;
+    throw;
+  }
+  generator_ = "Imported quaternions";
+  seed_ = 0;
+  base_count_ = n_;
+}
+
+void Engine::generate_random(uint32_t group, uint32_t count, uint32_t seed) {
+  if (count < 1 || count > kMaxCount) fail("InvalidCount");
+  const uint32_t old = group_;
+  set_group(group);
+  try {
+    q_.ensure(size_t(count) * 16);
+    launch_haar(prof, stream, q_.as<float4>(), count, seed);
+    fold(count);
+    adopt(count);
+  } catch (...) {
+    group_ = old;
+    throw;
+  }
+  generator_ = "Haar random";
+  seed_ = seed;
+  base_count_ = count;
+}
+
+void Engine::set_neighbourhood(double delta) {
+  if (!(delta == 0 || (delta >= 2 && delta <= 1000))) fail("NeighbourhoodOutside2To1000");
+  neighbourhood_ = delta;
+}
+
+void Engine::sync_neighbourhood() {
+  relax_state_.set_near(group_, neighbourhood_ > 0 ? &neighbourhood_ : nullptr);
+  measure_neighbourhood_ = neighbourhood_;
+}
+
+std::vector<float>& Engine::separation_out() {
+  point_separation_.resize(n_);
+  point_separation_iteration_ = long(iteration_);
+  return point_separation_;
+}
+
+Measures Engine::quick_measures(const std::vector<float4>& f, const std::vector<double>& energy,
+                                const std::vector<float>& nearest) const {
+  const uint32_t n = n_;
+  double nearv = 4.0, e = 0.0, sq = 0.0;
+  for (uint32_t i = 0; i < n; ++i) {
+    nearv = std::min(nearv, double(nearest[i]));
+    e += energy[i];
+    sq += double(f[i].x) * f[i].x + double(f[i].y) * f[i].y + double(f[i].z) * f[i].z;
+  }
+  if (!(nearv > 0)) fail("DuplicateOrientations");
+  Measures m;
+  const double nd = n;
+  m.count_extended = 2.0 * order() * nd;
+  m.quey_energy = e / (nd * order() * std::max(nd - 1.0, 1.0));
+  m.force_rms = std::sqrt(sq / nd);
+  m.separation_deg = nearest_degrees(nearv);
+  return m;
+}
+
+// One force pass of the current positions, read back (retried when the near field's
+// shell outgrew its buffers); the forces are kept for the next step.
+void Engine::read_forces(RelaxState& rs, std::vector<float4>& f, std::vector<double>& en, std::vector<float>& nr) {
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    rs.prepare(*this, n_);
+    rs.current = false;
+    rs.record_forces(*this, n_, order());
+    download(f, rs.force, n_);
+    download(en, rs.energy, n_);
+    download(nr, rs.nearest, n_);
+    sync();
+    if (rs.has_near && rs.near.overflowed(*this)) continue;
+    rs.current = true;
+    return;
+  }
+  fail("NearShellTooLarge");
+}
+
+// Measures of the current positions (pairs.zig:302-316).
+Measures Engine::relax_measure(RelaxState& rs, bool keep_separation) {
+  std::vector<float4> f;
+  std::vector<double> en;
+  std::vector<float> nr;
+  read_forces(rs, f, en, nr);
+  Measures m = quick_measures(f, en, nr);
+  if (keep_separation) {
+    auto& out = separation_out();
+    for (uint32_t i = 0; i < n_; ++i) out[i] = float(nearest_degrees(nr[i]));
+  }
+  return m;
+}
+
+void Engine::force_rows(float* force4, double* energy, float* nearest) {
+  if (n_ < 2) fail("TooFewOrientations");
+  sync_neighbourhood();
+  std::vector<float4> f;
+  std::vector<double> en;
+  std::vector<float> nr;
+  read_forces(relax_state_, f, en, nr);
+  std::memcpy(force4, f.data(), size_t(n_) * 16);
+  std::memcpy(energy, en.data(), size_t(n_) * 8);
+  std::memcpy(nearest, nr.data(), size_t(n_) * 4);
+}
+
+void Engine::measure() {
+  if (n_ == 0) fail("NoOrientations");
+  if (n_ > ratio_limit()) {
+    sync_neighbourhood();
+    measures_ = relax_measure(relax_state_, true);
+    full_measures_ = false;
+  } else {
+    // All-pairs energy and separation, then per-point separations over the relaxation's
+    // neighbourhood (engine.zig:317-324).
+    measure_state_.set_near(group_, nullptr);
+    measures_ = relax_measure(measure_state_, true);
+    full_measures_ = false;
+    sync_neighbourhood();
+    if (neighbourhood_ > 0) relax_measure(relax_state_, true);
+    measure_neighbourhood_ = 0;
+  }
+  measured_ = true;
+  measure_iteration_ = iteration_;
+}
+
+double Engine::nearest_pass(RelaxState& rs) {
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    rs.current = false;
+    rs.nn_chord.ensure(size_t(n_) * 4);
+    rs.nn_rank.ensure(size_t(n_) * 4);
+    float4* q = q_.as<float4>();
+    if (rs.has_near) {
+      rs.near.build(*this, q, n_, order(), group_, mean_radius(uint64_t(n_) * order()));
+      launch_near_nearest(prof, stream, q, n_, order(), rs.near.view(), rs.near.cut, rs.nn_chord.as<float>(),
+                          rs.nn_rank.as<uint32_t>());
+    } else {
+      rs.nn_partial.ensure(size_t(order()) * n_ * 8);
+      launch_allpairs_nearest(prof, stream, q, n_, ops_device(), order(), rs.nn_partial.as<uint64_t>(),
+                              rs.nn_chord.as<float>(), rs.nn_rank.as<uint32_t>());
+    }
+    std::vector<float> chord;
+    download(chord, rs.nn_chord, n_);
+    download(point_index_, rs.nn_rank, n_);
+    sync();
+    if (rs.has_near && rs.near.overflowed(*this)) continue;
+    auto& out = separation_out();
+    point_index_iteration_ = long(iteration_);
+    float least = 180.0f;
+    const float k = 180.0f / float(kPi);
+    for (uint32_t i = 0; i < n_; ++i) {
+      out[i] = 4.0f * std::asin(std::min(1.0f, std::sqrt(chord[i]) * 0.5f)) * k;
+      least = std::min(least, out[i]);
+    }
+    if (!(least > 0)) fail("DuplicateOrientations");
+    return least;
+  }
+  fail("NearShellTooLarge");
+}
+
+void Engine::spacing() {
+  if (n_ < 2) fail("TooFewOrientations");
+  const double delta = kSpacingDelta;
+  spacing_state_.set_near(group_, n_ > kSpacingAllPairs ? &delta : nullptr);
+  const double least = nearest_pass(spacing_state_);
+  if (measured_ && measure_iteration_ == iteration_) {
+    measures_.separation_deg = least;
+    return;
+  }
+  measures_ = Measures{};
+  measures_.count_extended = 2.0 * order() * double(n_);
+  measures_.separation_deg = least;
+  measured_ = true;
+  measure_iteration_ = iteration_;
+  full_measures_ = false;
+}
+
+void Engine::relax(uint32_t steps) {
+  if (n_ < 2) fail("TooFewOrientations");
+  cancelled.store(false);
+  sync_neighbourhood();
+  const uint32_t ord = order();
+  const double radius = mean_radius(uint64_t(n_) * ord);
+  RelaxState& rs = relax_state_;
+  bool recorded = false;
+  const uint32_t count = std::min<uint32_t>(steps, 1000);
+  for (uint32_t index = 0; index < count; ++index) {
+    if (cancelled.load()) break;
+    if (fix_identity_ && fixed_index_ < 0) fail("IdentityMissingEnableAddIdentity");
+    rs.prepare(*this, n_);
+    if (!rs.current) rs.record_forces(*this, n_, ord);
+    rs.current = false;
+    launch_bb_step(prof, stream, q_.as<float4>(), n_, rs.rows(), rs.prev.as<float4>(), rs.state.as<BBState>(),
+                   rs.partials.as<double>(), radius, 1.0f, fixed_index_ >= 0 ? uint32_t(fixed_index_ + 1) : 0u);
+    // Right-action folding keeps the left tangent frame of the BB history (engine.zig:914-916).
+    fold(n_);
+    ++iteration_;
+    recorded = true;
+  }
+  measured_ = false;
+  auto& sep = separation_out();
+  // One wait ends the batch: last step, positions, energy and separation (pairs.zig:377-399).
+  BBState st{};
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    rs.prepare(*this, n_);
+    rs.current = false;
+    rs.record_forces(*this, n_, ord);
+    if (recorded) check(hipMemcpyAsync(&st, rs.state.p, sizeof(BBState), hipMemcpyDeviceToHost, stream), "state");
+    host_.resize(size_t(n_) * 4);
+    check(hipMemcpyAsync(host_.data(), q_.p, size_t(n_) * 16, hipMemcpyDeviceToHost, stream), "positions");
+    std::vector<float4> f;
+    std::vector<double> en;
+    std::vector<float> nr;
+    download(f, rs.force, n_);
+    download(en, rs.energy, n_);
+    download(nr, rs.nearest, n_);
+    sync();
+    if (recorded) {
+      if (st.flags & 1u) fail("NonFiniteForce");
+      if (st.flags & 2u) fail("DuplicateOrientations");
+      float step_max;
+      std::memcpy(&step_max, &st.step_bits, 4);
+      step_deg_ = double(step_max) * 360.0 / kPi;
+      force_rms_ = std::sqrt(st.ff / double(n_));
+      recorded = false;
+    }
+    if (rs.has_near && rs.near.overflowed(*this)) continue;
+    measures_ = quick_measures(f, en, nr);
+    for (uint32_t i = 0; i < n_; ++i) sep[i] = float(nearest_degrees(nr[i]));
+    full_measures_ = false;
+    measured_ = true;
+    measure_iteration_ = iteration_;
+    rs.current = true;
+    return;
+  }
+  fail("NearShellTooLarge");
+}
+
+void Engine::identity(bool add, bool fixed) {
+  if (n_ == 0) fail("NoOrientations");
+  long index = identity_index();
+  if (add && index < 0) {
+    const std::string generator = generator_;
+    const uint32_t base = base_count_, seed = seed_;
+    std::vector<float> values(host_);
+    values.insert(values.end(), {1.0f, 0.0f, 0.0f, 0.0f});
+    set(group_, values.data(), values.size());
+    generator_ = generator;
+    base_count_ = base;
+    seed_ = seed;
+    index = long(n_) - 1;
+  }
+  if (fixed && index < 0) fail("IdentityMissingEnableAddIdentity");
+  if (fix_identity_ != fixed) relax_state_.release();
+  fix_identity_ = fixed;
+  fixed_index_ = fixed ? index : -1;
+  if (fixed) {
+    const float one[4] = {1.0f, 0.0f, 0.0f, 0.0f};
+    std::memcpy(&host_[size_t(index) * 4], one, 16);
+    check(hipMemcpyAsync(static_cast<char*>(q_.p) + size_t(index) * 16, one, 16, hipMemcpyHostToDevice, stream),
+          "identity");
+    sync();
+    relax_state_.current = false;
+  }
+}
+
+void Engine::force_passes(uint32_t passes) {
+  if (n_ < 2) fail("TooFewOrientations");
+  sync_neighbourhood();
+  RelaxState& rs = relax_state_;
+  rs.prepare(*this, n_);
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    for (uint32_t k = 0; k < passes; ++k) rs.record_forces(*this, n_, order());
+    sync();
+    if (rs.has_near && rs.near.overflowed(*this)) continue;
+    rs.current = true;
+    return;
+  }
+  fail("NearShellTooLarge");
+}
+
+const float* Engine::point_values(uint32_t kind) const {
+  if (kind != 0 || point_separation_iteration_ < 0 || point_separation_.size() != n_) return nullptr;
+  return point_separation_.data();
+}
+
+const uint32_t* Engine::point_indices() const {
+  if (point_index_iteration_ < 0 || point_index_.size() != n_) return nullptr;
+  return point_index_.data();
+}
+
+// ---- stats JSON (engine.zig:938-1020) ----------------------------------------------
+namespace {
+struct Json {
+  std::string s = "{";
+  bool first = true;
+  void key(const char* k) {
+    if (!first) s += ",";
+    first = false;
+    s += "\"";
+    s += k;
+    s += "\":";
+  }
+  void num(const char* k, double v) {
+    key(k);
+    if (!std::isfinite(v)) {
+      s += "null";
+      return;
+    }
+    char b[64];
+    snprintf(b, sizeof(b), "%.17g", v);
+    s += b;
+  }
+  void integer(const char* k, long long v) {
+    key(k);
+    s += std::to_string(v);
+  }
+  void null(const char* k) {
+    key(k);
+    s += "null";
+  }
+  void boolean(const char* k, bool v) {
+    key(k);
+    s += v ? "true" : "false";
+  }
+  void str(const char* k, const std::string& v) {
+    key(k);
+    s += "\"";
+    for (char c : v) {
+      if (c == '"' || c == '\\') s += '\\';
+      s += c;
+    }
+    s += "\"";
+  }
+};
+}  // namespace
+
+std::string Engine::stats() {
+  const bool m = measured_;
+  const bool current = m && measure_iteration_ == iteration_;
+  const uint32_t ord = order();
+  Json j;
+  j.str("version", kVersion);
+  j.str("group", kGroupNames[group_]);
+  j.integer("order", ord);
+  j.integer("count", n_);
+  j.integer("count_extended", 2ll * ord * n_);
+  j.str("generator", generator_);
+  j.null("grid");
+  j.integer("h", 0);
+  j.integer("seed", seed_);
+  j.integer("iteration", iteration_);
+  j.boolean("fixed_identity", fix_identity_);
+  const long id = n_ ? identity_index() : -1;
+  if (id >= 0)
+    j.integer("identity_index", id);
+  else
+    j.null("identity_index");
+  j.integer("base_count", base_count_);
+  j.integer("inserted_count", 0);
+  j.str("triangulation_backend", "none");
+  for (const char* k : {"simplex_bound_deg", "covering_excess", "mesh_ratio", "covering_radius_deg"}) j.null(k);
+  j.boolean("covering_current", false);
+  j.boolean("covering_cached", false);
+  for (const char* k : {"covering_unresolved_caps", "covering_iteration", "covering_witness",
+                        "covering_domain_bound_deg", "covering_shell_points", "covering_fence_points",
+                        "covering_tetrahedra", "covering_certificate_probes", "covering_exact_predicates",
+                        "covering_rounds"})
+    j.null(k);
+  j.boolean("measured", m);
+  j.boolean("measures_current", current);
+  if (m)
+    j.integer("measures_iteration", measure_iteration_);
+  else
+    j.null("measures_iteration");
+  j.boolean("energy_ratios_current", full_measures_ && current);
+  if (neighbourhood_ > 0)
+    j.num("relax_neighbourhood", neighbourhood_);
+  else
+    j.null("relax_neighbourhood");
+  if (point_separation_iteration_ >= 0)
+    j.integer("point_separation_iteration", point_separation_iteration_);
+  else
+    j.null("point_separation_iteration");
+  j.null("point_covering_iteration");
+  if (m && measure_neighbourhood_ > 0)
+    j.num("energy_neighbourhood", measure_neighbourhood_);
+  else
+    j.null("energy_neighbourhood");
+  j.integer("ratio_limit", ratio_limit());
+  std::string precision;
+  if (!m)
+    precision = "Not measured";
+  else if (!current)
+    precision = "Energy and separation from an earlier iteration";
+  else if (!std::isfinite(measures_.quey_energy))
+    precision = "Separation current; energy not measured";
+  else if (full_measures_)
+    precision = "All measures current";
+  else if (n_ > ratio_limit())
+    precision = "Energy and separation current; this sample exceeds the ratio limit";
+  else
+    precision = "Energy and separation current; refresh energy ratios";
+  j.str("precision", precision);
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  j.num("energy", m ? measures_.quey_energy : nan);
+  j.num("quey_energy", m ? measures_.quey_energy : nan);
+  j.num("riesz_1", m ? measures_.riesz_1 : nan);
+  j.num("riesz_2", m ? measures_.riesz_2 : nan);
+  j.num("riesz_3", m ? measures_.riesz_3 : nan);
+  j.num("green_ratio", m ? measures_.green_ratio : nan);
+  j.num("log_ratio", m ? measures_.log_ratio : nan);
+  j.num("separation_deg", m ? measures_.separation_deg : nan);
+  j.num("step_deg", step_deg_);
+  j.num("force_rms", m ? measures_.force_rms : force_rms_);
+  int dev = 0;
+  hipDeviceProp_t prop{};
+  std::string adapter = "HIP";
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    adapter = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+  j.str("adapter", adapter);
+  j.s += "}";
+  text = j.s;
+  return text;
+}
+
+}  // namespace zxt

@@ -63,7 +63,7 @@ struct NearField {
   // The last pass's sites, shell and overflow flag (valid after a wait).
   NearInfo status{};
   DevBuf planes, info, rim, uq, ucode, urank, ucell, perm, sq, scode, srank, offsets, cursor, src_cell, src_offsets,
-      src_cursor, order_buf, scan_tmp;
+      src_cursor, order_buf, scan_tmp, counters;
   NearBuffers view() const;
   void setup(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius);
   // Records sites and sorts for the positions q (setting up first when needed).
@@ -102,6 +102,7 @@ class Engine {
   void set_neighbourhood(double delta);
   void identity(bool add, bool fixed);
   void force_passes(uint32_t passes);
+  void force_rows(float* force4, double* energy, float* nearest);
   std::string stats();
   const float* point_values(uint32_t kind) const;
   const uint32_t* point_indices() const;
@@ -121,8 +122,10 @@ class Engine {
   std::string text;
   NearInfo last_near{};
   uint32_t last_near_sdim = 0;
+  unsigned long long last_near_counts[2] = {0, 0};  // accepted pairs, sites tested
 
   const float4* ops_device() const;
+  void* q_device() const;
   void sync();
   template <typename T>
   void download(std::vector<T>& dst, const DevBuf& src, size_t count) {
@@ -140,6 +143,7 @@ class Engine {
   long identity_index() const;
   void sync_neighbourhood();
   Measures relax_measure(RelaxState& rs, bool keep_separation);
+  void read_forces(RelaxState& rs, std::vector<float4>& f, std::vector<double>& en, std::vector<float>& nr);
   double nearest_pass(RelaxState& rs);
   Measures quick_measures(const std::vector<float4>& f, const std::vector<double>& energy,
                           const std::vector<float>& nearest) const;
@@ -157,7 +161,7 @@ class Engine {
   Measures measures_;
   uint32_t measure_iteration_ = 0;
   bool full_measures_ = false;
-  RelaxState relax_state_, spacing_state_;
+  RelaxState relax_state_, spacing_state_, measure_state_;
   double neighbourhood_ = 0, measure_neighbourhood_ = 0;
   std::vector<float> point_separation_;
   std::vector<uint32_t> point_index_;

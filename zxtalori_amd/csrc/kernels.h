@@ -135,6 +135,7 @@ struct NearBuffers {
   uint32_t* src_cursor = nullptr;   // kMortonCells + 1
   uint32_t* order = nullptr;     // n, sources in Morton order
   uint32_t* scan_tmp = nullptr;  // scan_tmp_words(max cells)
+  unsigned long long* counters = nullptr;  // [0] accepted pairs, [1] sites tested (force pass)
 };
 constexpr uint32_t kMortonBits = 6;
 constexpr uint32_t kMortonCells = 1u << (3 * kMortonBits);

@@ -830,7 +830,8 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
                                               const uint32_t* __restrict__ offsets, const NearInfo* __restrict__ info,
                                               uint32_t sdim, NearCut cut, uint32_t two_order, uint32_t real_blocks,
                                               Rows rows, float* __restrict__ chord_out,
-                                              uint32_t* __restrict__ rank_out) {
+                                              uint32_t* __restrict__ rank_out,
+                                              unsigned long long* __restrict__ counters) {
   __shared__ WaveLds lds[kBlock / 64];
   const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
   if (vb >= real_blocks) return;
@@ -863,13 +864,17 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   float nearest = 4.0f;
   uint64_t best = ~uint64_t(0);
   uint32_t qhead = 0, qcount = 0, rounds = 0;
+  uint32_t pairs = 0, candidates = 0;  // accepted pair evaluations (per lane), sites tested
 
   auto process = [&](uint32_t e) {
     const float4 y = L.qy[e];
     const uint32_t code = L.qc[e];
     const float dm2 = chord2(x, y), dp2 = chord2p(x, y);
     if (code != self_code) nearest = fminf(nearest, dm2);
-    if ((code & 0x7fffffffu) != src) pair_terms<true>(x, y, dm2, dp2, cut.taper_start, cut.inv_width, ps);
+    if ((code & 0x7fffffffu) != src) {
+      pair_terms<true>(x, y, dm2, dp2, cut.taper_start, cut.inv_width, ps);
+      ++pairs;
+    }
   };
   auto flush = [&]() {
     acc[0] += ps.fx;
@@ -897,6 +902,7 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
     }
     const uint32_t incl = wave_incl_scan(len);
     const uint32_t total = __shfl(incl, 63);
+    candidates += total;
     wave_sync();
     L.rstart[lane] = start;
     L.rexcl[lane] = incl - len;
@@ -957,6 +963,11 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
     double f0 = wave_sum(acc[0]), f1 = wave_sum(acc[1]), f2 = wave_sum(acc[2]), f3 = wave_sum(acc[3]);
     double e = wave_sum(acc[4]);
     nearest = wave_min(nearest);
+    pairs = wave_sum(pairs);
+    if (lane == 0) {
+      atomicAdd(&counters[0], (unsigned long long)pairs);
+      atomicAdd(&counters[1], (unsigned long long)candidates);
+    }
     if (lane == 0) {
       rows.force[src] = make_float4(float(f0), float(f1), float(f2), float(f3));
       rows.energy[src] = e;
@@ -974,9 +985,10 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
 void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, const NearBuffers& b, const NearCut& cut,
                        Rows out) {
   const uint32_t real = blocks_for(n, kBlock / 64);
+  launch_fill_u32(p, s, reinterpret_cast<uint32_t*>(b.counters), 0u, 4);
   p.run("near_force", [&] {
     hipLaunchKernelGGL(k_near<0>, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.srank,
-                       b.offsets, b.info, b.sdim, cut, 0u, real, out, nullptr, nullptr);
+                       b.offsets, b.info, b.sdim, cut, 0u, real, out, nullptr, nullptr, b.counters);
   });
 }
 
@@ -985,7 +997,7 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
   const uint32_t real = blocks_for(n, kBlock / 64);
   p.run("near_nearest", [&] {
     hipLaunchKernelGGL(k_near<1>, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.srank,
-                       b.offsets, b.info, b.sdim, cut, 2 * order, real, Rows{}, chord_out, rank_out);
+                       b.offsets, b.info, b.sdim, cut, 2 * order, real, Rows{}, chord_out, rank_out, b.counters);
   });
 }
 
