@@ -196,6 +196,10 @@ def test_measure_matches_dense(group):
         expected = dense.measures(grid.quaternions, grid.operations)
         for key in ("quey_energy", "separation_deg", "force_rms"):
             assert actual[key] == pytest.approx(expected[key], rel=1e-5), key
+        # The ratios are FP64 on the GPU: far tighter than the reference's 4e-4.
+        for key in ("riesz_1", "riesz_2", "riesz_3", "green_ratio", "log_ratio"):
+            assert actual[key] == pytest.approx(expected[key], rel=1e-9, abs=1e-12), key
+        assert actual["energy_ratios_current"] is True
         signed = grid.quaternions
         signed[::2] *= -1
         grid.set(signed, group)

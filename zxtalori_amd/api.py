@@ -218,7 +218,7 @@ class Grid:
         return self
 
     def measure(self):
-        """Quey energy and separation (Riesz, Green and log ratios are not computed)."""
+        """Quey energy and separation; below the ratio limit also the Riesz, Green and log ratios."""
         self._open()
         self._check(self._lib.zo_measure(self._handle))
         return self.stats

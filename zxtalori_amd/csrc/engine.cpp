@@ -457,6 +457,35 @@ void Engine::force_rows(float* force4, double* energy, float* nearest) {
   std::memcpy(nearest, nr.data(), size_t(n_) * 4);
 }
 
+// Riesz, Green and log ratios from the FP64 row sums, normalised as pairs.zig:142-173.
+void Engine::full_measure(Measures& m) {
+  const uint32_t ord = order();
+  const AllPairsPlan plan = plan_allpairs(n_, ord, pair_budget_scale);
+  measure_partial_.ensure(size_t(plan.splits()) * n_ * 5 * sizeof(double));
+  measure_out_.ensure(size_t(n_) * 5 * sizeof(double));
+  launch_measure(prof, stream, q_.as<float4>(), n_, ops_device(), ord, plan, measure_partial_.as<double>(),
+                 measure_out_.as<double>());
+  std::vector<double> rows;
+  download(rows, measure_out_, size_t(n_) * 5);
+  sync();
+  // The omitted identity self pair still has a real antipode (pairs.wgsl:231-236).
+  const double self[5] = {0.5, 0.25, 0.125, -0.4431471805599453, -3.0 / (8.0 * kPi * kPi)};
+  double sum[5] = {0, 0, 0, 0, 0};
+  for (uint32_t i = 0; i < n_; ++i)
+    for (int c = 0; c < 5; ++c) sum[c] += rows[5 * size_t(i) + c] + self[c];
+  const double extension = 2.0 * ord;
+  const double total = double(n_) * extension;
+  for (double& v : sum) v *= extension;
+  const double c_green =
+      std::pow(3.0, 5.0 / 3.0) / (5.0 * std::pow(2.0 * kPi * kPi, 1.0 / 3.0) * std::pow(4.0 * kPi, 2.0 / 3.0));
+  const double delta_log = (sum[3] + 0.25 * total + std::log(total) * total / 3.0) / total;
+  m.riesz_1 = sum[0] / (total * total * (8.0 / (3.0 * kPi)));
+  m.riesz_2 = sum[1] / (total * total);
+  m.riesz_3 = sum[2] / ((2.0 / (3.0 * kPi)) * total * total * std::log(total));
+  m.green_ratio = sum[4] / (-c_green * std::pow(total, 4.0 / 3.0));
+  m.log_ratio = delta_log / (-5.0 / 12.0);
+}
+
 void Engine::measure() {
   if (n_ == 0) fail("NoOrientations");
   if (n_ > ratio_limit()) {
@@ -468,7 +497,8 @@ void Engine::measure() {
     // neighbourhood (engine.zig:317-324).
     measure_state_.set_near(group_, nullptr);
     measures_ = relax_measure(measure_state_, true);
-    full_measures_ = false;
+    full_measure(measures_);
+    full_measures_ = true;
     sync_neighbourhood();
     if (neighbourhood_ > 0) relax_measure(relax_state_, true);
     measure_neighbourhood_ = 0;

@@ -143,6 +143,8 @@ class Engine {
   long identity_index() const;
   void sync_neighbourhood();
   Measures relax_measure(RelaxState& rs, bool keep_separation);
+  void full_measure(Measures& m);
+  DevBuf measure_partial_, measure_out_;
   void read_forces(RelaxState& rs, std::vector<float4>& f, std::vector<double>& en, std::vector<float>& nr);
   double nearest_pass(RelaxState& rs);
   Measures quick_measures(const std::vector<float4>& f, const std::vector<double>& energy,

@@ -110,6 +110,9 @@ void launch_allpairs(Prof& p, hipStream_t s, const float4* q, uint32_t n, const 
                      const AllPairsPlan& plan, double* partial, float* partial_nearest, Rows out);
 void launch_allpairs_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                              uint64_t* partial, float* chord_out, uint32_t* rank_out);
+// FP64 Riesz 1-3, log and Green row sums (out: n x 5), all pairs, self pair excluded.
+void launch_measure(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                    const AllPairsPlan& plan, double* partial, double* out);
 
 // Near field.  Buffers are owned by the engine's NearField (engine.cpp).
 struct NearBuffers {

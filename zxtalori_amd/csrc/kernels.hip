@@ -460,6 +460,99 @@ void launch_allpairs_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n
   });
 }
 
+// ---- full measure (pairs.wgsl:151-244) ----------------------------------------------
+// Riesz s=1,2,3, Green and log sums over every orbit point but the source itself, in
+// native FP64 (the reference carries the log in double-single arithmetic).  Images and
+// chords are FP64 from the FP32 positions, normalised as the reference's dense model
+// does.  Partial sums per (split, source): r1, r2, r3, log, green.
+__device__ inline double4 qmul_d(const double4 a, const double4 b) {
+  return make_double4(a.x * b.x - a.y * b.y - a.z * b.z - a.w * b.w, a.x * b.y + b.x * a.y + a.z * b.w - a.w * b.z,
+                      a.x * b.z + b.x * a.z + a.w * b.y - a.y * b.w, a.x * b.w + b.x * a.w + a.y * b.z - a.z * b.y);
+}
+__device__ inline double4 normalize_d(const double4 v) {
+  const double s = 1.0 / sqrt(v.x * v.x + v.y * v.y + v.z * v.z + v.w * v.w);
+  return make_double4(v.x * s, v.y * s, v.z * s, v.w * s);
+}
+__device__ inline double4 to_d(const float4 v) { return make_double4(v.x, v.y, v.z, v.w); }
+
+__global__ __launch_bounds__(256) void k_measure(const float4* __restrict__ q, uint32_t n,
+                                                 const float4* __restrict__ ops, uint32_t order,
+                                                 uint32_t ops_per_split, uint32_t site_chunk, uint32_t site_splits,
+                                                 double* __restrict__ partial) {
+  __shared__ double4 tile[kBlock];
+  const uint32_t split = blockIdx.y;
+  const uint32_t k0 = (split / site_splits) * ops_per_split, k1 = min(order, k0 + ops_per_split);
+  const uint32_t j0 = (split % site_splits) * site_chunk, j1 = min(n, j0 + site_chunk);
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool live = i < n;
+  const double4 x = normalize_d(to_d(live ? q[i] : make_float4(1.0f, 0.0f, 0.0f, 0.0f)));
+  double r1 = 0.0, r2 = 0.0, r3 = 0.0, lg = 0.0, green = 0.0;
+  constexpr double kPiD = 3.14159265358979323846;
+  constexpr double kInvFourPi2 = 1.0 / (4.0 * kPiD * kPiD);
+  for (uint32_t k = k0; k < k1; ++k) {
+    const double4 op = normalize_d(to_d(ops[k]));
+    for (uint32_t start = j0; start < j1; start += kBlock) {
+      const uint32_t j = start + threadIdx.x;
+      if (j < j1) {
+        const double4 y = to_d(q[j]);
+        tile[threadIdx.x] = normalize_d(k == 0 ? y : qmul_d(y, op));
+      }
+      __syncthreads();
+      if (live) {
+        const uint32_t cnt = min(kBlock, j1 - start);
+        for (uint32_t t = 0; t < cnt; ++t) {
+          if (k == 0 && start + t == i) continue;
+          const double4 y = tile[t];
+          const double mx = x.x - y.x, my = x.y - y.y, mz = x.z - y.z, mw = x.w - y.w;
+          const double px = x.x + y.x, py = x.y + y.y, pz = x.z + y.z, pw = x.w + y.w;
+          const double dm2 = mx * mx + my * my + mz * mz + mw * mw;
+          const double dp2 = px * px + py * py + pz * pz + pw * pw;
+          const double im = rsqrt(dm2), ip = rsqrt(dp2);
+          const double im2 = im * im, ip2 = ip * ip;
+          r1 += im + ip;
+          r2 += im2 + ip2;
+          r3 += im2 * im + ip2 * ip;
+          lg += 0.5 - 0.5 * log(dm2 * dp2);
+          const double a = 2.0 * atan(sqrt(fmin(dm2, dp2) / fmax(dm2, dp2)));
+          const double cot = fabs(dp2 - dm2) * 0.5 * im * ip;
+          green += ((kPiD - 2.0 * a) * cot - 1.0) * kInvFourPi2;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (live) {
+    double* o = partial + 5 * (size_t(split) * n + i);
+    o[0] = r1;
+    o[1] = r2;
+    o[2] = r3;
+    o[3] = lg;
+    o[4] = green;
+  }
+}
+
+__global__ void k_measure_reduce(const double* __restrict__ partial, uint32_t n, uint32_t splits,
+                                 double* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (uint32_t s = 0; s < splits; ++s)
+    for (int c = 0; c < 5; ++c) a[c] += partial[5 * (size_t(s) * n + i) + c];
+  for (int c = 0; c < 5; ++c) out[5 * size_t(i) + c] = a[c];
+}
+
+void launch_measure(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                    const AllPairsPlan& plan, double* partial, double* out) {
+  const uint32_t splits = plan.splits();
+  p.run("measure", [&] {
+    hipLaunchKernelGGL(k_measure, dim3(blocks_for(n, kBlock), splits), dim3(kBlock), 0, s, q, n, ops, order,
+                       plan.ops_per_split, plan.site_chunk, plan.site_splits, partial);
+  });
+  p.run("measure_reduce", [&] {
+    hipLaunchKernelGGL(k_measure_reduce, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, partial, n, splits, out);
+  });
+}
+
 // ---- scan (exclusive, in place; data[count] receives the total) -----------------------
 constexpr uint32_t kScanItems = 8;
 constexpr uint32_t kScanTile = kBlock * kScanItems;
