@@ -213,7 +213,7 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--group", default="I")
     ap.add_argument("--delta", type=float, default=20.0)
-    ap.add_argument("--cpu-sample", type=int, default=int(os.environ.get("ZXT_CPU_SAMPLE", 30000)))
+    ap.add_argument("--cpu-sample", type=int, default=int(os.environ.get("ZXT_CPU_SAMPLE", 600000)))
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
