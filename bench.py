@@ -215,7 +215,19 @@ def main():
     ap.add_argument("--delta", type=float, default=20.0)
     ap.add_argument("--cpu-sample", type=int, default=int(os.environ.get("ZXT_CPU_SAMPLE", 600000)))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run only a plain device copy of MIB MiB (the HBM counter calibration kernel)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the near-field pass is not sharded "
                                    "across GPUs here (see DESIGN.md for how it would shard)"}))
