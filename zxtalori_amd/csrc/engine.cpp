@@ -69,6 +69,7 @@ NearBuffers NearField::view() const {
   b.planes = planes.as<float4>();
   b.nplanes = nplanes;
   b.band = band;
+  b.reach = float(radius + 1e-4);
   b.capacity = capacity;
   b.sdim = sdim;
   b.info = info.as<NearInfo>();
@@ -89,13 +90,14 @@ NearBuffers NearField::view() const {
   b.order = order_buf.as<uint32_t>();
   b.scan_tmp = scan_tmp.as<uint32_t>();
   b.counters = counters.as<unsigned long long>();
+  b.counts = counts.as<uint32_t>();
   return b;
 }
 
 static void release_near(NearField& f) {
   for (DevBuf* b : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
                     &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor, &f.order_buf,
-                    &f.scan_tmp, &f.counters})
+                    &f.scan_tmp, &f.counters, &f.counts})
     b->release();
   f.ready = false;
 }
@@ -176,6 +178,7 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   order_buf.ensure(size_t(n) * 4);
   scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, kMortonCells)) * 4);
   counters.ensure(4 * sizeof(unsigned long long));
+  counts.ensure(size_t(n) * 8);
   ready = true;
 }
 

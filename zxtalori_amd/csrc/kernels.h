@@ -35,7 +35,7 @@ struct NearInfo {
   uint32_t sites;     // sites written (core + shell), may exceed capacity
   uint32_t overflow;  // set when sites > capacity
   uint32_t rim;       // rim sources listed
-  uint32_t pad0;
+  uint32_t wmin_key;  // order-preserving key of the smallest source scalar part
   uint32_t lo[3];
   uint32_t hi[3];
   float lower[3];
@@ -119,6 +119,7 @@ struct NearBuffers {
   const float4* planes = nullptr;
   uint32_t nplanes = 0;
   float band = 0;        // sin(r + 1e-4): rim and shell margin (shell.wgsl:19-22)
+  float reach = 0;       // r + 1e-4: the angle within which a site can matter
   uint32_t capacity = 0;
   uint32_t sdim = 1;
   NearInfo* info = nullptr;
@@ -139,6 +140,7 @@ struct NearBuffers {
   uint32_t* order = nullptr;     // n, sources in Morton order
   uint32_t* scan_tmp = nullptr;  // scan_tmp_words(max cells)
   unsigned long long* counters = nullptr;  // [0] accepted pairs, [1] sites tested (force pass)
+  uint32_t* counts = nullptr;    // 2n: per source, pairs and sites tested of the last force pass
 };
 constexpr uint32_t kMortonBits = 6;
 constexpr uint32_t kMortonCells = 1u << (3 * kMortonBits);

@@ -633,6 +633,7 @@ __global__ void k_near_reset(NearInfo* info, uint32_t n) {
   info->sites = n;
   info->overflow = 0;
   info->rim = 0;
+  info->wmin_key = 0xffffffffu;
   for (int c = 0; c < 3; ++c) {
     info->lo[c] = 0xffffffffu;
     info->hi[c] = 0u;
@@ -680,10 +681,22 @@ __device__ inline void box_add(uint32_t lo[3], uint32_t hi[3], const float4 y) {
 
 // Core sites (every source's identity image, at its own index) and the rim list: the
 // sources within the band of some zone plane (shell.wgsl:68-96).
+__device__ inline void block_min_commit(uint32_t* dst, uint32_t v) {
+  __shared__ uint32_t sm[kBlock / 64];
+  for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) v = min(v, sm[w]);
+    if (v != 0xffffffffu) atomicMin(dst, v);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, uint32_t n,
                                                   const float4* __restrict__ planes, uint32_t nplanes, float band,
                                                   uint32_t two_order, NearBuffers b) {
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+  uint32_t wmin = 0xffffffffu;
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t rounds = (n + stride - 1) / stride;
   for (uint32_t r = 0; r < rounds; ++r) {
@@ -695,6 +708,7 @@ __global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, 
       b.ucode[i] = i | 0x80000000u;
       b.urank[i] = i * two_order;
       box_add(lo, hi, x);
+      wmin = min(wmin, ordered_key(x.x));
       for (uint32_t k = 0; k < nplanes; ++k) {
         if (dot4(x, planes[k]) <= band + 2e-6f) {
           rim = true;
@@ -706,6 +720,7 @@ __global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, 
     if (rim) b.rim[slot] = i;
   }
   box_reduce_commit(b.info, lo, hi);
+  block_min_commit(&b.info->wmin_key, wmin);
 }
 
 // Images (k, sign) != (identity, +) of the rim sources within the band of the zone's
@@ -718,6 +733,10 @@ __global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uin
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   const uint32_t m = b.info->rim;
   const uint32_t per = 2 * order - 1;
+  // A site matters only within angle r of some source, so within acos(min source w) + r
+  // of the identity: its scalar part is at least w_floor.  Cheap (no normalisation), and
+  // exact as a necessary condition, it spares most images the plane tests.
+  const float w_floor = cosf(fminf(3.14159265f, acosf(fminf(1.0f, unordered_key(b.info->wmin_key))) + b.reach)) - 1e-6f;
   const uint64_t total = uint64_t(m) * per;
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
   const uint64_t rounds = (total + stride - 1) / stride;
@@ -730,9 +749,14 @@ __global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uin
       src = b.rim[t / per];
       img = uint32_t(t % per) + 1u;
       const uint32_t k = img >> 1;
-      y = image(q[src], ops[k], k);
-      if (img & 1u) y = neg4(y);
-      keep = y.x > -0.9999999f;
+      const float4 x = q[src], op = ops[k];
+      const float w = fmaf(-x.w, op.w, fmaf(-x.z, op.z, fmaf(-x.y, op.y, x.x * op.x)));
+      keep = ((img & 1u) ? -w : w) >= w_floor;
+      if (keep) {
+        y = image(x, op, k);
+        if (img & 1u) y = neg4(y);
+        keep = y.x > -0.9999999f;
+      }
       for (uint32_t pk = 0; keep && pk < nplanes; ++pk)
         if (dot4(y, planes[pk]) < -band - 2e-6f) keep = false;
     }
@@ -901,9 +925,7 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 // ---- near-field rows (pairs.wgsl:363-427) -------------------------------------------
 struct WaveLds {
   float4 qy[128];
-  uint32_t qc[128];
-  uint32_t rstart[64];
-  uint32_t rexcl[64];
+  float qd[128];
 };
 
 __device__ inline float row_gap(float c, int idx, uint32_t dim) {
@@ -916,6 +938,13 @@ __device__ inline float row_gap(float c, int idx, uint32_t dim) {
 
 // MODE 0: force rows.  MODE 1: nearest orbit point (chord^2, rank), the identity image
 // of the source excluded.
+//
+// Per source (one wave): up to 64 cell rows at a time, one row per lane, each trimmed in x
+// to the chart ball of the source's cap; their site runs form one index space that the
+// wave walks 64 candidates at a time.  A lane finds its candidate's row by walking the row
+// boundaries of the chunk with readlane (uniform, ~1.5 boundaries per chunk).  Candidates
+// within the cutoff update the nearest chord; those outside the source's own orbit enter
+// an LDS ring, and each full 64 runs the pair kernel on a full wave.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint32_t n,
                                               const uint32_t* __restrict__ order, const float4* __restrict__ sq,
@@ -923,8 +952,7 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
                                               const uint32_t* __restrict__ offsets, const NearInfo* __restrict__ info,
                                               uint32_t sdim, NearCut cut, uint32_t two_order, uint32_t real_blocks,
                                               Rows rows, float* __restrict__ chord_out,
-                                              uint32_t* __restrict__ rank_out,
-                                              unsigned long long* __restrict__ counters) {
+                                              uint32_t* __restrict__ rank_out, uint32_t* __restrict__ counts) {
   __shared__ WaveLds lds[kBlock / 64];
   const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
   if (vb >= real_blocks) return;
@@ -960,14 +988,8 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   uint32_t pairs = 0, candidates = 0;  // accepted pair evaluations (per lane), sites tested
 
   auto process = [&](uint32_t e) {
-    const float4 y = L.qy[e];
-    const uint32_t code = L.qc[e];
-    const float dm2 = chord2(x, y), dp2 = chord2p(x, y);
-    if (code != self_code) nearest = fminf(nearest, dm2);
-    if ((code & 0x7fffffffu) != src) {
-      pair_terms<true>(x, y, dm2, dp2, cut.taper_start, cut.inv_width, ps);
-      ++pairs;
-    }
+    near_pair_terms(x, L.qy[e], L.qd[e], cut.taper_start, cut.inv_width, ps);
+    ++pairs;
   };
   auto flush = [&]() {
     acc[0] += ps.fx;
@@ -995,37 +1017,44 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
     }
     const uint32_t incl = wave_incl_scan(len);
     const uint32_t total = __shfl(incl, 63);
+    const uint32_t excl = incl - len;  // rows past nrows: len 0, excl = total
     candidates += total;
-    wave_sync();
-    L.rstart[lane] = start;
-    L.rexcl[lane] = incl - len;
-    wave_sync();
+    uint32_t row = 0;  // uniform: the row holding the chunk's first candidate
     for (uint32_t cb = 0; cb < total; cb += 64) {
+      while (row < 63 && uint32_t(__builtin_amdgcn_readlane(int(excl), int(row + 1))) <= cb) ++row;
       const uint32_t v = cb + lane;
-      const bool valid = v < total;
-      uint32_t site = 0;
-      if (valid) {
-        uint32_t rr = 0;
-#pragma unroll
-        for (uint32_t step = 32; step > 0; step >>= 1)
-          if (L.rexcl[rr + step] <= v) rr += step;
-        site = L.rstart[rr] + (v - L.rexcl[rr]);
+      uint32_t mstart = uint32_t(__builtin_amdgcn_readlane(int(start), int(row)));
+      uint32_t mexcl = uint32_t(__builtin_amdgcn_readlane(int(excl), int(row)));
+      for (uint32_t rr = row + 1; rr < 64; ++rr) {
+        const uint32_t e = uint32_t(__builtin_amdgcn_readlane(int(excl), int(rr)));
+        if (e > cb + 63) break;
+        if (v >= e) {
+          mstart = uint32_t(__builtin_amdgcn_readlane(int(start), int(rr)));
+          mexcl = e;
+        }
       }
+      const bool valid = v < total;
+      const uint32_t site = mstart + (v - mexcl);
       if (MODE == 0) {
         float4 y = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        uint32_t code = 0;
+        float dm2 = 4.0f;
         bool accept = false;
         if (valid) {
           y = sq[site];
-          code = scode[site];
-          accept = chord2(x, y) <= cut.chord2;
+          const uint32_t code = scode[site];
+          dm2 = chord2(x, y);
+          const bool inside = dm2 <= cut.chord2;
+          // Every orbit point but the source itself bounds the nearest chord; the
+          // source's own images exert no force (pairs.wgsl:404-408).
+          if (inside && code != self_code) nearest = fminf(nearest, dm2);
+          accept = inside && (code & 0x7fffffffu) != src;
         }
         const uint64_t mask = __ballot(accept);
         if (mask) {
           if (accept) {
             const uint32_t e = (qhead + qcount + mbcnt(mask)) & 127u;
             L.qy[e] = y;
-            L.qc[e] = code;
+            L.qd[e] = dm2;
           }
           qcount += uint32_t(__popcll(mask));
           wave_sync();
@@ -1058,13 +1087,11 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
     nearest = wave_min(nearest);
     pairs = wave_sum(pairs);
     if (lane == 0) {
-      atomicAdd(&counters[0], (unsigned long long)pairs);
-      atomicAdd(&counters[1], (unsigned long long)candidates);
-    }
-    if (lane == 0) {
       rows.force[src] = make_float4(float(f0), float(f1), float(f2), float(f3));
       rows.energy[src] = e;
       rows.nearest[src] = nearest;
+      counts[2 * size_t(src)] = pairs;
+      counts[2 * size_t(src) + 1] = candidates;
     }
   } else {
     best = wave_min_u64(best);
@@ -1075,13 +1102,33 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   }
 }
 
+// Sums the per-source pair and candidate counts of a force pass into counters[0..1].
+__global__ __launch_bounds__(256) void k_count_reduce(const uint32_t* __restrict__ counts, uint32_t n,
+                                                      unsigned long long* __restrict__ counters) {
+  unsigned long long a = 0, b = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    a += counts[2 * size_t(i)];
+    b += counts[2 * size_t(i) + 1];
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&counters[0], a);
+    atomicAdd(&counters[1], b);
+  }
+}
+
 void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, const NearBuffers& b, const NearCut& cut,
                        Rows out) {
   const uint32_t real = blocks_for(n, kBlock / 64);
-  launch_fill_u32(p, s, reinterpret_cast<uint32_t*>(b.counters), 0u, 4);
   p.run("near_force", [&] {
     hipLaunchKernelGGL(k_near<0>, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.srank,
-                       b.offsets, b.info, b.sdim, cut, 0u, real, out, nullptr, nullptr, b.counters);
+                       b.offsets, b.info, b.sdim, cut, 0u, real, out, nullptr, nullptr, b.counts);
+  });
+  launch_fill_u32(p, s, reinterpret_cast<uint32_t*>(b.counters), 0u, 4);
+  p.run("count_reduce", [&] {
+    hipLaunchKernelGGL(k_count_reduce, dim3(std::min<uint32_t>(256, blocks_for(n, kBlock))), dim3(kBlock), 0, s,
+                       b.counts, n, b.counters);
   });
 }
 
@@ -1090,7 +1137,7 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
   const uint32_t real = blocks_for(n, kBlock / 64);
   p.run("near_nearest", [&] {
     hipLaunchKernelGGL(k_near<1>, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.srank,
-                       b.offsets, b.info, b.sdim, cut, 2 * order, real, Rows{}, chord_out, rank_out, b.counters);
+                       b.offsets, b.info, b.sdim, cut, 2 * order, real, Rows{}, chord_out, rank_out, nullptr);
   });
 }
 

@@ -123,4 +123,38 @@ __device__ __forceinline__ void pair_terms(const float4 x, const float4 y, float
   s.energy += energy;
 }
 
+// The near-field pair for a site within the cutoff: dm2 <= chord2 < 2, so y is the nearer
+// of +-y (sign +) and dp2 = 4 - dm2 for unit quaternions; then near + far = 4 and
+// 1/sin a = 2*rsq(near*far).  Otherwise the arithmetic of pair_terms<true>.
+__device__ __forceinline__ void near_pair_terms(const float4 x, const float4 y, float dm2, float taper_start,
+                                                float inv_width, PairSum& s) {
+  const float nr = fmaxf(dm2, 1e-30f);
+  const float fr = 4.0f - dm2;
+  const float R = __builtin_amdgcn_rsqf(nr * fr);
+  const float t = nr * R;
+  const float a = fmaxf(2.0f * atan01(t), 1e-12f);
+  const float b = kPiF - a;
+  const float aa = a * a, bb = b * b;
+  const float a2 = aa + 1e-9f, b2 = bb + 1e-9f;
+  const float a3 = fmaf(aa, a, 1e-9f), b3 = fmaf(bb, b, 1e-9f);
+  const float sine = fmaf(0.5f * t, fr, 1e-9f);
+  const float P1 = a2 * b2 * sine, P2 = a3 * b3, P3 = a * b;
+  const float P12 = P1 * P2;
+  const float D = __builtin_amdgcn_rcpf(P12 * P3);
+  const float iP1 = D * (P2 * P3), iP2 = D * (P1 * P3), iP3 = D * P12;
+  const float energy = kPiF * iP3;
+  const float u = fminf(fmaxf((a - taper_start) * inv_width, 0.0f), 1.0f);
+  const float w = fmaf(u * u, fmaf(2.0f, u, -3.0f), 1.0f);
+  const float slope = -6.0f * inv_width * (u * (1.0f - u));
+  const float coef = fmaf(w, (a2 - b2) * iP1, energy * slope * (2.0f * R));
+  const float d1 = fmaf(x.x, y.y, -y.x * x.y) + fmaf(x.z, y.w, -x.w * y.z);
+  const float d2 = fmaf(x.x, y.z, -y.x * x.z) + fmaf(x.w, y.y, -x.y * y.w);
+  const float d3 = fmaf(x.x, y.w, -y.x * x.w) + fmaf(x.y, y.z, -x.z * y.y);
+  s.fx = fmaf(coef, d1, s.fx);
+  s.fy = fmaf(coef, d2, s.fy);
+  s.fz = fmaf(coef, d3, s.fz);
+  s.curvature = fmaf((a3 + b3) * iP2, w, s.curvature);
+  s.energy = fmaf(energy, w, s.energy);
+}
+
 }  // namespace zxt
