@@ -723,9 +723,31 @@ __global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, 
   block_min_commit(&b.info->wmin_key, wmin);
 }
 
-// Images (k, sign) != (identity, +) of the rim sources within the band of the zone's
-// closed halfspaces, appended as shell sites (shell.wgsl:19-39, :97-130).  The append
-// order varies; the cell sort's in-cell key order makes the result deterministic.
+// Whether image img (1..2|K|-1: operator img/2, negated when odd) of x is a shell site;
+// the image in y.  A site matters only within angle r of some source, so within
+// acos(min source w) + r of the identity: its scalar part is at least w_floor (a cheap,
+// exact necessary condition that spares most images the normalisation and plane tests).
+// Then the reference's test: within the band of every zone halfspace (shell.wgsl:19-22).
+__device__ inline bool shell_image(const float4 x, uint32_t img, const float4* __restrict__ ops,
+                                   const float4* __restrict__ planes, uint32_t nplanes, float band, float w_floor,
+                                   float4& y) {
+  const uint32_t k = img >> 1;
+  const float4 op = ops[k];
+  const float w = fmaf(-x.w, op.w, fmaf(-x.z, op.z, fmaf(-x.y, op.y, x.x * op.x)));
+  if (((img & 1u) ? -w : w) < w_floor) return false;
+  y = image(x, op, k);
+  if (img & 1u) y = neg4(y);
+  if (!(y.x > -0.9999999f)) return false;
+  for (uint32_t pk = 0; pk < nplanes; ++pk)
+    if (dot4(y, planes[pk]) < -band - 2e-6f) return false;
+  return true;
+}
+
+// Images (k, sign) != (identity, +) of the rim sources that are shell sites, appended
+// after the core (shell.wgsl:97-130).  One thread per rim source: it counts its sites, the
+// wave reserves room for all of its lanes' sites with one atomic, then each thread writes
+// its own.  The append order varies; the cell sort's in-cell key order makes the result
+// deterministic.
 __global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uint32_t n,
                                                const float4* __restrict__ ops, uint32_t order,
                                                const float4* __restrict__ planes, uint32_t nplanes, float band,
@@ -733,42 +755,38 @@ __global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uin
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   const uint32_t m = b.info->rim;
   const uint32_t per = 2 * order - 1;
-  // A site matters only within angle r of some source, so within acos(min source w) + r
-  // of the identity: its scalar part is at least w_floor.  Cheap (no normalisation), and
-  // exact as a necessary condition, it spares most images the plane tests.
-  const float w_floor = cosf(fminf(3.14159265f, acosf(fminf(1.0f, unordered_key(b.info->wmin_key))) + b.reach)) - 1e-6f;
-  const uint64_t total = uint64_t(m) * per;
-  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  const uint64_t rounds = (total + stride - 1) / stride;
-  for (uint64_t r = 0; r < rounds; ++r) {
-    const uint64_t t = r * stride + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    bool keep = false;
-    float4 y = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    uint32_t src = 0, img = 0;
-    if (t < total) {
-      src = b.rim[t / per];
-      img = uint32_t(t % per) + 1u;
-      const uint32_t k = img >> 1;
-      const float4 x = q[src], op = ops[k];
-      const float w = fmaf(-x.w, op.w, fmaf(-x.z, op.z, fmaf(-x.y, op.y, x.x * op.x)));
-      keep = ((img & 1u) ? -w : w) >= w_floor;
-      if (keep) {
-        y = image(x, op, k);
-        if (img & 1u) y = neg4(y);
-        keep = y.x > -0.9999999f;
-      }
-      for (uint32_t pk = 0; keep && pk < nplanes; ++pk)
-        if (dot4(y, planes[pk]) < -band - 2e-6f) keep = false;
-    }
-    const uint32_t slot = wave_append(&b.info->sites, keep);
-    if (keep) {
-      if (slot < b.capacity) {
-        b.uq[slot] = y;
-        b.ucode[slot] = src;
-        b.urank[slot] = src * (2 * order) + img;
-        box_add(lo, hi, y);
-      } else {
-        b.info->overflow = 1u;
+  const float w_floor =
+      cosf(fminf(3.14159265f, acosf(fminf(1.0f, unordered_key(b.info->wmin_key))) + b.reach)) - 1e-6f;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t rounds = (m + stride - 1) / stride;
+  const uint32_t lane = lane_id();
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t t = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < m;
+    const uint32_t src = live ? b.rim[t] : 0u;
+    const float4 x = live ? q[src] : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t mine = 0;
+    float4 y;
+    if (live)
+      for (uint32_t img = 1; img <= per; ++img) mine += shell_image(x, img, ops, planes, nplanes, band, w_floor, y);
+    const uint32_t incl = wave_incl_scan(mine);
+    const uint32_t total = __shfl(incl, 63);
+    uint32_t base = 0;
+    if (lane == 63 && total) base = atomicAdd(&b.info->sites, total);
+    base = __shfl(base, 63) + incl - mine;
+    if (live && mine) {
+      uint32_t slot = base;
+      for (uint32_t img = 1; img <= per; ++img) {
+        if (!shell_image(x, img, ops, planes, nplanes, band, w_floor, y)) continue;
+        if (slot < b.capacity) {
+          b.uq[slot] = y;
+          b.ucode[slot] = src;
+          b.urank[slot] = src * (2 * order) + img;
+          box_add(lo, hi, y);
+        } else {
+          b.info->overflow = 1u;
+        }
+        ++slot;
       }
     }
   }
