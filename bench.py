@@ -198,10 +198,13 @@ def cpu_baseline(q, ops, cut, radius, sample):
 
 
 def traffic_from_profiles():
-    path = ROOT / "profiles" / "near_force_i1e6_traffic.json"
+    """HBM bytes per near-field force launch from the committed counter summary
+    (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a copy kernel of known size)."""
+    path = ROOT / "profiles" / "near_force_i1e6_traffic.md"
     if path.is_file():
-        data = json.loads(path.read_text())
-        return data.get("hbm_bytes_per_launch")
+        for line in path.read_text().splitlines():
+            if line.startswith("hbm_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
     return None
 
 

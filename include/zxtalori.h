@@ -103,6 +103,45 @@ int32_t zo_near_pairs(zxt_engine* e, uint64_t* out2);
 /* The last near-field pass's site count, shell count and cell grid dimension. */
 int32_t zo_near_info(zxt_engine* e, uint32_t* out3);
 
+/* ---- Batches: many samples relaxed together ------------------------------------------
+ * No counterpart in the reference, which runs one sample per engine and one call at a time
+ * (src/native_api.zig:9-19, python/zxtalori/api.py:95-122).  A batch holds jobs, each a
+ * sample with its own resident state; zo_batch_step advances every running relax job by
+ * one BB step with the all-pairs passes, BB steps and folds of all jobs as one launch per
+ * stage (near-field jobs add their own passes), so a job's result is bit-identical alone
+ * and in any mix.  Same int32 / error-string rules as the engine calls; jobs are uint64
+ * handles.  One thread per batch. */
+typedef struct zxt_batch zxt_batch;
+/* flags bit 0: replay steady-state steps (same running jobs) from a captured HIP graph.
+ * NULL on failure, message in zo_create_error(). */
+zxt_batch* zo_batch_create(uint32_t flags);
+void zo_batch_destroy(zxt_batch* b);
+const char* zo_batch_error(zxt_batch* b);
+/* kind 0 relax (`steps` BB steps, at most 1000, then the end-of-relax measure), 1 measure,
+ * 2 spacing.  q: n rows of (w,x,y,z) as zo_set takes them, or NULL for n seeded Haar
+ * orientations (zo_random).  delta: the relax neighbourhood (0 all pairs, else 2..1000). */
+int32_t zo_batch_add(zxt_batch* b, uint32_t group, const float* q, uint32_t n, uint32_t seed, float delta,
+                     uint32_t kind, uint32_t steps, uint64_t* id_out);
+/* One step of every running job (pending measure and spacing jobs run first); completed
+ * receives how many jobs ended (done or failed) in it. */
+int32_t zo_batch_step(zxt_batch* b, uint32_t* completed);
+/* 0 running, 1 done, 2 failed (zo_batch_job_error), -1 unknown id. */
+int32_t zo_batch_status(zxt_batch* b, uint64_t id);
+const char* zo_batch_job_error(zxt_batch* b, uint64_t id);
+/* A job's engine views, as zo_stats / zo_quaternions / zo_count / zo_point_values /
+ * zo_point_indices give them for an engine; valid until the job's next step or removal. */
+const char* zo_batch_stats(zxt_batch* b, uint64_t id);
+const float* zo_batch_quaternions(zxt_batch* b, uint64_t id);
+uint32_t zo_batch_count(zxt_batch* b, uint64_t id);
+const float* zo_batch_point_values(zxt_batch* b, uint64_t id, uint32_t kind);
+const uint32_t* zo_batch_point_indices(zxt_batch* b, uint64_t id);
+/* Device bytes a job holds now (admission by HBM budget). */
+uint64_t zo_batch_bytes(zxt_batch* b, uint64_t id);
+int32_t zo_batch_remove(zxt_batch* b, uint64_t id);
+uint32_t zo_batch_running(zxt_batch* b);
+/* out2[0]: steps captured into a graph, out2[1]: steps replayed from one. */
+int32_t zo_batch_graphs(zxt_batch* b, uint64_t* out2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,23 @@ def _library():
         "zo_near_info": ([ptr, C.POINTER(C.c_uint32)], C.c_int),
         "zo_force_rows": ([ptr, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
         "zo_near_pairs": ([ptr, C.POINTER(C.c_uint64)], C.c_int),
+        # batches (no reference counterpart)
+        "zo_batch_create": ([C.c_uint32], ptr), "zo_batch_destroy": ([ptr], None),
+        "zo_batch_error": ([ptr], C.c_char_p),
+        "zo_batch_add": ([ptr, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32,
+                          C.c_uint32, C.POINTER(C.c_uint64)], C.c_int),
+        "zo_batch_step": ([ptr, C.POINTER(C.c_uint32)], C.c_int),
+        "zo_batch_status": ([ptr, C.c_uint64], C.c_int),
+        "zo_batch_job_error": ([ptr, C.c_uint64], C.c_char_p),
+        "zo_batch_stats": ([ptr, C.c_uint64], C.c_char_p),
+        "zo_batch_quaternions": ([ptr, C.c_uint64], C.POINTER(C.c_float)),
+        "zo_batch_count": ([ptr, C.c_uint64], C.c_uint32),
+        "zo_batch_point_values": ([ptr, C.c_uint64, C.c_uint32], C.POINTER(C.c_float)),
+        "zo_batch_point_indices": ([ptr, C.c_uint64], C.POINTER(C.c_uint32)),
+        "zo_batch_bytes": ([ptr, C.c_uint64], C.c_uint64),
+        "zo_batch_remove": ([ptr, C.c_uint64], C.c_int),
+        "zo_batch_running": ([ptr], C.c_uint32),
+        "zo_batch_graphs": ([ptr, C.POINTER(C.c_uint64)], C.c_int),
     }
     for name, (args, result) in signatures.items():
         function = getattr(lib, name)

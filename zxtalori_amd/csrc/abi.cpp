@@ -7,11 +7,18 @@
 #include <string>
 
 #include "../../include/zxtalori.h"
+#include "batch.h"
 #include "engine.h"
 #include "groups_table.h"
 
 struct zxt_engine {
   zxt::Engine engine;
+};
+
+struct zxt_batch {
+  explicit zxt_batch(bool graphs) : batch(graphs) {}
+  zxt::Batch batch;
+  std::string text;
 };
 
 namespace {
@@ -30,6 +37,21 @@ int32_t guarded(zxt_engine* h, const char* prefix, F&& f) {
     h->engine.err = std::string(prefix) + ": OutOfMemory";
   } catch (...) {
     h->engine.err = std::string(prefix) + ": Unexpected";
+  }
+  return -1;
+}
+
+template <typename F>
+int32_t guarded_batch(zxt_batch* b, const char* prefix, F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const zxt::EngineError& err) {
+    b->batch.err = std::string(prefix) + ": " + err.message;
+  } catch (const std::bad_alloc&) {
+    b->batch.err = std::string(prefix) + ": OutOfMemory";
+  } catch (...) {
+    b->batch.err = std::string(prefix) + ": Unexpected";
   }
   return -1;
 }
@@ -176,6 +198,107 @@ int32_t zo_near_info(zxt_engine* e, uint32_t* out3) {
   out3[0] = e->engine.last_near.sites;
   out3[1] = e->engine.last_near.sites >= e->engine.count() ? e->engine.last_near.sites - e->engine.count() : 0;
   out3[2] = e->engine.last_near_sdim;
+  return 0;
+}
+
+// ---- batches ----------------------------------------------------------------------
+zxt_batch* zo_batch_create(uint32_t flags) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+    (void)hipGetLastError();
+    snprintf(g_create_error, sizeof(g_create_error), "GPU init: NoHipDevice");
+    return nullptr;
+  }
+  try {
+    return new zxt_batch((flags & 1u) != 0);
+  } catch (const zxt::EngineError& err) {
+    snprintf(g_create_error, sizeof(g_create_error), "Batch init: %s", err.message.c_str());
+  } catch (...) {
+    snprintf(g_create_error, sizeof(g_create_error), "Batch init: Unexpected");
+  }
+  return nullptr;
+}
+
+void zo_batch_destroy(zxt_batch* b) { delete b; }
+const char* zo_batch_error(zxt_batch* b) { return b->batch.err.c_str(); }
+
+int32_t zo_batch_add(zxt_batch* b, uint32_t group, const float* q, uint32_t n, uint32_t seed, float delta,
+                     uint32_t kind, uint32_t steps, uint64_t* id_out) {
+  if (!valid_group(group)) {
+    b->batch.err = "Unknown group";
+    return -1;
+  }
+  if (kind > 2) {
+    b->batch.err = "Batch add: UnknownJobKind";
+    return -1;
+  }
+  return guarded_batch(b, "Batch add", [&] {
+    *id_out = b->batch.add(group, q, n, seed, double(delta), zxt::JobKind(kind), steps);
+  });
+}
+
+int32_t zo_batch_step(zxt_batch* b, uint32_t* completed) {
+  return guarded_batch(b, "Batch step", [&] {
+    const uint32_t done = b->batch.step();
+    if (completed) *completed = done;
+  });
+}
+
+int32_t zo_batch_status(zxt_batch* b, uint64_t id) {
+  int32_t s = -1;
+  if (guarded_batch(b, "Batch status", [&] { s = b->batch.status(id); }) != 0) return -1;
+  return s;
+}
+
+const char* zo_batch_job_error(zxt_batch* b, uint64_t id) {
+  if (guarded_batch(b, "Batch job error", [&] { b->text = b->batch.error(id); }) != 0) return "";
+  return b->text.c_str();
+}
+
+const char* zo_batch_stats(zxt_batch* b, uint64_t id) {
+  if (guarded_batch(b, "Batch stats", [&] { b->text = b->batch.stats(id); }) != 0) return "";
+  return b->text.c_str();
+}
+
+const float* zo_batch_quaternions(zxt_batch* b, uint64_t id) {
+  const float* p = nullptr;
+  guarded_batch(b, "Batch quaternions", [&] { p = b->batch.quaternions(id); });
+  return p;
+}
+
+uint32_t zo_batch_count(zxt_batch* b, uint64_t id) {
+  uint32_t c = 0;
+  guarded_batch(b, "Batch count", [&] { c = b->batch.count(id); });
+  return c;
+}
+
+const float* zo_batch_point_values(zxt_batch* b, uint64_t id, uint32_t kind) {
+  const float* p = nullptr;
+  guarded_batch(b, "Batch point values", [&] { p = b->batch.engine(id).point_values(kind); });
+  return p;
+}
+
+const uint32_t* zo_batch_point_indices(zxt_batch* b, uint64_t id) {
+  const uint32_t* p = nullptr;
+  guarded_batch(b, "Batch point indices", [&] { p = b->batch.engine(id).point_indices(); });
+  return p;
+}
+
+uint64_t zo_batch_bytes(zxt_batch* b, uint64_t id) {
+  uint64_t v = 0;
+  guarded_batch(b, "Batch bytes", [&] { v = b->batch.bytes(id); });
+  return v;
+}
+
+int32_t zo_batch_remove(zxt_batch* b, uint64_t id) {
+  return guarded_batch(b, "Batch remove", [&] { b->batch.remove(id); });
+}
+
+uint32_t zo_batch_running(zxt_batch* b) { return b->batch.running(); }
+
+int32_t zo_batch_graphs(zxt_batch* b, uint64_t* out2) {
+  out2[0] = b->batch.captures();
+  out2[1] = b->batch.replays();
   return 0;
 }
 

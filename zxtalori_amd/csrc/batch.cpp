@@ -1,0 +1,266 @@
+// Batched relaxation of many samples: see batch.h.
+#include "batch.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace zxt {
+
+namespace {
+[[noreturn]] void fail(const std::string& what) { throw EngineError{what}; }
+}  // namespace
+
+Batch::Batch(bool use_graphs) : use_graphs_(use_graphs) {
+  Engine::check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+  prof.set_stream(stream);
+}
+
+Batch::~Batch() {
+  (void)hipStreamSynchronize(stream);
+  if (graph_) (void)hipGraphExecDestroy(graph_);
+  jobs_.clear();
+  for (Table* t : {&ap_table_, &step_table_}) {
+    if (t->host_desc) (void)hipHostFree(t->host_desc);
+    if (t->host_start) (void)hipHostFree(t->host_start);
+  }
+  (void)hipStreamDestroy(stream);
+}
+
+Batch::Job* Batch::find(uint64_t id) const {
+  for (const auto& j : jobs_)
+    if (j->id == id) return j.get();
+  fail("UnknownJob");
+}
+
+uint64_t Batch::add(uint32_t group, const float* q, uint32_t n, uint32_t seed, double delta, JobKind kind,
+                    uint32_t steps) {
+  auto job = std::make_unique<Job>();
+  job->engine = std::make_unique<Engine>(stream);
+  Engine& e = *job->engine;
+  if (q)
+    e.set(group, q, size_t(n) * 4);
+  else
+    e.generate_random(group, n, seed);
+  e.set_neighbourhood(delta);
+  job->kind = kind;
+  job->remaining = kind == JobKind::Relax ? std::min<uint32_t>(steps, 1000) : 0;
+  job->id = next_id_++;
+  const uint64_t id = job->id;
+  jobs_.push_back(std::move(job));
+  return id;
+}
+
+// Uploads a job table (only when it changed) and returns its JobSet.
+JobSet Batch::upload(Table& t, const std::vector<JobDesc>& descs) {
+  const size_t count = descs.size();
+  bool same = t.last.size() == count;
+  for (size_t k = 0; same && k < count; ++k) same = std::memcmp(&t.last[k], &descs[k], sizeof(JobDesc)) == 0;
+  if (same && count) return t.set;
+  // A captured step must replay with the tables it was captured with.
+  if (capturing_) throw GraphTableChanged{};
+  JobSet s;
+  s.count = uint32_t(count);
+  if (count == 0) {
+    t.last.clear();
+    t.set = s;
+    return s;
+  }
+  if (count > t.capacity) {
+    Engine::check(hipStreamSynchronize(stream), "wait");
+    if (t.host_desc) (void)hipHostFree(t.host_desc);
+    if (t.host_start) (void)hipHostFree(t.host_start);
+    t.capacity = std::max<size_t>(count, 2 * t.capacity);
+    Engine::check(hipHostMalloc(reinterpret_cast<void**>(&t.host_desc), t.capacity * sizeof(JobDesc)), "pinned");
+    Engine::check(hipHostMalloc(reinterpret_cast<void**>(&t.host_start), 3 * (t.capacity + 1) * sizeof(uint32_t)),
+                  "pinned");
+    t.desc.ensure(t.capacity * sizeof(JobDesc));
+    t.start.ensure(3 * (t.capacity + 1) * sizeof(uint32_t));
+  } else {
+    // The previous upload of these pinned buffers may still be in flight.
+    Engine::check(hipStreamSynchronize(stream), "wait");
+  }
+  uint32_t* point = t.host_start;
+  uint32_t* ap = point + (count + 1);
+  uint32_t* median = ap + (count + 1);
+  point[0] = ap[0] = median[0] = 0;
+  for (size_t k = 0; k < count; ++k) {
+    t.host_desc[k] = descs[k];
+    point[k + 1] = point[k] + descs[k].point_blocks;
+    ap[k + 1] = ap[k] + descs[k].point_blocks * descs[k].plan.splits();
+    median[k + 1] = median[k] + descs[k].median_blocks;
+  }
+  Engine::check(hipMemcpyAsync(t.desc.p, t.host_desc, count * sizeof(JobDesc), hipMemcpyHostToDevice, stream),
+                "table");
+  Engine::check(hipMemcpyAsync(t.start.p, t.host_start, 3 * (count + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               stream),
+                "table");
+  const uint32_t* dstart = t.start.as<uint32_t>();
+  s.table = t.desc.as<JobDesc>();
+  s.point_start = dstart;
+  s.ap_start = dstart + (count + 1);
+  s.median_start = dstart + 2 * (count + 1);
+  s.point_total = point[count];
+  s.ap_total = ap[count];
+  s.median_total = median[count];
+  t.last = descs;
+  t.set = s;
+  return s;
+}
+
+void Batch::finish(Job& j, bool recorded) {
+  try {
+    j.engine->finish_relax(recorded);
+    j.state = 1;
+  } catch (const EngineError& e) {
+    j.state = 2;
+    j.error = e.message;
+  }
+}
+
+uint32_t Batch::step() {
+  uint32_t completed = 0;
+  // Measure and spacing jobs run as they come (each ends with a wait).
+  for (auto& jp : jobs_) {
+    Job& j = *jp;
+    if (j.state != 0 || j.kind == JobKind::Relax) continue;
+    try {
+      if (j.kind == JobKind::Measure)
+        j.engine->measure();
+      else
+        j.engine->spacing();
+      j.state = 1;
+    } catch (const EngineError& e) {
+      j.state = 2;
+      j.error = e.message;
+    }
+    ++completed;
+  }
+  std::vector<Job*> active;
+  std::vector<uint64_t> key;
+  bool fresh = false;
+  for (auto& jp : jobs_) {
+    Job& j = *jp;
+    if (j.state != 0 || j.kind != JobKind::Relax) continue;
+    if (!j.started) {
+      try {
+        j.engine->begin_relax();
+        j.started = true;
+        fresh = true;
+      } catch (const EngineError& e) {
+        j.state = 2;
+        j.error = e.message;
+        ++completed;
+        continue;
+      }
+    }
+    if (j.remaining == 0) {  // a relax of 0 steps: only the measure that ends it
+      finish(j, false);
+      ++completed;
+      continue;
+    }
+    active.push_back(&j);
+    key.push_back(j.id);
+  }
+  if (active.empty()) return completed;
+  // A step is the same launch sequence as the last one when the jobs are the same and none
+  // starts now (a first step sizes buffers and may wait): then a captured graph replays it.
+  const bool steady = use_graphs_ && !prof.on() && !fresh && key == last_key_;
+  last_key_ = key;
+  auto record = [&]() {
+    std::vector<JobDesc> ap, steps;
+    for (Job* j : active) {
+      JobDesc d;
+      if (j->engine->step_forces(d)) ap.push_back(d);
+    }
+    if (!ap.empty()) launch_allpairs(prof, stream, upload(ap_table_, ap));
+    for (Job* j : active) steps.push_back(j->engine->step_job());
+    const JobSet set = upload(step_table_, steps);
+    launch_bb_step(prof, stream, set);
+    launch_fold(prof, stream, set);
+  };
+  try {
+    if (steady && graph_ && graph_key_ == key) {
+      Engine::check(hipGraphLaunch(graph_, stream), "graph");
+      ++replays_;
+    } else if (steady) {
+      if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+      }
+      graph_key_.clear();
+      hipGraph_t g = nullptr;
+      Engine::check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+      capturing_ = true;
+      bool changed = false;
+      try {
+        record();
+      } catch (const GraphTableChanged&) {
+        changed = true;
+      } catch (...) {
+        capturing_ = false;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      capturing_ = false;
+      const hipError_t ended = hipStreamEndCapture(stream, &g);
+      if (changed) {
+        // Nothing captured has run: record this step eagerly, capture the next one.
+        if (g) (void)hipGraphDestroy(g);
+        Engine::check(ended, "capture");
+        record();
+      } else {
+        Engine::check(ended, "capture");
+        const hipError_t made = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        Engine::check(made, "graph");
+        graph_key_ = key;
+        Engine::check(hipGraphLaunch(graph_, stream), "graph");
+        ++captures_;
+      }
+    } else {
+      record();
+    }
+  } catch (const EngineError& e) {
+    for (Job* j : active) {
+      j->state = 2;
+      j->error = e.message;
+    }
+    err = e.message;
+    return completed + uint32_t(active.size());
+  }
+  for (Job* j : active) {
+    j->engine->stepped();
+    if (--j->remaining == 0) {
+      finish(*j, true);
+      ++completed;
+    }
+  }
+  return completed;
+}
+
+int Batch::status(uint64_t id) const { return find(id)->state; }
+std::string Batch::stats(uint64_t id) { return find(id)->engine->stats(); }
+std::string Batch::error(uint64_t id) const { return find(id)->error; }
+const float* Batch::quaternions(uint64_t id) const { return find(id)->engine->quaternions(); }
+uint32_t Batch::count(uint64_t id) const { return find(id)->engine->count(); }
+size_t Batch::bytes(uint64_t id) const { return find(id)->engine->device_bytes(); }
+
+void Batch::remove(uint64_t id) {
+  Engine::check(hipStreamSynchronize(stream), "wait");
+  for (auto it = jobs_.begin(); it != jobs_.end(); ++it) {
+    if ((*it)->id == id) {
+      jobs_.erase(it);
+      return;
+    }
+  }
+  fail("UnknownJob");
+}
+
+uint32_t Batch::running() const {
+  uint32_t c = 0;
+  for (const auto& j : jobs_) c += j->state == 0;
+  return c;
+}
+
+}  // namespace zxt

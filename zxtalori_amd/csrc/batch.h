@@ -1,0 +1,86 @@
+// Many samples relaxed together (no counterpart in the reference, which runs one sample
+// per engine): every job is an Engine on the batch's stream; per step, the all-pairs
+// force passes of all jobs run as one launch, near-field jobs record their own site build
+// and pair kernel, and the BB step and fold of all jobs run as one launch per stage.  A
+// job's arithmetic is the same as alone (the multi-sample kernels are the ones a single
+// engine uses), so its result is bit-identical alone and in the mix.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+namespace zxt {
+
+enum class JobKind : uint32_t { Relax = 0, Measure = 1, Spacing = 2 };
+
+class Batch {
+ public:
+  explicit Batch(bool use_graphs);
+  ~Batch();
+
+  // Adds a job: positions q (n rows) or, when q is null, n seeded Haar orientations.
+  // delta: relaxation neighbourhood (0 all pairs).  Returns its id.  Throws EngineError.
+  uint64_t add(uint32_t group, const float* q, uint32_t n, uint32_t seed, double delta, JobKind kind,
+               uint32_t steps);
+  // One BB step for every running relax job, then the jobs that reached their steps are
+  // finished (one wait each) and pending measure/spacing jobs run.  Returns how many jobs
+  // completed.
+  uint32_t step();
+  // 0 running, 1 done, 2 failed (message in error(id)); throws for an unknown id.
+  int status(uint64_t id) const;
+  std::string stats(uint64_t id);
+  std::string error(uint64_t id) const;
+  const float* quaternions(uint64_t id) const;
+  uint32_t count(uint64_t id) const;
+  size_t bytes(uint64_t id) const;
+  void remove(uint64_t id);
+  uint32_t running() const;
+  Engine& engine(uint64_t id) const { return *find(id)->engine; }
+  // Steps captured into a graph, and steps replayed from one.
+  uint64_t captures() const { return captures_; }
+  uint64_t replays() const { return replays_; }
+
+  hipStream_t stream = nullptr;
+  Prof prof;
+  std::string err;
+
+ private:
+  struct Job {
+    uint64_t id = 0;
+    std::unique_ptr<Engine> engine;
+    JobKind kind = JobKind::Relax;
+    uint32_t remaining = 0;
+    bool started = false;
+    int state = 0;  // 0 running, 1 done, 2 failed
+    std::string error;
+  };
+  struct Table {
+    DevBuf desc, start;
+    JobDesc* host_desc = nullptr;  // pinned
+    uint32_t* host_start = nullptr;
+    size_t capacity = 0;
+    std::vector<JobDesc> last;
+    JobSet set;
+  };
+  Job* find(uint64_t id) const;
+  JobSet upload(Table& t, const std::vector<JobDesc>& descs);
+  void finish(Job& j, bool recorded);
+
+  std::vector<std::unique_ptr<Job>> jobs_;
+  uint64_t next_id_ = 1;
+  Table ap_table_, step_table_;
+  bool use_graphs_ = false;
+  bool capturing_ = false;
+  struct GraphTableChanged {};
+  uint64_t captures_ = 0, replays_ = 0;
+  hipGraphExec_t graph_ = nullptr;
+  std::vector<uint64_t> graph_key_;
+  std::vector<uint64_t> last_key_;
+};
+
+}  // namespace zxt

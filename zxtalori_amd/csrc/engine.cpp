@@ -91,13 +91,14 @@ NearBuffers NearField::view() const {
   b.scan_tmp = scan_tmp.as<uint32_t>();
   b.counters = counters.as<unsigned long long>();
   b.counts = counts.as<uint32_t>();
+  b.self_pos = self_pos.as<uint32_t>();
   return b;
 }
 
 static void release_near(NearField& f) {
   for (DevBuf* b : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
                     &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor, &f.order_buf,
-                    &f.scan_tmp, &f.counters, &f.counts})
+                    &f.scan_tmp, &f.counters, &f.counts, &f.self_pos})
     b->release();
   f.ready = false;
 }
@@ -120,6 +121,12 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   cut.c0 = float(c0);
   cut.s0 = float(std::sqrt(std::max(0.0, 1.0 - c0 * c0)));
   band = float(std::sin(radius + 1e-4));
+  // x.g lies at S3 angle acos|g.w| from x (and -x.g at pi minus that) for every x, so the
+  // source's own non-identity images enter the cutoff only when r reaches the smallest.
+  double own_angle = kPi;
+  for (uint32_t i = 1; i < order; ++i)
+    own_angle = std::min(own_angle, std::acos(std::min(1.0, std::fabs(double(kOperators[kGroupOffset[group] + i][0])))));
+  cut.own = radius + 1e-3 >= own_angle ? 1u : 0u;
   // Zone planes: the bisectors of the identity and every other signed image
   // (covering.zig:382-397), normalised in FP64.
   std::vector<float4> pl;
@@ -179,6 +186,7 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, kMortonCells)) * 4);
   counters.ensure(4 * sizeof(unsigned long long));
   counts.ensure(size_t(n) * 8);
+  self_pos.ensure(size_t(n) * 4);
   ready = true;
 }
 
@@ -257,17 +265,42 @@ void RelaxState::record_forces(Engine& e, uint32_t n_, uint32_t order_) {
     near.build(e, q, n_, order_, e.group(), mean_radius(uint64_t(n_) * order_));
     launch_near_force(e.prof, e.stream, q, n_, near.view(), near.cut, rows());
   } else {
-    const AllPairsPlan plan = plan_allpairs(n_, order_, e.pair_budget_scale);
-    ap_partial.ensure(size_t(plan.splits()) * n_ * 5 * sizeof(double));
-    ap_pnear.ensure(size_t(plan.splits()) * n_ * sizeof(float));
-    launch_allpairs(e.prof, e.stream, q, n_, e.ops_device(), order_, plan, ap_partial.as<double>(),
-                    ap_pnear.as<float>(), rows());
+    launch_allpairs(e.prof, e.stream, single_job(allpairs_desc(e, n_, order_)));
   }
 }
 
+JobDesc RelaxState::allpairs_desc(Engine& e, uint32_t n_, uint32_t order_) {
+  ensure_rows(n_);
+  JobDesc d = describe_job(static_cast<float4*>(e.q_device()), e.ops_device(), n_, order_, e.group());
+  d.plan = plan_allpairs(n_, order_, e.pair_budget_scale);
+  ap_partial.ensure(size_t(d.plan.splits()) * n_ * 5 * sizeof(double));
+  ap_pnear.ensure(size_t(d.plan.splits()) * n_ * sizeof(float));
+  d.ap_partial = ap_partial.as<double>();
+  d.ap_pnear = ap_pnear.as<float>();
+  d.rows = rows();
+  return d;
+}
+
+JobDesc RelaxState::step_desc(Engine& e, double radius, uint32_t fixed) {
+  JobDesc d = describe_job(static_cast<float4*>(e.q_device()), e.ops_device(), n, e.order(), e.group());
+  d.rows = rows();
+  d.prev = prev.as<float4>();
+  d.state = state.as<BBState>();
+  d.bb_partials = partials.as<double>();
+  d.radius = radius;
+  d.step_scale = 1.0f;
+  d.fixed = fixed;
+  return d;
+}
+
 // ---- Engine ------------------------------------------------------------------------
-Engine::Engine() {
-  check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+Engine::Engine(hipStream_t shared) {
+  if (shared) {
+    stream = shared;
+  } else {
+    check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    owns_stream_ = true;
+  }
   prof.set_stream(stream);
   ops_.ensure(sizeof(kOperators));
   check(hipMemcpy(ops_.p, kOperators, sizeof(kOperators), hipMemcpyHostToDevice), "operators");
@@ -281,7 +314,7 @@ Engine::~Engine() {
   measure_state_.release();
   q_.release();
   ops_.release();
-  (void)hipStreamDestroy(stream);
+  if (owns_stream_) (void)hipStreamDestroy(stream);
 }
 
 void Engine::sync() { check(hipStreamSynchronize(stream), "wait"); }
@@ -561,31 +594,60 @@ void Engine::spacing() {
   full_measures_ = false;
 }
 
-void Engine::relax(uint32_t steps) {
+void Engine::begin_relax() {
   if (n_ < 2) fail("TooFewOrientations");
   cancelled.store(false);
   sync_neighbourhood();
-  const uint32_t ord = order();
-  const double radius = mean_radius(uint64_t(n_) * ord);
+  relax_radius_ = mean_radius(uint64_t(n_) * order());
+}
+
+// The forces of one BB step (if the last pass's are not current): a near-field pass
+// (recorded here) or, for all pairs, the job to add to a multi-sample launch.
+bool Engine::step_forces(JobDesc& allpairs) {
+  if (fix_identity_ && fixed_index_ < 0) fail("IdentityMissingEnableAddIdentity");
   RelaxState& rs = relax_state_;
+  rs.prepare(*this, n_);
+  if (rs.current) return false;
+  if (rs.has_near) {
+    rs.record_forces(*this, n_, order());
+    return false;
+  }
+  allpairs = rs.allpairs_desc(*this, n_, order());
+  return true;
+}
+
+JobDesc Engine::step_job() { return relax_state_.step_desc(*this, relax_radius_, fixed_slot()); }
+
+void Engine::stepped() {
+  relax_state_.current = false;
+  ++iteration_;
+}
+
+void Engine::relax(uint32_t steps) {
+  begin_relax();
   bool recorded = false;
   const uint32_t count = std::min<uint32_t>(steps, 1000);
   for (uint32_t index = 0; index < count; ++index) {
     if (cancelled.load()) break;
-    if (fix_identity_ && fixed_index_ < 0) fail("IdentityMissingEnableAddIdentity");
-    rs.prepare(*this, n_);
-    if (!rs.current) rs.record_forces(*this, n_, ord);
-    rs.current = false;
-    launch_bb_step(prof, stream, q_.as<float4>(), n_, rs.rows(), rs.prev.as<float4>(), rs.state.as<BBState>(),
-                   rs.partials.as<double>(), radius, 1.0f, fixed_index_ >= 0 ? uint32_t(fixed_index_ + 1) : 0u);
+    JobDesc ap;
+    if (step_forces(ap)) launch_allpairs(prof, stream, single_job(ap));
+    const JobSet one = single_job(step_job());
+    launch_bb_step(prof, stream, one);
     // Right-action folding keeps the left tangent frame of the BB history (engine.zig:914-916).
-    fold(n_);
-    ++iteration_;
+    launch_fold(prof, stream, one);
+    stepped();
     recorded = true;
   }
+  finish_relax(recorded);
+}
+
+// One wait ends a batch of steps: last step, positions, energy and separation
+// (pairs.zig:377-399); these forces start the next step.
+void Engine::finish_relax(bool recorded) {
+  RelaxState& rs = relax_state_;
+  const uint32_t ord = order();
   measured_ = false;
   auto& sep = separation_out();
-  // One wait ends the batch: last step, positions, energy and separation (pairs.zig:377-399).
   BBState st{};
   for (int attempt = 0; attempt < 4; ++attempt) {
     rs.prepare(*this, n_);
@@ -620,6 +682,21 @@ void Engine::relax(uint32_t steps) {
     return;
   }
   fail("NearShellTooLarge");
+}
+
+size_t Engine::device_bytes() const {
+  size_t b = q_.bytes + ops_.bytes + measure_partial_.bytes + measure_out_.bytes;
+  for (const RelaxState* rs : {&relax_state_, &spacing_state_, &measure_state_}) {
+    for (const DevBuf* d : {&rs->force, &rs->energy, &rs->nearest, &rs->prev, &rs->state, &rs->partials,
+                            &rs->ap_partial, &rs->ap_pnear, &rs->nn_partial, &rs->nn_chord, &rs->nn_rank})
+      b += d->bytes;
+    const NearField& f = rs->near;
+    for (const DevBuf* d : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
+                            &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor,
+                            &f.order_buf, &f.scan_tmp, &f.counters, &f.counts, &f.self_pos})
+      b += d->bytes;
+  }
+  return b;
 }
 
 void Engine::identity(bool add, bool fixed) {

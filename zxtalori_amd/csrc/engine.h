@@ -63,7 +63,7 @@ struct NearField {
   // The last pass's sites, shell and overflow flag (valid after a wait).
   NearInfo status{};
   DevBuf planes, info, rim, uq, ucode, urank, ucell, perm, sq, scode, srank, offsets, cursor, src_cell, src_offsets,
-      src_cursor, order_buf, scan_tmp, counters, counts;
+      src_cursor, order_buf, scan_tmp, counters, counts, self_pos;
   NearBuffers view() const;
   void setup(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius);
   // Records sites and sorts for the positions q (setting up first when needed).
@@ -86,11 +86,20 @@ struct RelaxState {
   void prepare(Engine& e, uint32_t n);
   void ensure_rows(uint32_t n);
   void record_forces(Engine& e, uint32_t n, uint32_t order);
+  // The job descriptors of the multi-sample kernels: an all-pairs force pass (buffers
+  // ensured) and a BB step (after prepare).
+  JobDesc allpairs_desc(Engine& e, uint32_t n, uint32_t order);
+  JobDesc step_desc(Engine& e, double radius, uint32_t fixed);
 };
 
+class Batch;
+
 class Engine {
+  friend class Batch;
+
  public:
-  Engine();
+  // With a stream, launches go to it (a batch's); otherwise the engine owns one.
+  explicit Engine(hipStream_t shared = nullptr);
   ~Engine();
 
   // Public operations (src/engine.zig).  They throw EngineError.
@@ -103,6 +112,7 @@ class Engine {
   void identity(bool add, bool fixed);
   void force_passes(uint32_t passes);
   void force_rows(float* force4, double* energy, float* nearest);
+  size_t device_bytes() const;
   std::string stats();
   const float* point_values(uint32_t kind) const;
   const uint32_t* point_indices() const;
@@ -126,6 +136,8 @@ class Engine {
 
   const float4* ops_device() const;
   void* q_device() const;
+  // index + 1 of the held identity, or 0.
+  uint32_t fixed_slot() const { return fixed_index_ >= 0 ? uint32_t(fixed_index_ + 1) : 0u; }
   void sync();
   template <typename T>
   void download(std::vector<T>& dst, const DevBuf& src, size_t count) {
@@ -150,6 +162,14 @@ class Engine {
   Measures quick_measures(const std::vector<float4>& f, const std::vector<double>& energy,
                           const std::vector<float>& nearest) const;
   std::vector<float>& separation_out();
+  // relax() in parts, so that a batch runs the same steps for many samples.
+  void begin_relax();
+  bool step_forces(JobDesc& allpairs);
+  JobDesc step_job();
+  void stepped();
+  void finish_relax(bool recorded);
+  double relax_radius_ = 0;
+  bool owns_stream_ = false;
 
   uint32_t group_ = 9;
   DevBuf ops_, q_;

@@ -27,6 +27,7 @@ struct NearCut {
   float inv_width;    // 1 / (0.5 r)
   float c0;           // x.y >= c0 bounds every accepted site (with margin)
   float s0;           // sqrt(1 - c0^2)
+  uint32_t own;       // the source's own non-identity images can lie within the cutoff
 };
 
 // Device-resident status of one near-field pass.  lo/hi are order-preserving u32 keys of
@@ -63,7 +64,7 @@ struct AllPairsPlan {
   uint32_t site_chunk = 256;
   uint32_t site_splits = 1;
   uint32_t op_splits = 1;
-  uint32_t splits() const { return ops_per_split ? site_splits * op_splits : 0; }
+  __host__ __device__ uint32_t splits() const { return ops_per_split ? site_splits * op_splits : 0; }
 };
 
 // Per-launch timing with HIP events on the engine's stream (zo_profile).
@@ -100,14 +101,51 @@ class Prof {
   hipEvent_t get_event();
 };
 
+// One sample's device state for the multi-sample kernels (all pairs, BB step, fold).
+struct JobDesc {
+  float4* q = nullptr;
+  const float4* ops = nullptr;
+  uint32_t n = 0, order = 0, group = 0;
+  uint32_t fixed = 0;  // index + 1 of a held identity, or 0
+  Rows rows;
+  float4* prev = nullptr;        // 2n: previous force, previous move
+  BBState* state = nullptr;
+  double* bb_partials = nullptr;  // 5 per point block
+  double* ap_partial = nullptr;   // 5 * splits * n
+  float* ap_pnear = nullptr;      // splits * n
+  AllPairsPlan plan;
+  double radius = 0;
+  float step_scale = 1.0f;
+  uint32_t point_blocks = 0;   // ceil(n / 256)
+  uint32_t median_blocks = 0;  // min(point_blocks, 256)
+  uint32_t reserved = 0;       // keeps the struct free of padding (tables compare bytewise)
+};
+static_assert(sizeof(JobDesc) == 136, "JobDesc has no padding bytes");
+
+// The jobs of one launch: a device table of JobDesc with per-stage prefix sums of each
+// job's workgroups (grid block b belongs to the job j with start[j] <= b < start[j+1]),
+// or a single job passed by value (table == nullptr).
+struct JobSet {
+  const JobDesc* table = nullptr;
+  uint32_t count = 0;
+  const uint32_t* point_start = nullptr;   // count + 1
+  const uint32_t* ap_start = nullptr;      // count + 1
+  const uint32_t* median_start = nullptr;  // count + 1
+  uint32_t point_total = 0, ap_total = 0, median_total = 0;
+  JobDesc one;
+};
+JobDesc describe_job(float4* q, const float4* ops, uint32_t n, uint32_t order, uint32_t group);
+JobSet single_job(const JobDesc& d);
+
 // ---- launchers (kernels.hip) ----------------------------------------------------
 void launch_haar(Prof& p, hipStream_t s, float4* q, uint32_t n, uint32_t seed);
+void launch_fold(Prof& p, hipStream_t s, const JobSet& jobs);
 void launch_fold(Prof& p, hipStream_t s, float4* q, uint32_t n, const float4* ops, uint32_t order, uint32_t group);
 void launch_fill_u32(Prof& p, hipStream_t s, uint32_t* dst, uint32_t value, size_t count);
 
 AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale);
-void launch_allpairs(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
-                     const AllPairsPlan& plan, double* partial, float* partial_nearest, Rows out);
+// All-pairs rows of every job in the set (each job's plan, partial rows and rows).
+void launch_allpairs(Prof& p, hipStream_t s, const JobSet& jobs);
 void launch_allpairs_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                              uint64_t* partial, float* chord_out, uint32_t* rank_out);
 // FP64 Riesz 1-3, log and Green row sums (out: n x 5), all pairs, self pair excluded.
@@ -141,6 +179,7 @@ struct NearBuffers {
   uint32_t* scan_tmp = nullptr;  // scan_tmp_words(max cells)
   unsigned long long* counters = nullptr;  // [0] accepted pairs, [1] sites tested (force pass)
   uint32_t* counts = nullptr;    // 2n: per source, pairs and sites tested of the last force pass
+  uint32_t* self_pos = nullptr;  // n: sorted position of each source's identity image
 };
 constexpr uint32_t kMortonBits = 6;
 constexpr uint32_t kMortonCells = 1u << (3 * kMortonBits);
@@ -159,9 +198,8 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, uint32_t order, const NearBuffers& b,
                          const NearCut& cut, float* chord_out, uint32_t* rank_out);
 
-// BB step (relax.wgsl), resident: stats, sums, median, move.
-void launch_bb_step(Prof& p, hipStream_t s, float4* q, uint32_t n, Rows rows, float4* prev, BBState* state,
-                    double* partials, double radius, float step_scale, uint32_t fixed);
+// BB step (relax.wgsl) of every job in the set, resident: stats, sums, median, move.
+void launch_bb_step(Prof& p, hipStream_t s, const JobSet& jobs);
 uint32_t bb_partials(uint32_t n);
 void launch_bb_reset(Prof& p, hipStream_t s, BBState* state);
 

@@ -244,10 +244,28 @@ __device__ bool in_zone(const float4 p, uint32_t group, const float4* __restrict
   }
 }
 
-__global__ void k_fold(float4* __restrict__ q, uint32_t n, const float4* __restrict__ ops, uint32_t order,
-                       uint32_t group) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ inline uint32_t find_job(const uint32_t* __restrict__ start, uint32_t count, uint32_t b) {
+  if (!start) return 0;
+  uint32_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+__device__ inline JobDesc load_job(const JobSet& s, uint32_t j) { return s.table ? s.table[j] : s.one; }
+__device__ inline uint32_t job_start(const uint32_t* __restrict__ start, uint32_t j) { return start ? start[j] : 0u; }
+
+__global__ void k_fold(JobSet s) {
+  const uint32_t j = find_job(s.point_start, s.count, blockIdx.x);
+  const JobDesc d = load_job(s, j);
+  const uint32_t i = (blockIdx.x - job_start(s.point_start, j)) * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  float4* __restrict__ q = d.q;
+  const float4* __restrict__ ops = d.ops;
   const float4 a = q[i];
   const float scale = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w)));
   const int maximum = int((__float_as_uint(scale) >> 23) & 255u);
@@ -263,13 +281,13 @@ __global__ void k_fold(float4* __restrict__ q, uint32_t n, const float4* __restr
   }
   float4 p = normalize4(make_float4(c[0], c[1], c[2], c[3]));
   if (p.x < 0.0f) p = neg4(p);
-  if (in_zone(p, group, ops, order)) {
+  if (in_zone(p, d.group, ops, d.order)) {
     q[i] = p;
     return;
   }
   float4 best = p;
   float w = p.x;
-  for (uint32_t k = 0; k < order; ++k) {
+  for (uint32_t k = 0; k < d.order; ++k) {
     const float4 cq = qmul(p, ops[k]);
     if (fabsf(cq.x) > w) {
       best = cq.x >= 0.0f ? cq : neg4(cq);
@@ -279,10 +297,35 @@ __global__ void k_fold(float4* __restrict__ q, uint32_t n, const float4* __restr
   q[i] = normalize4(best);
 }
 
+JobDesc describe_job(float4* q, const float4* ops, uint32_t n, uint32_t order, uint32_t group) {
+  JobDesc d;
+  d.q = q;
+  d.ops = ops;
+  d.n = n;
+  d.order = order;
+  d.group = group;
+  d.point_blocks = blocks_for(n, kBlock);
+  d.median_blocks = std::max(1u, std::min(d.point_blocks, 256u));
+  return d;
+}
+
+JobSet single_job(const JobDesc& d) {
+  JobSet s;
+  s.count = 1;
+  s.one = d;
+  s.point_total = d.point_blocks;
+  s.ap_total = d.point_blocks * d.plan.splits();
+  s.median_total = d.median_blocks;
+  return s;
+}
+
+void launch_fold(Prof& p, hipStream_t s, const JobSet& jobs) {
+  if (!jobs.point_total) return;
+  p.run("fold", [&] { hipLaunchKernelGGL(k_fold, dim3(jobs.point_total), dim3(kBlock), 0, s, jobs); });
+}
+
 void launch_fold(Prof& p, hipStream_t s, float4* q, uint32_t n, const float4* ops, uint32_t order, uint32_t group) {
-  p.run("fold", [&] {
-    hipLaunchKernelGGL(k_fold, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, q, n, ops, order, group);
-  });
+  launch_fold(p, s, single_job(describe_job(q, ops, n, order, group)));
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ dst, uint32_t value, size_t count) {
@@ -295,23 +338,26 @@ void launch_fill_u32(Prof& p, hipStream_t s, uint32_t* dst, uint32_t value, size
 }
 
 // ---- all pairs (pairs.wgsl:263-326) -------------------------------------------------
-// Partial row layout: 5 doubles (fx, fy, fz, curvature, energy) per (split, source),
-// nearest chord^2 separately.
-__global__ __launch_bounds__(256) void k_allpairs(const float4* __restrict__ q, uint32_t n,
-                                                  const float4* __restrict__ ops, uint32_t order,
-                                                  uint32_t ops_per_split, uint32_t site_chunk, uint32_t site_splits,
-                                                  double* __restrict__ partial, float* __restrict__ pnear) {
+// A job's workgroups are (split, source block): split = operator batch x site chunk of
+// its plan.  Partial row layout: 5 doubles (fx, fy, fz, curvature, energy) per (split,
+// source), nearest chord^2 separately; the reduce sums the splits in a fixed order.
+__global__ __launch_bounds__(256) void k_allpairs(JobSet s) {
   __shared__ float4 tile[kBlock];
-  const uint32_t split = blockIdx.y;
-  const uint32_t k0 = (split / site_splits) * ops_per_split, k1 = min(order, k0 + ops_per_split);
-  const uint32_t j0 = (split % site_splits) * site_chunk, j1 = min(n, j0 + site_chunk);
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t jb = find_job(s.ap_start, s.count, blockIdx.x);
+  const JobDesc d = load_job(s, jb);
+  const uint32_t local = blockIdx.x - job_start(s.ap_start, jb);
+  const uint32_t split = local / d.point_blocks, sblock = local % d.point_blocks;
+  const float4* __restrict__ q = d.q;
+  const uint32_t n = d.n;
+  const uint32_t k0 = (split / d.plan.site_splits) * d.plan.ops_per_split, k1 = min(d.order, k0 + d.plan.ops_per_split);
+  const uint32_t j0 = (split % d.plan.site_splits) * d.plan.site_chunk, j1 = min(n, j0 + d.plan.site_chunk);
+  const uint32_t i = sblock * kBlock + threadIdx.x;
   const bool live = i < n;
   const float4 x = live ? q[i] : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   float nearest = 4.0f;
   for (uint32_t k = k0; k < k1; ++k) {
-    const float4 op = ops[k];
+    const float4 op = d.ops[k];
     for (uint32_t start = j0; start < j1; start += kBlock) {
       const uint32_t j = start + threadIdx.x;
       if (j < j1) tile[threadIdx.x] = image(q[j], op, k);
@@ -341,26 +387,28 @@ __global__ __launch_bounds__(256) void k_allpairs(const float4* __restrict__ q, 
   if (live) {
     const size_t o = size_t(split) * n + i;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) partial[5 * o + c] = acc[c];
-    pnear[o] = nearest;
+    for (int c = 0; c < 5; ++c) d.ap_partial[5 * o + c] = acc[c];
+    d.ap_pnear[o] = nearest;
   }
 }
 
-__global__ void k_allpairs_reduce(const double* __restrict__ partial, const float* __restrict__ pnear, uint32_t n,
-                                  uint32_t splits, Rows out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__global__ void k_allpairs_reduce(JobSet s) {
+  const uint32_t jb = find_job(s.point_start, s.count, blockIdx.x);
+  const JobDesc d = load_job(s, jb);
+  const uint32_t i = (blockIdx.x - job_start(s.point_start, jb)) * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
   double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   float nearest = 4.0f;
-  for (uint32_t s = 0; s < splits; ++s) {
-    const size_t o = size_t(s) * n + i;
+  const uint32_t splits = d.plan.splits();
+  for (uint32_t sp = 0; sp < splits; ++sp) {
+    const size_t o = size_t(sp) * d.n + i;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) a[c] += partial[5 * o + c];
-    nearest = fminf(nearest, pnear[o]);
+    for (int c = 0; c < 5; ++c) a[c] += d.ap_partial[5 * o + c];
+    nearest = fminf(nearest, d.ap_pnear[o]);
   }
-  out.force[i] = make_float4(float(a[0]), float(a[1]), float(a[2]), float(a[3]));
-  out.energy[i] = a[4];
-  out.nearest[i] = nearest;
+  d.rows.force[i] = make_float4(float(a[0]), float(a[1]), float(a[2]), float(a[3]));
+  d.rows.energy[i] = a[4];
+  d.rows.nearest[i] = nearest;
 }
 
 AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale) {
@@ -388,17 +436,11 @@ AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale) {
   return p;
 }
 
-void launch_allpairs(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
-                     const AllPairsPlan& plan, double* partial, float* partial_nearest, Rows out) {
-  const uint32_t splits = plan.splits();
-  p.run("allpairs", [&] {
-    hipLaunchKernelGGL(k_allpairs, dim3(blocks_for(n, kBlock), splits), dim3(kBlock), 0, s, q, n, ops, order,
-                       plan.ops_per_split, plan.site_chunk, plan.site_splits, partial, partial_nearest);
-  });
-  p.run("allpairs_reduce", [&] {
-    hipLaunchKernelGGL(k_allpairs_reduce, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, partial, partial_nearest,
-                       n, splits, out);
-  });
+void launch_allpairs(Prof& p, hipStream_t s, const JobSet& jobs) {
+  if (!jobs.ap_total) return;
+  p.run("allpairs", [&] { hipLaunchKernelGGL(k_allpairs, dim3(jobs.ap_total), dim3(kBlock), 0, s, jobs); });
+  p.run("allpairs_reduce",
+        [&] { hipLaunchKernelGGL(k_allpairs_reduce, dim3(jobs.point_total), dim3(kBlock), 0, s, jobs); });
 }
 
 // Nearest orbit point over all pairs with its rank j*2*order + 2*k + sign; the minimum
@@ -861,9 +903,11 @@ __global__ void k_site_gather(NearBuffers b) {
   const uint32_t S = min(b.info->sites, b.capacity);
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x) {
     const uint32_t s = b.perm[k];
+    const uint32_t code = b.ucode[s];
     b.sq[k] = b.uq[s];
-    b.scode[k] = b.ucode[s];
+    b.scode[k] = code;
     b.srank[k] = b.urank[s];
+    if (code & 0x80000000u) b.self_pos[code & 0x7fffffffu] = k;  // the source's own identity image
   }
 }
 
@@ -963,14 +1007,21 @@ __device__ inline float row_gap(float c, int idx, uint32_t dim) {
 // boundaries of the chunk with readlane (uniform, ~1.5 boundaries per chunk).  Candidates
 // within the cutoff update the nearest chord; those outside the source's own orbit enter
 // an LDS ring, and each full 64 runs the pair kernel on a full wave.
-template <int MODE>
+//
+// OWN: whether the source's own orbit can lie within the cutoff.  x.g is at S3 angle
+// acos|g.w| from x for every x, so below the smallest such angle (0.63 rad for I) no own
+// image is ever a candidate: then the candidate stage needs no site codes, the only site
+// to skip is the source's own identity image (its sorted position), and the nearest chord
+// is taken over the compacted pairs.
+template <int MODE, bool OWN>
 __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint32_t n,
                                               const uint32_t* __restrict__ order, const float4* __restrict__ sq,
                                               const uint32_t* __restrict__ scode, const uint32_t* __restrict__ srank,
                                               const uint32_t* __restrict__ offsets, const NearInfo* __restrict__ info,
-                                              uint32_t sdim, NearCut cut, uint32_t two_order, uint32_t real_blocks,
-                                              Rows rows, float* __restrict__ chord_out,
-                                              uint32_t* __restrict__ rank_out, uint32_t* __restrict__ counts) {
+                                              const uint32_t* __restrict__ self_pos, uint32_t sdim, NearCut cut,
+                                              uint32_t two_order, uint32_t real_blocks, Rows rows,
+                                              float* __restrict__ chord_out, uint32_t* __restrict__ rank_out,
+                                              uint32_t* __restrict__ counts) {
   __shared__ WaveLds lds[kBlock / 64];
   const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
   if (vb >= real_blocks) return;
@@ -997,6 +1048,7 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   const uint32_t ny = uint32_t(yhi - ylo + 1), nrows = ny * uint32_t(zhi - zlo + 1);
   const float rc2 = rc * rc;
   const uint32_t self_code = src | 0x80000000u, self_rank = src * two_order;
+  const uint32_t self_site = (MODE == 0 && !OWN) ? self_pos[src] : 0u;
 
   PairSum ps{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1006,7 +1058,9 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   uint32_t pairs = 0, candidates = 0;  // accepted pair evaluations (per lane), sites tested
 
   auto process = [&](uint32_t e) {
-    near_pair_terms(x, L.qy[e], L.qd[e], cut.taper_start, cut.inv_width, ps);
+    const float dm2 = L.qd[e];
+    if (!OWN) nearest = fminf(nearest, dm2);
+    near_pair_terms(x, L.qy[e], dm2, cut.taper_start, cut.inv_width, ps);
     ++pairs;
   };
   auto flush = [&]() {
@@ -1059,13 +1113,17 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
         bool accept = false;
         if (valid) {
           y = sq[site];
-          const uint32_t code = scode[site];
           dm2 = chord2(x, y);
           const bool inside = dm2 <= cut.chord2;
-          // Every orbit point but the source itself bounds the nearest chord; the
-          // source's own images exert no force (pairs.wgsl:404-408).
-          if (inside && code != self_code) nearest = fminf(nearest, dm2);
-          accept = inside && (code & 0x7fffffffu) != src;
+          if (OWN) {
+            // Every orbit point but the source itself bounds the nearest chord; the
+            // source's own images exert no force (pairs.wgsl:404-408).
+            const uint32_t code = scode[site];
+            if (inside && code != self_code) nearest = fminf(nearest, dm2);
+            accept = inside && (code & 0x7fffffffu) != src;
+          } else {
+            accept = inside && site != self_site;
+          }
         }
         const uint64_t mask = __ballot(accept);
         if (mask) {
@@ -1074,13 +1132,14 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
             L.qy[e] = y;
             L.qd[e] = dm2;
           }
-          qcount += uint32_t(__popcll(mask));
+          qcount = __builtin_amdgcn_readfirstlane(qcount + uint32_t(__popcll(mask)));
           wave_sync();
           if (qcount >= 64) {
             process((qhead + lane) & 127u);
-            qhead = (qhead + 64) & 127u;
+            qhead = __builtin_amdgcn_readfirstlane((qhead + 64) & 127u);
             qcount -= 64;
-            if ((++rounds & 15u) == 0) flush();
+            rounds = __builtin_amdgcn_readfirstlane(rounds + 1);
+            if ((rounds & 15u) == 0) flush();
             wave_sync();
           }
         }
@@ -1140,8 +1199,14 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
                        Rows out) {
   const uint32_t real = blocks_for(n, kBlock / 64);
   p.run("near_force", [&] {
-    hipLaunchKernelGGL(k_near<0>, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.srank,
-                       b.offsets, b.info, b.sdim, cut, 0u, real, out, nullptr, nullptr, b.counts);
+    if (cut.own)
+      hipLaunchKernelGGL((k_near<0, true>), dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode,
+                         b.srank, b.offsets, b.info, b.self_pos, b.sdim, cut, 0u, real, out, nullptr, nullptr,
+                         b.counts);
+    else
+      hipLaunchKernelGGL((k_near<0, false>), dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode,
+                         b.srank, b.offsets, b.info, b.self_pos, b.sdim, cut, 0u, real, out, nullptr, nullptr,
+                         b.counts);
   });
   launch_fill_u32(p, s, reinterpret_cast<uint32_t*>(b.counters), 0u, 4);
   p.run("count_reduce", [&] {
@@ -1154,12 +1219,17 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
                          const NearCut& cut, float* chord_out, uint32_t* rank_out) {
   const uint32_t real = blocks_for(n, kBlock / 64);
   p.run("near_nearest", [&] {
-    hipLaunchKernelGGL(k_near<1>, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.srank,
-                       b.offsets, b.info, b.sdim, cut, 2 * order, real, Rows{}, chord_out, rank_out, nullptr);
+    hipLaunchKernelGGL((k_near<1, true>), dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode,
+                       b.srank, b.offsets, b.info, b.self_pos, b.sdim, cut, 2 * order, real, Rows{}, chord_out,
+                       rank_out, nullptr);
   });
 }
 
 // ---- BB step (relax.wgsl:29-165) -----------------------------------------------------
+// Every kernel takes a JobSet: point-indexed kernels map workgroups to jobs through the
+// job's point-block prefix, the one-workgroup stages (sums, median pick) take one
+// workgroup per job.  A job's arithmetic and reduction order do not depend on the other
+// jobs in the set.
 uint32_t bb_partials(uint32_t n) { return blocks_for(n, kBlock); }
 
 __global__ void k_bb_reset(BBState* st) {
@@ -1198,24 +1268,26 @@ __device__ inline void block_sum5(double v[5], double* out) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_bb_stats(Rows rows, const float4* __restrict__ prev, uint32_t n,
-                                                  uint32_t fixed, BBState* __restrict__ st,
-                                                  double* __restrict__ partials) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(256) void k_bb_stats(JobSet s) {
+  const uint32_t jb = find_job(s.point_start, s.count, blockIdx.x);
+  const JobDesc d = load_job(s, jb);
+  const uint32_t block = blockIdx.x - job_start(s.point_start, jb);
+  const uint32_t i = block * kBlock + threadIdx.x;
+  BBState* __restrict__ st = d.state;
   const bool first = st->iteration == 0;
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   uint32_t fl = 0;
-  if (i < n) {
-    const float4 f = rows.force[i];
+  if (i < d.n) {
+    const float4 f = d.rows.force[i];
     const bool finite = (f.x == f.x) && (f.y == f.y) && (f.z == f.z) && fabsf(f.x) <= 3.4e38f &&
                         fabsf(f.y) <= 3.4e38f && fabsf(f.z) <= 3.4e38f;
     if (!finite) fl |= 1u;
-    if (!(rows.nearest[i] > 0.0f)) fl |= 2u;
+    if (!(d.rows.nearest[i] > 0.0f)) fl |= 2u;
     const double fx = f.x, fy = f.y, fz = f.z;
     v[3] = fx * fx + fy * fy + fz * fz;
-    v[4] = rows.energy[i];
-    if (!first && i + 1 != fixed) {
-      const float4 pf = prev[2 * size_t(i)], pd = prev[2 * size_t(i) + 1];
+    v[4] = d.rows.energy[i];
+    if (!first && i + 1 != d.fixed) {
+      const float4 pf = d.prev[2 * size_t(i)], pd = d.prev[2 * size_t(i) + 1];
       const double sx = pd.x, sy_ = pd.y, sz = pd.z;
       const double yx = fx - double(pf.x), yy_ = fy - double(pf.y), yz = fz - double(pf.z);
       v[0] = sx * yx + sy_ * yy_ + sz * yz;
@@ -1224,15 +1296,16 @@ __global__ __launch_bounds__(256) void k_bb_stats(Rows rows, const float4* __res
     }
   }
   if (fl) atomicOr(&st->flags, fl);
-  block_sum5(v, partials + 5 * size_t(blockIdx.x));
+  block_sum5(v, d.bb_partials + 5 * size_t(block));
 }
 
-// One workgroup: partial sums in a fixed order, then the median's rank bookkeeping.
-__global__ __launch_bounds__(256) void k_bb_sums(const double* __restrict__ partials, uint32_t nparts, uint32_t n,
-                                                 BBState* __restrict__ st) {
+// One workgroup per job: partial sums in a fixed order, then the median's rank bookkeeping.
+__global__ __launch_bounds__(256) void k_bb_sums(JobSet s) {
+  const JobDesc d = load_job(s, blockIdx.x);
+  BBState* __restrict__ st = d.state;
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (uint32_t b = threadIdx.x; b < nparts; b += kBlock)
-    for (int c = 0; c < 5; ++c) v[c] += partials[5 * size_t(b) + c];
+  for (uint32_t b = threadIdx.x; b < d.point_blocks; b += kBlock)
+    for (int c = 0; c < 5; ++c) v[c] += d.bb_partials[5 * size_t(b) + c];
   __shared__ double out[5];
   block_sum5(v, out);
   __syncthreads();
@@ -1242,8 +1315,8 @@ __global__ __launch_bounds__(256) void k_bb_sums(const double* __restrict__ part
     st->ss = out[2];
     st->ff = out[3];
     st->energy = out[4];
-    st->rank[0] = (n - 1) / 2;
-    st->rank[1] = n / 2;
+    st->rank[0] = (d.n - 1) / 2;
+    st->rank[1] = d.n / 2;
     st->prefix[0] = st->prefix[1] = 0;
     st->step_bits = 0;
   }
@@ -1252,17 +1325,21 @@ __global__ __launch_bounds__(256) void k_bb_sums(const double* __restrict__ part
 
 __device__ inline uint32_t curvature_bits(const float4 f) { return __float_as_uint(f.w > 0.0f ? f.w : 0.0f); }
 
-__global__ __launch_bounds__(256) void k_median_count(const float4* __restrict__ force, uint32_t n, uint32_t round,
-                                                      BBState* __restrict__ st) {
+// Exact median of max(curvature, 0) by four 8-bit radix rounds over both middle ranks.
+__global__ __launch_bounds__(256) void k_median_count(JobSet s, uint32_t round) {
   __shared__ uint32_t local[512];
+  const uint32_t jb = find_job(s.median_start, s.count, blockIdx.x);
+  const JobDesc d = load_job(s, jb);
+  const uint32_t block = blockIdx.x - job_start(s.median_start, jb);
   local[threadIdx.x] = 0;
   local[threadIdx.x + 256] = 0;
   __syncthreads();
+  BBState* __restrict__ st = d.state;
   const uint32_t shift = 24 - 8 * round;
   const uint32_t high = round > 0 ? (0xffffffffu << (32 - 8 * round)) : 0u;
   const uint32_t p0 = st->prefix[0], p1 = st->prefix[1];
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const uint32_t bits = curvature_bits(force[i]);
+  for (uint32_t i = block * kBlock + threadIdx.x; i < d.n; i += d.median_blocks * kBlock) {
+    const uint32_t bits = curvature_bits(d.rows.force[i]);
     const uint32_t digit = (bits >> shift) & 255u;
     if ((bits & high) == p0) atomicAdd(&local[digit], 1u);
     if ((bits & high) == p1) atomicAdd(&local[256 + digit], 1u);
@@ -1272,16 +1349,18 @@ __global__ __launch_bounds__(256) void k_median_count(const float4* __restrict__
     if (local[k]) atomicAdd(&st->hist[k], local[k]);
 }
 
-__global__ __launch_bounds__(256) void k_median_pick(uint32_t round, double radius, BBState* __restrict__ st) {
+__global__ __launch_bounds__(256) void k_median_pick(JobSet s, uint32_t round) {
+  const JobDesc d = load_job(s, blockIdx.x);
+  BBState* __restrict__ st = d.state;
   if (threadIdx.x == 0) {
     const uint32_t shift = 24 - 8 * round;
     for (int r = 0; r < 2; ++r) {
       uint32_t remaining = st->rank[r];
       uint32_t digit = 255;
-      for (uint32_t d = 0; d < 256; ++d) {
-        const uint32_t c = st->hist[256 * r + d];
+      for (uint32_t dg = 0; dg < 256; ++dg) {
+        const uint32_t c = st->hist[256 * r + dg];
         if (remaining < c) {
-          digit = d;
+          digit = dg;
           break;
         }
         remaining -= c;
@@ -1294,6 +1373,7 @@ __global__ __launch_bounds__(256) void k_median_pick(uint32_t round, double radi
           fmaxf(1e-12f, 0.5f * (__uint_as_float(st->prefix[0]) + __uint_as_float(st->prefix[1])));
       st->median = median;
       const uint32_t it = st->iteration;
+      const double radius = d.radius;
       const double alpha0 = (0.8 / 3.4) * radius * radius * radius;
       double alpha = alpha0;
       if (it > 0) {
@@ -1311,58 +1391,53 @@ __global__ __launch_bounds__(256) void k_median_pick(uint32_t round, double radi
   st->hist[threadIdx.x + 256] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_bb_move(float4* __restrict__ q, const float4* __restrict__ force,
-                                                 float4* __restrict__ prev, uint32_t n, uint32_t fixed, float radius,
-                                                 float step_scale, BBState* __restrict__ st) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(256) void k_bb_move(JobSet s) {
+  const uint32_t jb = find_job(s.point_start, s.count, blockIdx.x);
+  const JobDesc d = load_job(s, jb);
+  const uint32_t i = (blockIdx.x - job_start(s.point_start, jb)) * kBlock + threadIdx.x;
+  BBState* __restrict__ st = d.state;
   if (st->flags != 0) return;
   float moved_size = 0.0f;
-  if (i < n) {
-    if (i + 1 == fixed) {
-      prev[2 * size_t(i)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      prev[2 * size_t(i) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      q[i] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+  if (i < d.n) {
+    if (i + 1 == d.fixed) {
+      d.prev[2 * size_t(i)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      d.prev[2 * size_t(i) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      d.q[i] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
     } else {
+      const float radius = float(d.radius);
       const float alpha = st->alpha, median = st->median;
-      const float4 f = force[i];
-      const float scale = alpha * step_scale / fmaxf(1e-6f, f.w / median + 1e-2f);
+      const float4 f = d.rows.force[i];
+      const float scale = alpha * d.step_scale / fmaxf(1e-6f, f.w / median + 1e-2f);
       const float size = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z) * scale;
       const float clip = fminf(1.0f, 0.5f * radius / fmaxf(size, 1e-12f));
-      const float3 d = make_float3(f.x * (scale * clip), f.y * (scale * clip), f.z * (scale * clip));
-      prev[2 * size_t(i)] = make_float4(f.x, f.y, f.z, 0.0f);
-      prev[2 * size_t(i) + 1] = make_float4(d.x, d.y, d.z, 0.0f);
+      const float3 dv = make_float3(f.x * (scale * clip), f.y * (scale * clip), f.z * (scale * clip));
+      d.prev[2 * size_t(i)] = make_float4(f.x, f.y, f.z, 0.0f);
+      d.prev[2 * size_t(i) + 1] = make_float4(dv.x, dv.y, dv.z, 0.0f);
       moved_size = size * clip;
-      const float angle = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+      const float angle = sqrtf(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
       const float sinc = angle > 1e-4f ? sinf(angle) / angle : 1.0f - angle * angle / 6.0f;
-      const float4 rot = make_float4(cosf(angle), d.x * sinc, d.y * sinc, d.z * sinc);
-      float4 m = normalize4(qmul(rot, q[i]));
+      const float4 rot = make_float4(cosf(angle), dv.x * sinc, dv.y * sinc, dv.z * sinc);
+      float4 m = normalize4(qmul(rot, d.q[i]));
       if (m.x < 0.0f) m = neg4(m);
-      q[i] = m;
+      d.q[i] = m;
     }
   }
   const uint32_t mx = wave_max_u32(__float_as_uint(moved_size));
   if ((threadIdx.x & 63) == 0 && mx) atomicMax(&st->step_bits, mx);
 }
 
-void launch_bb_step(Prof& p, hipStream_t s, float4* q, uint32_t n, Rows rows, float4* prev, BBState* state,
-                    double* partials, double radius, float step_scale, uint32_t fixed) {
-  const uint32_t nb = bb_partials(n);
-  p.run("bb_stats", [&] {
-    hipLaunchKernelGGL(k_bb_stats, dim3(nb), dim3(kBlock), 0, s, rows, prev, n, fixed, state, partials);
-  });
-  p.run("bb_sums", [&] { hipLaunchKernelGGL(k_bb_sums, dim3(1), dim3(kBlock), 0, s, partials, nb, n, state); });
-  const uint32_t mg = std::max(1u, std::min(nb, 512u));
+void launch_bb_step(Prof& p, hipStream_t s, const JobSet& jobs) {
+  if (!jobs.count) return;
+  p.run("bb_stats", [&] { hipLaunchKernelGGL(k_bb_stats, dim3(jobs.point_total), dim3(kBlock), 0, s, jobs); });
+  p.run("bb_sums", [&] { hipLaunchKernelGGL(k_bb_sums, dim3(jobs.count), dim3(kBlock), 0, s, jobs); });
   for (uint32_t round = 0; round < 4; ++round) {
     p.run("median_count", [&] {
-      hipLaunchKernelGGL(k_median_count, dim3(mg), dim3(kBlock), 0, s, rows.force, n, round, state);
+      hipLaunchKernelGGL(k_median_count, dim3(jobs.median_total), dim3(kBlock), 0, s, jobs, round);
     });
     p.run("median_pick",
-          [&] { hipLaunchKernelGGL(k_median_pick, dim3(1), dim3(kBlock), 0, s, round, radius, state); });
+          [&] { hipLaunchKernelGGL(k_median_pick, dim3(jobs.count), dim3(kBlock), 0, s, jobs, round); });
   }
-  p.run("bb_move", [&] {
-    hipLaunchKernelGGL(k_bb_move, dim3(nb), dim3(kBlock), 0, s, q, rows.force, prev, n, fixed, float(radius),
-                       step_scale, state);
-  });
+  p.run("bb_move", [&] { hipLaunchKernelGGL(k_bb_move, dim3(jobs.point_total), dim3(kBlock), 0, s, jobs); });
 }
 
 }  // namespace zxt
