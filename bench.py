@@ -218,6 +218,10 @@ def main():
     ap.add_argument("--delta", type=float, default=20.0)
     ap.add_argument("--cpu-sample", type=int, default=int(os.environ.get("ZXT_CPU_SAMPLE", 600000)))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-jobs", type=int, default=256, help="0 skips the batched comparison")
+    ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--serving-requests", type=int, default=40, help="0 skips the serving run")
+    ap.add_argument("--serving-rate", type=float, default=4.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (the HBM counter calibration kernel)")
     args = ap.parse_args()
@@ -295,7 +299,62 @@ def main():
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cut, radius = dense.near_cut(len(q), len(ops), args.delta)
         out["cpu_baseline"] = cpu_baseline(q, ops, cut, radius, args.cpu_sample)
+    if args.batched_jobs > 0:
+        out["batched"] = batched_section(args.batched_jobs, args.batched_steps)
+    if args.serving_requests > 0:
+        out["serving"] = serving_section(args.serving_requests, args.serving_rate)
     print(json.dumps(out), flush=True)
+
+
+def _same(a, b):
+    """Whether two runs of the same jobs gave bit-identical results (stats and positions)."""
+    return all(x.state == y.state == "done" and x.stats == y.stats for x, y in zip(a, b))
+
+
+def batched_section(total, steps):
+    """Mixed-size relax jobs (12 groups with C1, N log-uniform in [500, 2e5], all pairs below
+    5,000 points, neighbourhood 20 above, Haar seed = job index) run in one batch with graph
+    replay, then one after another on their own Grids (the plain per-sample path)."""
+    from zxtalori_amd import serve
+    jobs = serve.mixed_relax_jobs(total, seed=193, steps=steps)
+    print(f"[bench] batched {total} jobs x {steps} steps", file=sys.stderr, flush=True)
+    batched, b = serve.run_batched(jobs)
+    print(f"[bench] batched {b['elapsed_s']:.2f} s; plain loop", file=sys.stderr, flush=True)
+    plain, p = serve.serve_plain(jobs, realtime=False)
+    points = sum(j.count for j in jobs)
+    return {"workload": f"{total} relax jobs, 12 groups (C1 included), N log-uniform [500, 2e5] "
+                        f"(rng 193), all pairs below 5000 points, neighbourhood 20 above, {steps} BB steps",
+            "points": points, "near_jobs": sum(j.neighbourhood > 0 for j in jobs),
+            "batched_s": b["elapsed_s"], "plain_s": p["elapsed_s"],
+            "batched_point_steps_per_s": b["point_steps_per_s"], "plain_point_steps_per_s": p["point_steps_per_s"],
+            "speedup": p["elapsed_s"] / b["elapsed_s"], "graph_captures": b["graph_captures"],
+            "graph_replays": b["graph_replays"], "identical_results": _same(batched, plain),
+            "failed": b["failed"] + p["failed"]}
+
+
+def serving_section(total, rate):
+    """Poisson request stream (relax 20-100 steps, measure, spacing; N log-uniform in
+    [1e3, 1e6]; 12 groups) served by the continuous-batching loop and by the plain path, at
+    the arrival rate (latency) and with every request queued at the start (throughput)."""
+    from zxtalori_amd import serve
+    reqs = serve.poisson_requests(total, rate, seed=5)
+    print(f"[bench] serving {total} requests at {rate}/s", file=sys.stderr, flush=True)
+    with serve.Server(hbm_budget=64 << 30) as server:
+        served, s = server.run(reqs, realtime=True)
+        max_live = server.max_live
+    print(f"[bench] served p50 {s['p50_ms']:.1f} ms p99 {s['p99_ms']:.1f} ms; plain", file=sys.stderr, flush=True)
+    plain, p = serve.serve_plain(reqs, realtime=True)
+    print("[bench] saturated throughput", file=sys.stderr, flush=True)
+    with serve.Server(hbm_budget=64 << 30) as server:
+        _, s_sat = server.run(reqs, realtime=False)
+    _, p_sat = serve.serve_plain(reqs, realtime=False)
+    return {"workload": f"{total} requests, Poisson rate {rate}/s (seed 5): 70% relax (20-100 steps), 15% "
+                        f"measure, 15% spacing; N log-uniform [1e3, 1e6], 12 groups; HBM budget 64 GiB",
+            "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"], "plain_p50_ms": p["p50_ms"], "plain_p99_ms": p["p99_ms"],
+            "jobs_per_s": s["jobs_per_s"], "plain_jobs_per_s": p["jobs_per_s"], "max_live": max_live,
+            "saturated_jobs_per_s": s_sat["jobs_per_s"], "plain_saturated_jobs_per_s": p_sat["jobs_per_s"],
+            "saturated_speedup": p_sat["elapsed_s"] / s_sat["elapsed_s"],
+            "identical_results": _same(served, plain), "failed": s["failed"] + p["failed"]}
 
 
 if __name__ == "__main__":

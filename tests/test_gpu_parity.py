@@ -104,6 +104,31 @@ def test_near_rows(operator_fixture, group, count, delta):
     check_rows(force, energy, nearest, ref)
 
 
+def test_near_rows_repeat_bitwise_in_crowded_cells(operator_fixture):
+    """A wide cutoff over a small sample (C2, N=5073, neighbourhood 20: r = 1.23 rad) crowds
+    thousands of sites into the chart's central cells; their in-cell order must still be the
+    same every pass, so fresh engines give bit-identical rows and trajectories."""
+    note("near crowded repeat C2 5073")
+    runs = []
+    for _ in range(2):
+        with zo.haar("C2", 5073, 67) as grid:
+            grid.relax(0, neighbourhood=20)
+            rows = grid.force_rows()
+            grid.relax(3)
+            runs.append((rows, grid.quaternions, grid.stats["energy"]))
+    for a, b in zip(runs[0][0], runs[1][0]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2] == runs[1][2]
+    # and the rows are the oracle's
+    with zo.haar("C2", 5073, 67) as grid:
+        q = grid.quaternions
+    ops = ops32(operator_fixture, "C2")
+    cut, _ = dense.near_cut(5073, len(ops), 20)
+    force, energy, nearest = runs[0][0]
+    check_rows(force, energy, nearest, oracle.near_brute(q, ops, cut))
+
+
 def test_near_sums_match_dense_reference(operator_fixture):
     """test_near_field.py:43-54."""
     for group, count in (("O", 1000), ("D6", 1000), ("C2", 800)):

@@ -92,13 +92,15 @@ NearBuffers NearField::view() const {
   b.counters = counters.as<unsigned long long>();
   b.counts = counts.as<uint32_t>();
   b.self_pos = self_pos.as<uint32_t>();
+  b.crowded = crowded.as<uint32_t>();
+  b.crowded_capacity = crowded.p ? crowded_capacity(capacity) : 0u;
   return b;
 }
 
 static void release_near(NearField& f) {
   for (DevBuf* b : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
                     &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor, &f.order_buf,
-                    &f.scan_tmp, &f.counters, &f.counts, &f.self_pos})
+                    &f.scan_tmp, &f.counters, &f.counts, &f.self_pos, &f.crowded})
     b->release();
   f.ready = false;
 }
@@ -187,6 +189,7 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   counters.ensure(4 * sizeof(unsigned long long));
   counts.ensure(size_t(n) * 8);
   self_pos.ensure(size_t(n) * 4);
+  crowded.ensure(size_t(crowded_capacity(capacity)) * 4);
   ready = true;
 }
 
@@ -693,7 +696,7 @@ size_t Engine::device_bytes() const {
     const NearField& f = rs->near;
     for (const DevBuf* d : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
                             &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor,
-                            &f.order_buf, &f.scan_tmp, &f.counters, &f.counts, &f.self_pos})
+                            &f.order_buf, &f.scan_tmp, &f.counters, &f.counts, &f.self_pos, &f.crowded})
       b += d->bytes;
   }
   return b;

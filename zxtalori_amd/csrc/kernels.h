@@ -37,6 +37,7 @@ struct NearInfo {
   uint32_t overflow;  // set when sites > capacity
   uint32_t rim;       // rim sources listed
   uint32_t wmin_key;  // order-preserving key of the smallest source scalar part
+  uint32_t crowded;   // cells listed for the crowded-cell sort
   uint32_t lo[3];
   uint32_t hi[3];
   float lower[3];
@@ -180,7 +181,12 @@ struct NearBuffers {
   unsigned long long* counters = nullptr;  // [0] accepted pairs, [1] sites tested (force pass)
   uint32_t* counts = nullptr;    // 2n: per source, pairs and sites tested of the last force pass
   uint32_t* self_pos = nullptr;  // n: sorted position of each source's identity image
+  uint32_t* crowded = nullptr;   // crowded_capacity: cells with more sites than one thread sorts
+  uint32_t crowded_capacity = 0;
 };
+// capacity / 512 + 1 bounds the cells holding more than 512 of `capacity` sites.
+inline uint32_t crowded_capacity(uint32_t capacity) { return capacity / 512 + 1; }
+constexpr uint32_t kCrowdedBlocks = 64;
 constexpr uint32_t kMortonBits = 6;
 constexpr uint32_t kMortonCells = 1u << (3 * kMortonBits);
 size_t scan_tmp_words(size_t count);

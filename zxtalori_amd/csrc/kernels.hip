@@ -676,6 +676,7 @@ __global__ void k_near_reset(NearInfo* info, uint32_t n) {
   info->overflow = 0;
   info->rim = 0;
   info->wmin_key = 0xffffffffu;
+  info->crowded = 0;
   for (int c = 0; c < 3; ++c) {
     info->lo[c] = 0xffffffffu;
     info->hi[c] = 0u;
@@ -877,14 +878,20 @@ __global__ void k_site_scatter(NearBuffers b) {
 }
 
 // In-cell order by a unique key (the site's rank, or the item itself), so sums run in
-// the same order every pass (near.wgsl:70-82).  Crowded cells beyond 512 keep the
-// scatter's order.
+// the same order every pass (near.wgsl:70-82).  One thread sorts a cell of up to
+// kSmallCell items; larger cells are listed (when `crowded` is given) for k_cell_sort_big.
+constexpr uint32_t kSmallCell = 512;
 __global__ void k_cell_sort(const uint32_t* __restrict__ offsets, uint32_t cells, uint32_t* __restrict__ perm,
-                            const uint32_t* __restrict__ key) {
+                            const uint32_t* __restrict__ key, uint32_t* __restrict__ crowded_list,
+                            uint32_t* __restrict__ crowded) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cells) return;
   const uint32_t begin = offsets[c], end = offsets[c + 1];
-  if (end - begin < 2 || end - begin > 512) return;
+  if (end - begin < 2) return;
+  if (end - begin > kSmallCell) {
+    if (crowded) crowded_list[atomicAdd(crowded, 1u)] = c;
+    return;
+  }
   for (uint32_t at = begin + 1; at < end; ++at) {
     const uint32_t v = perm[at];
     const uint32_t kv = key ? key[v] : v;
@@ -896,6 +903,79 @@ __global__ void k_cell_sort(const uint32_t* __restrict__ offsets, uint32_t cells
       --before;
     }
     perm[before] = v;
+  }
+}
+
+// Crowded cells (more than kSmallCell items; a wide cutoff over a small sample crowds the
+// chart's centre), one workgroup each: chunks of kSortChunk sorted in LDS (bitonic, keys
+// unique), then every item placed at its rank (its index in its chunk plus, per other
+// chunk, the count of smaller keys found by binary search) through tmp.
+constexpr uint32_t kSortChunk = 2048;
+__global__ __launch_bounds__(256) void k_cell_sort_big(const uint32_t* __restrict__ offsets, uint32_t* __restrict__ perm,
+                                                       const uint32_t* __restrict__ key,
+                                                       const uint32_t* __restrict__ crowded_list,
+                                                       const uint32_t* __restrict__ crowded, uint32_t list_capacity,
+                                                       uint32_t* __restrict__ tmp) {
+  __shared__ uint32_t sk[kSortChunk], sv[kSortChunk];
+  const uint32_t total = min(*crowded, list_capacity);
+  for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const uint32_t c = crowded_list[b];
+    const uint32_t begin = offsets[c], m = offsets[c + 1] - begin;
+    for (uint32_t cb = 0; cb < m; cb += kSortChunk) {
+      const uint32_t len = min(kSortChunk, m - cb);
+      for (uint32_t t = threadIdx.x; t < kSortChunk; t += blockDim.x) {
+        const uint32_t v = t < len ? perm[begin + cb + t] : 0xffffffffu;
+        sv[t] = v;
+        sk[t] = t < len ? (key ? key[v] : v) : 0xffffffffu;
+      }
+      __syncthreads();
+      for (uint32_t size = 2; size <= kSortChunk; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+          for (uint32_t t = threadIdx.x; t < kSortChunk / 2; t += blockDim.x) {
+            const uint32_t lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+            const bool up = (lo & size) == 0;
+            const uint32_t klo = sk[lo], khi = sk[hi];
+            if ((klo > khi) == up) {
+              sk[lo] = khi;
+              sk[hi] = klo;
+              const uint32_t v = sv[lo];
+              sv[lo] = sv[hi];
+              sv[hi] = v;
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) perm[begin + cb + t] = sv[t];
+      __syncthreads();
+    }
+    if (m > kSortChunk) {
+      const uint32_t chunks = (m + kSortChunk - 1) / kSortChunk;
+      for (uint32_t at = threadIdx.x; at < m; at += blockDim.x) {
+        const uint32_t v = perm[begin + at];
+        const uint32_t kv = key ? key[v] : v;
+        const uint32_t own = at / kSortChunk;
+        uint32_t rank = at - own * kSortChunk;
+        for (uint32_t o = 0; o < chunks; ++o) {
+          if (o == own) continue;
+          uint32_t lo = o * kSortChunk, hi = min(m, lo + kSortChunk);  // first key >= kv in [lo, hi)
+          const uint32_t base = lo;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            const uint32_t u = perm[begin + mid];
+            if ((key ? key[u] : u) < kv)
+              lo = mid + 1;
+            else
+              hi = mid;
+          }
+          rank += lo - base;
+        }
+        tmp[begin + rank] = v;
+      }
+      __syncthreads();
+      for (uint32_t at = threadIdx.x; at < m; at += blockDim.x) perm[begin + at] = tmp[begin + at];
+      __syncthreads();
+    }
   }
 }
 
@@ -967,7 +1047,10 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
   p.run("site_scatter", [&] { hipLaunchKernelGGL(k_site_scatter, dim3(g), dim3(kBlock), 0, s, b); });
   p.run("site_sort", [&] {
     hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(cells, kBlock)), dim3(kBlock), 0, s, b.offsets, uint32_t(cells),
-                       b.perm, b.urank);
+                       b.perm, b.urank, b.crowded, b.crowded ? &b.info->crowded : nullptr);
+    if (b.crowded)
+      hipLaunchKernelGGL(k_cell_sort_big, dim3(kCrowdedBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank,
+                         b.crowded, &b.info->crowded, b.crowded_capacity, b.ucell);
   });
   p.run("site_gather", [&] { hipLaunchKernelGGL(k_site_gather, dim3(g), dim3(kBlock), 0, s, b); });
   const uint32_t gn = stride_grid(n);
@@ -979,8 +1062,10 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
   });
   p.run("src_scatter", [&] { hipLaunchKernelGGL(k_src_scatter, dim3(gn), dim3(kBlock), 0, s, n, b); });
   p.run("src_sort", [&] {
+    // Source order only schedules the waves (each source's row is its own), so crowded
+    // Morton cells keep the scatter's order.
     hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(kMortonCells, kBlock)), dim3(kBlock), 0, s, b.src_offsets,
-                       kMortonCells, b.order, nullptr);
+                       kMortonCells, b.order, nullptr, nullptr, nullptr);
   });
 }
 

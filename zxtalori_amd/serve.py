@@ -15,7 +15,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .api import EXTENDED_GROUPS, Grid, haar
+from .api import EXTENDED_GROUPS, haar
 from .batch import Batch
 
 ORDERS = {"C2": 2, "C3": 3, "C4": 4, "C6": 6, "D2": 4, "D3": 6, "D4": 8, "D6": 12, "T": 12, "O": 24, "I": 60,
@@ -74,6 +74,28 @@ def poisson_requests(total, rate, seed=0, count_range=(1_000, 1_000_000), steps_
         neighbourhood = 20.0 if count >= near_from else 0.0
         out.append(Request(group, count, index, kind, steps, neighbourhood, t))
     return out
+
+
+def mixed_relax_jobs(total=256, seed=193, count_range=(500, 200_000), steps=50, near_from=5_000,
+                     groups=EXTENDED_GROUPS):
+    """Independent relax jobs of uneven size: group uniform over `groups`, N log-uniform in
+    count_range (drawn as floor(exp(U(ln lo, ln hi)))), Haar seed = job index, all pairs
+    below near_from points and neighbourhood 20 from there, `steps` BB steps each."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for index in range(int(total)):
+        group = groups[int(rng.integers(len(groups)))]
+        count = int(np.floor(np.exp(rng.uniform(np.log(count_range[0]), np.log(count_range[1])))))
+        out.append(Request(group, count, index, "relax", int(steps), 20.0 if count >= near_from else 0.0, 0.0))
+    return out
+
+
+def run_batched(jobs, *, graphs=True, keep_results=False, clock=time.perf_counter):
+    """All jobs admitted at once into one Batch and stepped to the end (no budget)."""
+    with Server(hbm_budget=1 << 62, graphs=graphs, keep_results=keep_results) as server:
+        outcomes, stats = server.run(jobs, realtime=False, clock=clock)
+        stats["graph_captures"], stats["graph_replays"] = server.batch.graphs
+    return outcomes, stats
 
 
 def _percentiles(values):
@@ -207,4 +229,4 @@ def serve_plain(requests, *, realtime=True, keep_results=False, clock=time.perf_
     return outcomes, summary(outcomes, clock() - start)
 
 
-__all__ = ["Request", "Outcome", "Server", "poisson_requests", "serve_plain", "summary", "Grid"]
+__all__ = ["Request", "Outcome", "Server", "mixed_relax_jobs", "poisson_requests", "run_batched", "serve_plain", "summary"]
