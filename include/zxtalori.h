@@ -112,7 +112,9 @@ int32_t zo_near_info(zxt_engine* e, uint32_t* out3);
  * and in any mix.  Same int32 / error-string rules as the engine calls; jobs are uint64
  * handles.  One thread per batch. */
 typedef struct zxt_batch zxt_batch;
-/* flags bit 0: replay steady-state steps (same running jobs) from a captured HIP graph.
+/* flags bit 0: replay steady-state steps (same running jobs) from a captured HIP graph;
+ * bits 8-15: extra streams (at most 8) that the near-field passes of different jobs are
+ * spread over, so that small passes overlap on the device (0: one stream).
  * NULL on failure, message in zo_create_error(). */
 zxt_batch* zo_batch_create(uint32_t flags);
 void zo_batch_destroy(zxt_batch* b);

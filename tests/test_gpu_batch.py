@@ -77,10 +77,10 @@ def expected():
     return out
 
 
-@pytest.mark.parametrize("graphs", [False, True])
-def test_alone_equals_in_mix(expected, graphs):
-    note("batch mix, graphs", graphs)
-    with Batch(graphs=graphs) as batch:
+@pytest.mark.parametrize("graphs,streams", [(False, 0), (True, 0), (False, 3), (True, 3)])
+def test_alone_equals_in_mix(expected, graphs, streams):
+    note("batch mix, graphs", graphs, "streams", streams)
+    with Batch(graphs=graphs, streams=streams) as batch:
         jobs = [batch.add(g, count=n, seed=s, kind=k, steps=t, neighbourhood=d) for g, n, s, k, t, d in MIX]
         steps = 0
         while batch.running:

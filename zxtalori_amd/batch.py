@@ -23,9 +23,9 @@ STATES = {0: "running", 1: "done", 2: "failed"}
 class Batch:
     """A set of jobs advanced together by step().  Close it to release the device."""
 
-    def __init__(self, *, graphs=True):
+    def __init__(self, *, graphs=True, streams=3):
         self._lib = _library()
-        self._handle = self._lib.zo_batch_create(1 if graphs else 0)
+        self._handle = self._lib.zo_batch_create((1 if graphs else 0) | (min(int(streams), 8) << 8))
         if not self._handle:
             raise RuntimeError(self._lib.zo_create_error().decode(errors="replace"))
 

@@ -16,7 +16,7 @@ struct zxt_engine {
 };
 
 struct zxt_batch {
-  explicit zxt_batch(bool graphs) : batch(graphs) {}
+  zxt_batch(bool graphs, uint32_t streams) : batch(graphs, streams) {}
   zxt::Batch batch;
   std::string text;
 };
@@ -210,7 +210,7 @@ zxt_batch* zo_batch_create(uint32_t flags) {
     return nullptr;
   }
   try {
-    return new zxt_batch((flags & 1u) != 0);
+    return new zxt_batch((flags & 1u) != 0, (flags >> 8) & 0xffu);
   } catch (const zxt::EngineError& err) {
     snprintf(g_create_error, sizeof(g_create_error), "Batch init: %s", err.message.c_str());
   } catch (...) {

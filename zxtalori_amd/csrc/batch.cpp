@@ -10,9 +10,20 @@ namespace {
 [[noreturn]] void fail(const std::string& what) { throw EngineError{what}; }
 }  // namespace
 
-Batch::Batch(bool use_graphs) : use_graphs_(use_graphs) {
+Batch::Batch(bool use_graphs, uint32_t streams) : use_graphs_(use_graphs) {
   Engine::check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
   prof.set_stream(stream);
+  if (streams > 0) {
+    Engine::check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "event");
+    for (uint32_t k = 0; k < std::min<uint32_t>(streams, 8); ++k) {
+      hipStream_t s = nullptr;
+      Engine::check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+      aux_.push_back(s);
+      hipEvent_t ev = nullptr;
+      Engine::check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+      join_.push_back(ev);
+    }
+  }
 }
 
 Batch::~Batch() {
@@ -23,6 +34,9 @@ Batch::~Batch() {
     if (t->host_desc) (void)hipHostFree(t->host_desc);
     if (t->host_start) (void)hipHostFree(t->host_start);
   }
+  for (hipStream_t s : aux_) (void)hipStreamDestroy(s);
+  for (hipEvent_t e : join_) (void)hipEventDestroy(e);
+  if (fork_) (void)hipEventDestroy(fork_);
   (void)hipStreamDestroy(stream);
 }
 
@@ -50,9 +64,20 @@ uint64_t Batch::add(uint32_t group, const float* q, uint32_t n, uint32_t seed, d
   return id;
 }
 
+// Every device pointer a multi-job kernel dereferences must be set before launch.
+static void check_desc(const JobDesc& d, bool allpairs) {
+  bool ok = d.q && d.ops && d.n && d.rows.force && d.rows.energy && d.rows.nearest;
+  if (allpairs)
+    ok = ok && d.ap_partial && d.ap_pnear && d.plan.splits();
+  else
+    ok = ok && d.prev && d.state && d.bb_partials;
+  if (!ok) fail("IncompleteJobDescriptor");
+}
+
 // Uploads a job table (only when it changed) and returns its JobSet.
 JobSet Batch::upload(Table& t, const std::vector<JobDesc>& descs) {
   const size_t count = descs.size();
+  for (const JobDesc& d : descs) check_desc(d, &t == &ap_table_);
   bool same = t.last.size() == count;
   for (size_t k = 0; same && k < count; ++k) same = std::memcmp(&t.last[k], &descs[k], sizeof(JobDesc)) == 0;
   if (same && count) return t.set;
@@ -167,16 +192,54 @@ uint32_t Batch::step() {
   const bool steady = use_graphs_ && !prof.on() && !fresh && key == last_key_;
   last_key_ = key;
   auto record = [&]() {
+    // Tables first (a captured step cannot upload), then the near-field passes, forked
+    // onto the extra streams after the last step's folds and joined back before this
+    // step's all-pairs pass, BB step and fold.
     std::vector<JobDesc> ap, steps;
+    std::vector<Engine*> near;
     for (Job* j : active) {
-      JobDesc d;
-      if (j->engine->step_forces(d)) ap.push_back(d);
+      Engine& e = *j->engine;
+      e.prepare_step();
+      if (e.relax_state_.has_near) {
+        near.push_back(&e);
+      } else {
+        JobDesc d;
+        if (e.step_forces(d)) ap.push_back(d);
+      }
     }
-    if (!ap.empty()) launch_allpairs(prof, stream, upload(ap_table_, ap));
     for (Job* j : active) steps.push_back(j->engine->step_job());
-    const JobSet set = upload(step_table_, steps);
-    launch_bb_step(prof, stream, set);
-    launch_fold(prof, stream, set);
+    const JobSet ap_set = upload(ap_table_, ap);
+    const JobSet step_set = upload(step_table_, steps);
+    const bool split = !aux_.empty() && !near.empty();
+    if (split) {
+      Engine::check(hipEventRecord(fork_, stream), "fork");
+      for (hipStream_t s : aux_) Engine::check(hipStreamWaitEvent(s, fork_, 0), "fork");
+    }
+    size_t next = 0;
+    for (Engine* e : near) {
+      JobDesc unused;
+      if (!split) {
+        e->step_forces(unused);
+        continue;
+      }
+      const hipStream_t own = e->use_stream(aux_[next++ % aux_.size()]);
+      try {
+        e->step_forces(unused);
+      } catch (...) {
+        e->use_stream(own);
+        throw;
+      }
+      e->use_stream(own);
+    }
+    if (split) {
+      for (size_t k = 0; k < aux_.size(); ++k) {
+        Engine::check(hipEventRecord(join_[k], aux_[k]), "join");
+        Engine::check(hipStreamWaitEvent(stream, join_[k], 0), "join");
+      }
+    }
+    if (!ap.empty()) launch_allpairs(prof, stream, ap_set);
+    launch_bb_step(prof, stream, step_set);
+    launch_fold(prof, stream, step_set);
   };
   try {
     if (steady && graph_ && graph_key_ == key) {

@@ -20,7 +20,10 @@ enum class JobKind : uint32_t { Relax = 0, Measure = 1, Spacing = 2 };
 
 class Batch {
  public:
-  explicit Batch(bool use_graphs);
+  // use_graphs: replay steady-state steps from a captured graph.  streams: near-field
+  // passes of different jobs go round-robin to this many extra streams so that small
+  // passes overlap on the device (0: all on the batch stream).
+  Batch(bool use_graphs, uint32_t streams);
   ~Batch();
 
   // Adds a job: positions q (n rows) or, when q is null, n seeded Haar orientations.
@@ -78,6 +81,9 @@ class Batch {
   bool capturing_ = false;
   struct GraphTableChanged {};
   uint64_t captures_ = 0, replays_ = 0;
+  std::vector<hipStream_t> aux_;
+  std::vector<hipEvent_t> join_;
+  hipEvent_t fork_ = nullptr;
   hipGraphExec_t graph_ = nullptr;
   std::vector<uint64_t> graph_key_;
   std::vector<uint64_t> last_key_;

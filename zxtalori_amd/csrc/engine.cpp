@@ -94,13 +94,15 @@ NearBuffers NearField::view() const {
   b.self_pos = self_pos.as<uint32_t>();
   b.crowded = crowded.as<uint32_t>();
   b.crowded_capacity = crowded.p ? crowded_capacity(capacity) : 0u;
+  b.huge = huge.as<uint32_t>();
+  b.huge_capacity = huge.p ? huge_capacity(capacity) : 0u;
   return b;
 }
 
 static void release_near(NearField& f) {
   for (DevBuf* b : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
                     &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor, &f.order_buf,
-                    &f.scan_tmp, &f.counters, &f.counts, &f.self_pos, &f.crowded})
+                    &f.scan_tmp, &f.counters, &f.counts, &f.self_pos, &f.crowded, &f.huge})
     b->release();
   f.ready = false;
 }
@@ -190,6 +192,7 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   counts.ensure(size_t(n) * 8);
   self_pos.ensure(size_t(n) * 4);
   crowded.ensure(size_t(crowded_capacity(capacity)) * 4);
+  huge.ensure(size_t(huge_capacity(capacity)) * 4);
   ready = true;
 }
 
@@ -606,10 +609,17 @@ void Engine::begin_relax() {
 
 // The forces of one BB step (if the last pass's are not current): a near-field pass
 // (recorded here) or, for all pairs, the job to add to a multi-sample launch.
-bool Engine::step_forces(JobDesc& allpairs) {
+// BB state and row buffers of the next step, so that its step descriptor is complete
+// before its force pass is recorded.
+void Engine::prepare_step() {
   if (fix_identity_ && fixed_index_ < 0) fail("IdentityMissingEnableAddIdentity");
+  relax_state_.prepare(*this, n_);
+  relax_state_.ensure_rows(n_);
+}
+
+bool Engine::step_forces(JobDesc& allpairs) {
+  prepare_step();
   RelaxState& rs = relax_state_;
-  rs.prepare(*this, n_);
   if (rs.current) return false;
   if (rs.has_near) {
     rs.record_forces(*this, n_, order());
@@ -696,7 +706,7 @@ size_t Engine::device_bytes() const {
     const NearField& f = rs->near;
     for (const DevBuf* d : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
                             &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor,
-                            &f.order_buf, &f.scan_tmp, &f.counters, &f.counts, &f.self_pos, &f.crowded})
+                            &f.order_buf, &f.scan_tmp, &f.counters, &f.counts, &f.self_pos, &f.crowded, &f.huge})
       b += d->bytes;
   }
   return b;

@@ -63,7 +63,7 @@ struct NearField {
   // The last pass's sites, shell and overflow flag (valid after a wait).
   NearInfo status{};
   DevBuf planes, info, rim, uq, ucode, urank, ucell, perm, sq, scode, srank, offsets, cursor, src_cell, src_offsets,
-      src_cursor, order_buf, scan_tmp, counters, counts, self_pos, crowded;
+      src_cursor, order_buf, scan_tmp, counters, counts, self_pos, crowded, huge;
   NearBuffers view() const;
   void setup(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius);
   // Records sites and sorts for the positions q (setting up first when needed).
@@ -134,6 +134,14 @@ class Engine {
   uint32_t last_near_sdim = 0;
   unsigned long long last_near_counts[2] = {0, 0};  // accepted pairs, sites tested
 
+  // Sends this engine's launches to s (returns the previous stream); a batch runs the
+  // near-field passes of its jobs on several streams this way.
+  hipStream_t use_stream(hipStream_t s) {
+    hipStream_t old = stream;
+    stream = s;
+    prof.set_stream(s);
+    return old;
+  }
   const float4* ops_device() const;
   void* q_device() const;
   // index + 1 of the held identity, or 0.
@@ -164,6 +172,7 @@ class Engine {
   std::vector<float>& separation_out();
   // relax() in parts, so that a batch runs the same steps for many samples.
   void begin_relax();
+  void prepare_step();
   bool step_forces(JobDesc& allpairs);
   JobDesc step_job();
   void stepped();

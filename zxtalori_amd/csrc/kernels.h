@@ -37,7 +37,8 @@ struct NearInfo {
   uint32_t overflow;  // set when sites > capacity
   uint32_t rim;       // rim sources listed
   uint32_t wmin_key;  // order-preserving key of the smallest source scalar part
-  uint32_t crowded;   // cells listed for the crowded-cell sort
+  uint32_t crowded;   // cells listed for the crowded-cell sort (more than 16 sites)
+  uint32_t huge;      // of those, cells of more than 1024 sites
   uint32_t lo[3];
   uint32_t hi[3];
   float lower[3];
@@ -183,10 +184,15 @@ struct NearBuffers {
   uint32_t* self_pos = nullptr;  // n: sorted position of each source's identity image
   uint32_t* crowded = nullptr;   // crowded_capacity: cells with more sites than one thread sorts
   uint32_t crowded_capacity = 0;
+  uint32_t* huge = nullptr;      // huge_capacity: cells with more sites than one wave sorts
+  uint32_t huge_capacity = 0;
 };
-// capacity / 512 + 1 bounds the cells holding more than 512 of `capacity` sites.
-inline uint32_t crowded_capacity(uint32_t capacity) { return capacity / 512 + 1; }
-constexpr uint32_t kCrowdedBlocks = 64;
+// capacity / 17 + 1 bounds the cells holding more than 16 of `capacity` sites, and
+// capacity / 1025 + 1 those holding more than 1024.
+inline uint32_t crowded_capacity(uint32_t capacity) { return capacity / 17 + 1; }
+inline uint32_t huge_capacity(uint32_t capacity) { return capacity / 1025 + 1; }
+constexpr uint32_t kCrowdedBlocks = 512;
+constexpr uint32_t kHugeBlocks = 64;
 constexpr uint32_t kMortonBits = 6;
 constexpr uint32_t kMortonCells = 1u << (3 * kMortonBits);
 size_t scan_tmp_words(size_t count);

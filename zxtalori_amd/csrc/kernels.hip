@@ -677,6 +677,7 @@ __global__ void k_near_reset(NearInfo* info, uint32_t n) {
   info->rim = 0;
   info->wmin_key = 0xffffffffu;
   info->crowded = 0;
+  info->huge = 0;
   for (int c = 0; c < 3; ++c) {
     info->lo[c] = 0xffffffffu;
     info->hi[c] = 0u;
@@ -879,8 +880,9 @@ __global__ void k_site_scatter(NearBuffers b) {
 
 // In-cell order by a unique key (the site's rank, or the item itself), so sums run in
 // the same order every pass (near.wgsl:70-82).  One thread sorts a cell of up to
-// kSmallCell items; larger cells are listed (when `crowded` is given) for k_cell_sort_big.
-constexpr uint32_t kSmallCell = 512;
+// kThreadCell items; larger cells are listed (when `crowded` is given) for
+// k_cell_sort_waves.
+constexpr uint32_t kThreadCell = 16;
 __global__ void k_cell_sort(const uint32_t* __restrict__ offsets, uint32_t cells, uint32_t* __restrict__ perm,
                             const uint32_t* __restrict__ key, uint32_t* __restrict__ crowded_list,
                             uint32_t* __restrict__ crowded) {
@@ -888,7 +890,7 @@ __global__ void k_cell_sort(const uint32_t* __restrict__ offsets, uint32_t cells
   if (c >= cells) return;
   const uint32_t begin = offsets[c], end = offsets[c + 1];
   if (end - begin < 2) return;
-  if (end - begin > kSmallCell) {
+  if (end - begin > kThreadCell) {
     if (crowded) crowded_list[atomicAdd(crowded, 1u)] = c;
     return;
   }
@@ -906,32 +908,90 @@ __global__ void k_cell_sort(const uint32_t* __restrict__ offsets, uint32_t cells
   }
 }
 
-// Crowded cells (more than kSmallCell items; a wide cutoff over a small sample crowds the
-// chart's centre), one workgroup each: chunks of kSortChunk sorted in LDS (bitonic, keys
-// unique), then every item placed at its rank (its index in its chunk plus, per other
-// chunk, the count of smaller keys found by binary search) through tmp.
-constexpr uint32_t kSortChunk = 2048;
-__global__ __launch_bounds__(256) void k_cell_sort_big(const uint32_t* __restrict__ offsets, uint32_t* __restrict__ perm,
-                                                       const uint32_t* __restrict__ key,
-                                                       const uint32_t* __restrict__ crowded_list,
-                                                       const uint32_t* __restrict__ crowded, uint32_t list_capacity,
-                                                       uint32_t* __restrict__ tmp) {
-  __shared__ uint32_t sk[kSortChunk], sv[kSortChunk];
+// Crowded cells (more than kThreadCell items; a wide cutoff over a small sample crowds the
+// chart's centre), one wave each: the cell sorted in the wave's LDS by a bitonic network
+// padded to a power of two (keys are unique).  Cells of more than kWaveChunk items are
+// listed for k_cell_sort_block.
+constexpr uint32_t kWaveChunk = 1024;
+struct SortLds {
+  uint32_t k[kWaveChunk];
+  uint32_t v[kWaveChunk];
+};
+__global__ __launch_bounds__(256) void k_cell_sort_waves(const uint32_t* __restrict__ offsets,
+                                                         uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
+                                                         const uint32_t* __restrict__ crowded_list,
+                                                         const uint32_t* __restrict__ crowded, uint32_t list_capacity,
+                                                         uint32_t* __restrict__ huge_list, uint32_t* __restrict__ huge) {
+  __shared__ SortLds lds[kBlock / 64];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  SortLds& L = lds[wave];
   const uint32_t total = min(*crowded, list_capacity);
-  for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  for (uint32_t b = blockIdx.x * (kBlock / 64) + wave; b < total; b += waves) {
     const uint32_t c = crowded_list[b];
     const uint32_t begin = offsets[c], m = offsets[c + 1] - begin;
-    for (uint32_t cb = 0; cb < m; cb += kSortChunk) {
-      const uint32_t len = min(kSortChunk, m - cb);
-      for (uint32_t t = threadIdx.x; t < kSortChunk; t += blockDim.x) {
+    if (m > kWaveChunk) {
+      if (lane == 0) huge_list[atomicAdd(huge, 1u)] = c;
+      continue;
+    }
+    uint32_t width = 64;
+    while (width < m) width <<= 1;
+    for (uint32_t t = lane; t < width; t += 64) {
+      const uint32_t v = t < m ? perm[begin + t] : 0xffffffffu;
+      L.v[t] = v;
+      L.k[t] = t < m ? (key ? key[v] : v) : 0xffffffffu;
+    }
+    wave_sync();
+    for (uint32_t size = 2; size <= width; size <<= 1) {
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (uint32_t t = lane; t < width / 2; t += 64) {
+          const uint32_t lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+          const bool up = (lo & size) == 0;
+          const uint32_t klo = L.k[lo], khi = L.k[hi];
+          if ((klo > khi) == up) {
+            L.k[lo] = khi;
+            L.k[hi] = klo;
+            const uint32_t v = L.v[lo];
+            L.v[lo] = L.v[hi];
+            L.v[hi] = v;
+          }
+        }
+        wave_sync();
+      }
+    }
+    for (uint32_t t = lane; t < m; t += 64) perm[begin + t] = L.v[t];
+    wave_sync();
+  }
+}
+
+// Cells of more than kWaveChunk items, one workgroup each: chunks of up to kBlockChunk
+// sorted in LDS (bitonic), then, for cells of more than one chunk, every item placed at
+// its rank (its index in its chunk plus, per other chunk, the count of smaller keys found
+// by binary search) through tmp.
+constexpr uint32_t kBlockChunk = 8192;
+__global__ __launch_bounds__(256) void k_cell_sort_block(const uint32_t* __restrict__ offsets,
+                                                         uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
+                                                         const uint32_t* __restrict__ huge_list,
+                                                         const uint32_t* __restrict__ huge, uint32_t list_capacity,
+                                                         uint32_t* __restrict__ tmp) {
+  __shared__ uint32_t sk[kBlockChunk], sv[kBlockChunk];
+  const uint32_t total = min(*huge, list_capacity);
+  for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const uint32_t c = huge_list[b];
+    const uint32_t begin = offsets[c], m = offsets[c + 1] - begin;
+    for (uint32_t cb = 0; cb < m; cb += kBlockChunk) {
+      const uint32_t len = min(kBlockChunk, m - cb);
+      uint32_t width = 256;
+      while (width < len) width <<= 1;
+      for (uint32_t t = threadIdx.x; t < width; t += blockDim.x) {
         const uint32_t v = t < len ? perm[begin + cb + t] : 0xffffffffu;
         sv[t] = v;
         sk[t] = t < len ? (key ? key[v] : v) : 0xffffffffu;
       }
       __syncthreads();
-      for (uint32_t size = 2; size <= kSortChunk; size <<= 1) {
+      for (uint32_t size = 2; size <= width; size <<= 1) {
         for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-          for (uint32_t t = threadIdx.x; t < kSortChunk / 2; t += blockDim.x) {
+          for (uint32_t t = threadIdx.x; t < width / 2; t += blockDim.x) {
             const uint32_t lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
             const bool up = (lo & size) == 0;
             const uint32_t klo = sk[lo], khi = sk[hi];
@@ -949,16 +1009,16 @@ __global__ __launch_bounds__(256) void k_cell_sort_big(const uint32_t* __restric
       for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) perm[begin + cb + t] = sv[t];
       __syncthreads();
     }
-    if (m > kSortChunk) {
-      const uint32_t chunks = (m + kSortChunk - 1) / kSortChunk;
+    if (m > kBlockChunk) {
+      const uint32_t chunks = (m + kBlockChunk - 1) / kBlockChunk;
       for (uint32_t at = threadIdx.x; at < m; at += blockDim.x) {
         const uint32_t v = perm[begin + at];
         const uint32_t kv = key ? key[v] : v;
-        const uint32_t own = at / kSortChunk;
-        uint32_t rank = at - own * kSortChunk;
+        const uint32_t own = at / kBlockChunk;
+        uint32_t rank = at - own * kBlockChunk;
         for (uint32_t o = 0; o < chunks; ++o) {
           if (o == own) continue;
-          uint32_t lo = o * kSortChunk, hi = min(m, lo + kSortChunk);  // first key >= kv in [lo, hi)
+          uint32_t lo = o * kBlockChunk, hi = min(m, lo + kBlockChunk);  // first key >= kv in [lo, hi)
           const uint32_t base = lo;
           while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
@@ -1048,9 +1108,12 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
   p.run("site_sort", [&] {
     hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(cells, kBlock)), dim3(kBlock), 0, s, b.offsets, uint32_t(cells),
                        b.perm, b.urank, b.crowded, b.crowded ? &b.info->crowded : nullptr);
-    if (b.crowded)
-      hipLaunchKernelGGL(k_cell_sort_big, dim3(kCrowdedBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank,
-                         b.crowded, &b.info->crowded, b.crowded_capacity, b.ucell);
+    if (b.crowded) {
+      hipLaunchKernelGGL(k_cell_sort_waves, dim3(kCrowdedBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank,
+                         b.crowded, &b.info->crowded, b.crowded_capacity, b.huge, &b.info->huge);
+      hipLaunchKernelGGL(k_cell_sort_block, dim3(kHugeBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank,
+                         b.huge, &b.info->huge, b.huge_capacity, b.ucell);
+    }
   });
   p.run("site_gather", [&] { hipLaunchKernelGGL(k_site_gather, dim3(g), dim3(kBlock), 0, s, b); });
   const uint32_t gn = stride_grid(n);
@@ -1061,12 +1124,8 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
                        size_t(kMortonCells));
   });
   p.run("src_scatter", [&] { hipLaunchKernelGGL(k_src_scatter, dim3(gn), dim3(kBlock), 0, s, n, b); });
-  p.run("src_sort", [&] {
-    // Source order only schedules the waves (each source's row is its own), so crowded
-    // Morton cells keep the scatter's order.
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(kMortonCells, kBlock)), dim3(kBlock), 0, s, b.src_offsets,
-                       kMortonCells, b.order, nullptr, nullptr, nullptr);
-  });
+  // Sources stay in the scatter's order within a Morton cell: the order only schedules the
+  // waves, each source's row being its own.
 }
 
 // ---- near-field rows (pairs.wgsl:363-427) -------------------------------------------
