@@ -37,7 +37,9 @@ def lib():
             "orc_allpairs": ([f32p, u32, f32p, u32, f64p], None),
             "orc_near_brute": ([f32p, u32, f32p, u32, f32p, f64p], None),
             "orc_near_grid": ([f32p, u32, f32p, u32, f32p, C.c_double, u32, u32, f64p, f64p], C.c_int),
+            "orc_near_grid64": ([f32p, u32, f32p, f64p, u32, f32p, C.c_double, u32, u32, f64p], C.c_int),
             "orc_nearest": ([f32p, u32, f32p, u32, C.c_float, u32p, f32p], None),
+            "orc_nearest_grid": ([f32p, u32, f32p, u32, C.c_float, C.c_double, u32, u32, u32p, f32p], None),
             "orc_degrees": ([f32p, u32, f32p], None),
             "orc_median": ([f32p, u32], C.c_float),
             "orc_bb_step": ([f32p, u32, f32p, f32p, f64p, f32p, u32p, C.c_double, u32, C.c_float, f64p], None),
@@ -100,6 +102,29 @@ def near_grid(q, ops, cut, radius, first=0, count=None):
     lib().orc_near_grid(q, len(q), ops, len(ops), np.ascontiguousarray(cut, np.float32), float(radius), first, count,
                         out, stats)
     return out, dict(zip(("sites", "shell", "sdim", "candidates", "accepted"), stats.tolist()))
+
+
+def near_grid64(q, ops, ops64, cut, radius, first=0, count=None):
+    """orc_near_grid's pair set with FP64 pair terms (images from the FP64 operators):
+    rows (count, 7) fx fy fz curvature energy nearest-chord^2 sum-of-|pair force|."""
+    q, ops = _q(q), _q(ops)
+    ops64 = np.ascontiguousarray(ops64, np.float64).reshape(-1, 4)
+    count = len(q) - first if count is None else count
+    out = np.empty((count, 7), np.float64)
+    lib().orc_near_grid64(q, len(q), ops, ops64, len(ops), np.ascontiguousarray(cut, np.float32), float(radius),
+                          first, count, out)
+    return out
+
+
+def nearest_grid(q, ops, cut, radius, first=0, count=None):
+    """Nearest orbit point within the cutoff chord^2 `cut` for sources [first, first+count)
+    through the near grid of `radius`: (rank, chord^2)."""
+    q, ops = _q(q), _q(ops)
+    count = len(q) - first if count is None else count
+    rank = np.empty(count, np.uint32)
+    chord = np.empty(count, np.float32)
+    lib().orc_nearest_grid(q, len(q), ops, len(ops), float(cut), float(radius), first, count, rank, chord)
+    return rank, chord
 
 
 def nearest(q, ops, cut=0.0):

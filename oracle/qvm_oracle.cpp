@@ -310,113 +310,32 @@ void orc_near_brute(const float* q, uint32_t n, const float* ops, uint32_t order
   }
 }
 
-// Near-field rows through the reference's data structures: zone planes, rim sources,
-// shell images within sin(r+1e-4) of the zone (shell.wgsl:19-39), a uniform grid of the
+}  // extern "C"
+
+namespace {
+
+// The near field's data structures (src/near.zig:90-188, shaders/shell.wgsl:19-39): zone
+// planes, rim sources, shell images within sin(r+1e-4) of the zone, a uniform grid of the
 // stereographic chart p = q.yzw/(1+q.x), and per source a scan of the cells its cutoff
-// can reach.  Rows are written for sources [first, first+count).  stats receives
-// {sites, shell, sdim, candidates, accepted}.  This is the CPU baseline's timed path.
-int orc_near_grid(const float* q, uint32_t n, const float* ops, uint32_t order, const float* cut4, double radius,
-                  uint32_t first, uint32_t count, double* rows, double* stats) {
-  Cut c{cut4[0], cut4[1], cut4[2], cut4[3]};
-  std::vector<Q> planes = zone_planes(ops, order);
-  float band = float(std::sin(radius + 1e-4));
-  // Sites: core (identity images) then shell images of rim sources.
-  std::vector<Q> site;
-  std::vector<uint32_t> code;
-  site.reserve(size_t(n) * 2);
-  code.reserve(size_t(n) * 2);
-  for (uint32_t i = 0; i < n; ++i) {
-    site.push_back(load(q, i));
-    code.push_back(i | 0x80000000u);
-  }
-  int nt = omp_get_max_threads();
-  std::vector<std::vector<Q>> tsite(nt);
-  std::vector<std::vector<uint32_t>> tcode(nt);
-#pragma omp parallel
-  {
-    int t = omp_get_thread_num();
-#pragma omp for schedule(static)
-    for (int64_t ii = 0; ii < int64_t(n); ++ii) {
-      Q x = load(q, size_t(ii));
-      bool rim = false;
-      for (const Q& pl : planes)
-        if (dot4(x, pl) <= band + 2e-6f) {
-          rim = true;
-          break;
-        }
-      if (!rim) continue;
-      for (uint32_t k = 0; k < order; ++k) {
-        Q base = image(x, load(ops, k), k);
-        for (int s = 0; s < 2; ++s) {
-          if (k == 0 && s == 0) continue;
-          Q y = s ? neg(base) : base;
-          if (y.w <= -0.9999999f) continue;
-          bool inside = true;
-          for (const Q& pl : planes)
-            if (dot4(y, pl) < -band - 2e-6f) {
-              inside = false;
-              break;
-            }
-          if (!inside) continue;
-          tsite[t].push_back(y);
-          tcode[t].push_back(uint32_t(ii));
-        }
-      }
-    }
-  }
-  for (int t = 0; t < nt; ++t) {
-    site.insert(site.end(), tsite[t].begin(), tsite[t].end());
-    code.insert(code.end(), tcode[t].begin(), tcode[t].end());
-  }
-  size_t S = site.size();
-  std::vector<float> px(S), py(S), pz(S);
-  float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
-  for (size_t s = 0; s < S; ++s) {
-    float d = 1.0f + site[s].w;
-    px[s] = site[s].x / d;
-    py[s] = site[s].y / d;
-    pz[s] = site[s].z / d;
-    lo[0] = std::min(lo[0], px[s]);
-    hi[0] = std::max(hi[0], px[s]);
-    lo[1] = std::min(lo[1], py[s]);
-    hi[1] = std::max(hi[1], py[s]);
-    lo[2] = std::min(lo[2], pz[s]);
-    hi[2] = std::max(hi[2], pz[s]);
-  }
-  float width = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-6f}) * 1.0001f;
-  int sdim = int(std::min(256.0, std::max(1.0, std::round(std::cbrt(double(S) / 2.0)))));
-  double cpu = sdim / double(width);
-  auto cell1 = [&](double v, int axis) {
+// cap (a ball in the chart) can reach.
+struct NearGrid {
+  uint32_t n = 0, order = 0;
+  int sdim = 1;
+  double cpu = 1, c0 = 1;
+  float lo[3] = {0, 0, 0};
+  std::vector<Q> site;          // sorted by cell, in scatter order within a cell
+  std::vector<uint32_t> code;   // source | 0x80000000 for the identity image
+  std::vector<uint32_t> rank;   // source*2*order + 2*operator + sign
+  std::vector<uint32_t> off;    // sdim^3 + 1
+  int cell1(double v, int axis) const {
     int cidx = int(std::floor((v - lo[axis]) * cpu));
     return std::min(std::max(cidx, 0), sdim - 1);
-  };
-  size_t cells = size_t(sdim) * sdim * sdim;
-  std::vector<uint32_t> off(cells + 1, 0), perm(S);
-  std::vector<uint32_t> scell(S);
-  for (size_t s = 0; s < S; ++s) {
-    scell[s] = uint32_t(cell1(px[s], 0) + sdim * (cell1(py[s], 1) + size_t(sdim) * cell1(pz[s], 2)));
-    off[scell[s] + 1]++;
   }
-  for (size_t k = 0; k < cells; ++k) off[k + 1] += off[k];
-  {
-    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
-    for (size_t s = 0; s < S; ++s) perm[cur[scell[s]]++] = uint32_t(s);
-  }
-  std::vector<Q> ssite(S);
-  std::vector<uint32_t> scode(S);
-  for (size_t k = 0; k < S; ++k) {
-    ssite[k] = site[perm[k]];
-    scode[k] = code[perm[k]];
-  }
-  double cand_total = 0, acc_total = 0;
-  // The cap {y : x.y >= c0} is a ball in the chart: centre v/(c0+w), radius sqrt(1-c0^2)/(c0+w).
-  double c0 = 1.0 - double(c.chord2) * (1.0 + 1e-4) / 2.0 - 1e-6;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : cand_total, acc_total)
-  for (int64_t ii = 0; ii < int64_t(count); ++ii) {
-    uint32_t i = first + uint32_t(ii);
-    Q x = load(q, i);
+  // Calls f(k) for every site k in the cell rows the cap of x crosses.  Returns the count.
+  template <typename F>
+  uint64_t visit(const Q& x, F&& f) const {
     double denom = c0 + x.w;
-    double ctr[3], rho;
+    double ctr[3] = {0, 0, 0}, rho = 0;
     bool all = denom < 1e-3;
     if (!all) {
       ctr[0] = x.x / denom;
@@ -424,8 +343,7 @@ int orc_near_grid(const float* q, uint32_t n, const float* ops, uint32_t order, 
       ctr[2] = x.z / denom;
       rho = std::sqrt(std::max(0.0, 1.0 - c0 * c0)) / denom * 1.001 + 1e-6;
     }
-    double row[6] = {0, 0, 0, 0, 0, 4.0};
-    float nearest = 4.0f;
+    uint64_t seen = 0;
     int zlo = all ? 0 : cell1(ctr[2] - rho, 2), zhi = all ? sdim - 1 : cell1(ctr[2] + rho, 2);
     int ylo = all ? 0 : cell1(ctr[1] - rho, 1), yhi = all ? sdim - 1 : cell1(ctr[1] + rho, 1);
     for (int z = zlo; z <= zhi; ++z) {
@@ -444,32 +362,269 @@ int orc_near_grid(const float* q, uint32_t n, const float* ops, uint32_t order, 
         }
         size_t rowbase = size_t(sdim) * (y + size_t(sdim) * z);
         for (uint32_t k = off[rowbase + x0]; k < off[rowbase + x1 + 1]; ++k) {
-          cand_total += 1;
-          const Q& s = ssite[k];
-          float dm2 = chord2(x, s);
-          if (!(dm2 <= c.chord2)) continue;
-          uint32_t cd = scode[k];
-          if ((cd & 0x7fffffffu) == i) {
-            if (cd == i) nearest = std::min(nearest, dm2);
-            continue;
-          }
-          nearest = std::min(nearest, dm2);
-          acc_total += 1;
-          add_pair(row, near_pair(x, s, dm2, chord2p(x, s), c));
+          ++seen;
+          f(k);
         }
       }
     }
+    return seen;
+  }
+};
+
+NearGrid build_grid(const float* q, uint32_t n, const float* ops, uint32_t order, float chord2, double radius) {
+  NearGrid g;
+  g.n = n;
+  g.order = order;
+  std::vector<Q> planes = zone_planes(ops, order);
+  float band = float(std::sin(radius + 1e-4));
+  // Sites: core (identity images) then shell images of rim sources.
+  std::vector<Q> site;
+  std::vector<uint32_t> code, rank;
+  site.reserve(size_t(n) * 2);
+  code.reserve(size_t(n) * 2);
+  rank.reserve(size_t(n) * 2);
+  for (uint32_t i = 0; i < n; ++i) {
+    site.push_back(load(q, i));
+    code.push_back(i | 0x80000000u);
+    rank.push_back(i * 2 * order);
+  }
+  int nt = omp_get_max_threads();
+  std::vector<std::vector<Q>> tsite(nt);
+  std::vector<std::vector<uint32_t>> tcode(nt), trank(nt);
+#pragma omp parallel
+  {
+    int t = omp_get_thread_num();
+#pragma omp for schedule(static)
+    for (int64_t ii = 0; ii < int64_t(n); ++ii) {
+      Q x = load(q, size_t(ii));
+      bool rim = false;
+      for (const Q& pl : planes)
+        if (dot4(x, pl) <= band + 2e-6f) {
+          rim = true;
+          break;
+        }
+      if (!rim) continue;
+      for (uint32_t k = 0; k < order; ++k) {
+        Q base = image(x, load(ops, k), k);
+        for (uint32_t s = 0; s < 2; ++s) {
+          if (k == 0 && s == 0) continue;
+          Q y = s ? neg(base) : base;
+          if (y.w <= -0.9999999f) continue;
+          bool inside = true;
+          for (const Q& pl : planes)
+            if (dot4(y, pl) < -band - 2e-6f) {
+              inside = false;
+              break;
+            }
+          if (!inside) continue;
+          tsite[t].push_back(y);
+          tcode[t].push_back(uint32_t(ii));
+          trank[t].push_back(uint32_t(ii) * 2 * order + 2 * k + s);
+        }
+      }
+    }
+  }
+  for (int t = 0; t < nt; ++t) {
+    site.insert(site.end(), tsite[t].begin(), tsite[t].end());
+    code.insert(code.end(), tcode[t].begin(), tcode[t].end());
+    rank.insert(rank.end(), trank[t].begin(), trank[t].end());
+  }
+  size_t S = site.size();
+  std::vector<float> px(S), py(S), pz(S);
+  float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+  for (size_t s = 0; s < S; ++s) {
+    float d = 1.0f + site[s].w;
+    px[s] = site[s].x / d;
+    py[s] = site[s].y / d;
+    pz[s] = site[s].z / d;
+    lo[0] = std::min(lo[0], px[s]);
+    hi[0] = std::max(hi[0], px[s]);
+    lo[1] = std::min(lo[1], py[s]);
+    hi[1] = std::max(hi[1], py[s]);
+    lo[2] = std::min(lo[2], pz[s]);
+    hi[2] = std::max(hi[2], pz[s]);
+  }
+  float width = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-6f}) * 1.0001f;
+  g.sdim = int(std::min(256.0, std::max(1.0, std::round(std::cbrt(double(S) / 2.0)))));
+  g.cpu = g.sdim / double(width);
+  for (int a = 0; a < 3; ++a) g.lo[a] = lo[a];
+  size_t cells = size_t(g.sdim) * g.sdim * g.sdim;
+  g.off.assign(cells + 1, 0);
+  std::vector<uint32_t> scell(S), perm(S);
+  for (size_t s = 0; s < S; ++s) {
+    scell[s] = uint32_t(g.cell1(px[s], 0) + g.sdim * (g.cell1(py[s], 1) + size_t(g.sdim) * g.cell1(pz[s], 2)));
+    g.off[scell[s] + 1]++;
+  }
+  for (size_t k = 0; k < cells; ++k) g.off[k + 1] += g.off[k];
+  {
+    std::vector<uint32_t> cur(g.off.begin(), g.off.end() - 1);
+    for (size_t s = 0; s < S; ++s) perm[cur[scell[s]]++] = uint32_t(s);
+  }
+  g.site.resize(S);
+  g.code.resize(S);
+  g.rank.resize(S);
+  for (size_t k = 0; k < S; ++k) {
+    g.site[k] = site[perm[k]];
+    g.code[k] = code[perm[k]];
+    g.rank[k] = rank[perm[k]];
+  }
+  // The cap {y : x.y >= c0} is a ball in the chart: centre v/(c0+w), radius sqrt(1-c0^2)/(c0+w).
+  g.c0 = 1.0 - double(chord2) * (1.0 + 1e-4) / 2.0 - 1e-6;
+  return g;
+}
+
+// The reference's near-field pair (pairs.wgsl:246-262, :349-361) in FP64 with exact
+// atan and sin: the error model the FP32 kernels are measured against.
+struct D4 {
+  double w, x, y, z;
+};
+inline D4 d_normalize(D4 v) {
+  double s = std::sqrt(v.w * v.w + v.x * v.x + v.y * v.y + v.z * v.z);
+  return {v.w / s, v.x / s, v.y / s, v.z / s};
+}
+inline D4 d_mul(const D4& a, const D4& b) {
+  return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+          a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x, a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w};
+}
+void near_pair64(const D4& x, const D4& y, double radius, double* row) {
+  const double kPi64 = 3.14159265358979323846;
+  double dm2 = 0, dp2 = 0;
+  for (int c = 0; c < 4; ++c) {
+    double xc = (&x.w)[c], yc = (&y.w)[c];
+    dm2 += (xc - yc) * (xc - yc);
+    dp2 += (xc + yc) * (xc + yc);
+  }
+  double nearv = std::min(dm2, dp2), farv = std::max(dm2, dp2);
+  double a = 2.0 * std::atan(std::sqrt(nearv / farv));
+  double b = kPi64 - a;
+  double taper = 0.5 * radius, width = 0.5 * radius;
+  double s = std::min(std::max((a - taper) / width, 0.0), 1.0);
+  double window = 1.0 - s * s * (3.0 - 2.0 * s);
+  double slope = -6.0 * s * (1.0 - s) / width;
+  double energy = kPi64 / (a * b);
+  double sign = dm2 <= dp2 ? 1.0 : -1.0;
+  double d[3] = {x.w * y.x - y.w * x.x + (x.y * y.z - x.z * y.y), x.w * y.y - y.w * x.y + (x.z * y.x - x.x * y.z),
+                 x.w * y.z - y.w * x.z + (x.x * y.y - x.y * y.x)};
+  double along = sign * (1.0 / (a * a + 1e-9) - 1.0 / (b * b + 1e-9)) / (std::sin(a) + 1e-9);
+  double extra = sign * energy * slope / std::sin(a);
+  double f2 = 0;
+  for (int c = 0; c < 3; ++c) {
+    double f = -along * d[c] * window + extra * d[c];
+    row[c] += f;
+    f2 += f * f;
+  }
+  row[3] += window * (1.0 / (a * a * a + 1e-9) + 1.0 / (b * b * b + 1e-9));
+  row[4] += energy * window;
+  row[6] += std::sqrt(f2);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Near-field rows through the reference's data structures (NearGrid above).  Rows are
+// written for sources [first, first+count).  stats receives {sites, shell, sdim,
+// candidates, accepted}.  This is the CPU baseline's timed path.
+int orc_near_grid(const float* q, uint32_t n, const float* ops, uint32_t order, const float* cut4, double radius,
+                  uint32_t first, uint32_t count, double* rows, double* stats) {
+  Cut c{cut4[0], cut4[1], cut4[2], cut4[3]};
+  NearGrid g = build_grid(q, n, ops, order, c.chord2, radius);
+  double cand_total = 0, acc_total = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : cand_total, acc_total)
+  for (int64_t ii = 0; ii < int64_t(count); ++ii) {
+    uint32_t i = first + uint32_t(ii);
+    Q x = load(q, i);
+    double row[6] = {0, 0, 0, 0, 0, 4.0};
+    float nearest = 4.0f;
+    double accepted = 0;
+    cand_total += double(g.visit(x, [&](uint32_t k) {
+      const Q& s = g.site[k];
+      float dm2 = chord2(x, s);
+      if (!(dm2 <= c.chord2)) return;
+      uint32_t cd = g.code[k];
+      if ((cd & 0x7fffffffu) == i) {
+        if (cd == i) nearest = std::min(nearest, dm2);
+        return;
+      }
+      nearest = std::min(nearest, dm2);
+      accepted += 1;
+      add_pair(row, near_pair(x, s, dm2, chord2p(x, s), c));
+    }));
+    acc_total += accepted;
     row[5] = nearest;
     std::memcpy(rows + 6 * size_t(ii), row, sizeof(row));
   }
   if (stats) {
-    stats[0] = double(S);
-    stats[1] = double(S - n);
-    stats[2] = sdim;
+    stats[0] = double(g.site.size());
+    stats[1] = double(g.site.size() - n);
+    stats[2] = g.sdim;
     stats[3] = cand_total;
     stats[4] = acc_total;
   }
   return 0;
+}
+
+// The same pair set as orc_near_grid (the FP32 cutoff test) with every pair term in FP64
+// (near_pair64), on the kernels' geometry in exact arithmetic: the source and identity
+// images are the given FP32 positions, other images normalize(q_j g_k) in FP64 with the
+// FP64 operators ops64 (the reference's f64 literals).  Rows of 7: orc_near_grid's
+// columns with column 5 the FP64 nearest chord^2 over the accepted pairs, and column 6
+// the sum of the pair forces' magnitudes (the scale that cancels in a relaxed row).
+int orc_near_grid64(const float* q, uint32_t n, const float* ops, const double* ops64, uint32_t order,
+                    const float* cut4, double radius, uint32_t first, uint32_t count, double* rows) {
+  NearGrid g = build_grid(q, n, ops, order, cut4[0], radius);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t ii = 0; ii < int64_t(count); ++ii) {
+    uint32_t i = first + uint32_t(ii);
+    Q x = load(q, i);
+    const D4 x64{x.w, x.x, x.y, x.z};
+    double row[7] = {0, 0, 0, 0, 0, 4.0, 0};
+    g.visit(x, [&](uint32_t k) {
+      const Q& s = g.site[k];
+      float dm2 = chord2(x, s);
+      if (!(dm2 <= cut4[0])) return;
+      if ((g.code[k] & 0x7fffffffu) == i) return;
+      uint32_t r = g.rank[k], j = r / (2 * order), op = (r % (2 * order)) / 2, sg = r % 2;
+      Q qj = load(q, j);
+      D4 y{qj.w, qj.x, qj.y, qj.z};
+      if (op != 0) y = d_normalize(d_mul(y, {ops64[4 * op], ops64[4 * op + 1], ops64[4 * op + 2], ops64[4 * op + 3]}));
+      if (sg) y = {-y.w, -y.x, -y.y, -y.z};
+      double c2 = 0;
+      for (int c = 0; c < 4; ++c) c2 += ((&x64.w)[c] - (&y.w)[c]) * ((&x64.w)[c] - (&y.w)[c]);
+      row[5] = std::min(row[5], c2);
+      near_pair64(x64, y, radius, row);
+    });
+    std::memcpy(rows + 7 * size_t(ii), row, sizeof(row));
+  }
+  return 0;
+}
+
+// Nearest orbit point within the cutoff for sources [first, first+count) through the near
+// grid: chord^2 in FP32, the source's own identity image excluded, ties to the lowest rank
+// source*2*order + 2*operator + sign (the spacing pass, pairs.zig:318-335); none within
+// the cutoff gives rank 0xffffffff and chord 4.
+void orc_nearest_grid(const float* q, uint32_t n, const float* ops, uint32_t order, float cut, double radius,
+                      uint32_t first, uint32_t count, uint32_t* rank_out, float* chord_out) {
+  NearGrid g = build_grid(q, n, ops, order, cut, radius);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t ii = 0; ii < int64_t(count); ++ii) {
+    uint32_t i = first + uint32_t(ii);
+    Q x = load(q, i);
+    float best = 4.0f;
+    uint32_t brank = 0xffffffffu;
+    const uint32_t self = i * 2 * order;
+    g.visit(x, [&](uint32_t k) {
+      float dm2 = chord2(x, g.site[k]);
+      uint32_t r = g.rank[k];
+      if (!(dm2 <= cut) || r == self) return;
+      if (dm2 < best || (dm2 == best && r < brank)) {
+        best = dm2;
+        brank = r;
+      }
+    });
+    rank_out[ii] = brank;
+    chord_out[ii] = best;
+  }
 }
 
 // Nearest orbit point of each source by brute force: chord^2 in FP32 over every

@@ -212,6 +212,76 @@ def test_spacing_indices_near_field(operator_fixture, group, count):
     np.testing.assert_array_equal(src * 2 * order + op * 2 + sign, rank_ref)
 
 
+def test_spacing_ten_million_indices_match_oracle(operator_fixture):
+    """Spacing of 1e7 Haar orientations of I (seed 193): every index in range and its chord
+    consistent, and two ranges of 20,000 points bit-exact against the oracle's grid search
+    of the same near field (4 mean radii)."""
+    n, group = 10_000_000, "I"
+    note("spacing I 1e7")
+    ops = ops32(operator_fixture, group)
+    order = len(ops)
+    with zo.haar(group, n, 193) as grid:
+        stats = grid.spacing()
+        src, op, sign = grid.nearest_indices()
+        values = grid.point_values("separation")
+        q = grid.quaternions
+    rank = (src * 2 * order + op * 2 + sign).astype(np.int64)
+    assert (src >= 0).all() and (src < n).all() and (src != np.arange(n)).any()
+    assert np.isfinite(values).all() and (values > 0).all()
+    assert stats["separation_deg"] == pytest.approx(float(values.min()))
+    cut, r = dense.near_cut(n, order, 4)
+    for first in (0, 6_543_210):
+        note("oracle nearest grid", first)
+        ref_rank, ref_chord = oracle.nearest_grid(q, ops, float(cut[0]), r, first=first, count=20_000)
+        np.testing.assert_array_equal(rank[first:first + 20_000], ref_rank.astype(np.int64))
+        np.testing.assert_allclose(values[first:first + 20_000], oracle.degrees(ref_chord), rtol=2e-6, atol=1e-6)
+    # size-independent: recompute a sample of the reported chords from the indices
+    pick = np.random.default_rng(1).choice(n, 2000, replace=False)
+    images = oracle.images(q[src[pick]], ops)  # (order, len(pick), 4)
+    y = images[op[pick], np.arange(len(pick))].astype(np.float64)
+    y = np.where(sign[pick, None] == 1, -y, y)
+    chord = ((q[pick].astype(np.float64) - y) ** 2).sum(1)
+    np.testing.assert_allclose(oracle.degrees(chord.astype(np.float32)), values[pick], rtol=1e-4, atol=1e-4)
+
+
+def test_relaxed_rows_against_fp64_subsample(operator_fixture):
+    """Group O, N=1e5 Haar (seed 193), near-field relax (neighbourhood 20): after 100 BB
+    steps the GPU's FP32 pair terms with FP64 sums are measured against FP64 pair terms
+    over the same pair set for 10,000 sources.  Bars: energy sum within 1e-6 relative,
+    every row's energy and curvature within 1e-5, the separation within 1e-3 degrees, and
+    the force error within 2e-6 of the pair-force scale (sum of |pair force|: a relaxed
+    row's net force is a small difference of large terms) and no more than twice the
+    error of the reference's own FP32 formulas (the oracle's FP32 rows)."""
+    n, group = 100_000, "O"
+    note("fp64 subsample O 1e5")
+    ops = ops32(operator_fixture, group)
+    ops64 = np.asarray(operator_fixture[group], np.float64)
+    with zo.haar(group, n, 193) as grid:
+        grid.relax(100, neighbourhood=20)
+        force, energy, nearest = grid.force_rows()
+        q = grid.quaternions
+    cut, r = dense.near_cut(n, len(ops), 20)
+    ref = oracle.near_grid64(q, ops, ops64, cut, r, first=0, count=10_000)
+    e_err = abs(energy[:10_000].sum() - ref[:, 4].sum()) / ref[:, 4].sum()
+    note("energy sum rel err", e_err)
+    assert e_err <= 1e-6
+    np.testing.assert_allclose(energy[:10_000], ref[:, 4], rtol=1e-5)
+    np.testing.assert_allclose(force[:10_000, 3], ref[:, 3], rtol=1e-5)
+    diff = np.linalg.norm(force[:10_000, :3] - ref[:, :3], axis=1)
+    f_err = np.linalg.norm(diff) / np.linalg.norm(ref[:, :3])
+    scaled = np.linalg.norm(diff) / np.linalg.norm(ref[:, 6])
+    rows32, _ = oracle.near_grid(q, ops, cut, r, first=0, count=10_000)
+    ref_err = np.linalg.norm(rows32[:, :3] - ref[:, :3]) / np.linalg.norm(ref[:, 6])
+    note("force rel err (net)", f_err, "vs pair-force scale", scaled, "oracle fp32 vs scale", ref_err)
+    assert scaled <= 2e-6
+    assert scaled <= 2 * ref_err
+    sep = oracle.degrees(nearest[:10_000])
+    sep64 = 4 * np.degrees(np.arcsin(np.minimum(1, np.sqrt(ref[:, 5]) / 2)))
+    note("row energy max rel err", np.abs(energy[:10_000] / ref[:, 4] - 1).max(),
+         "separation max abs err deg", np.abs(sep - sep64).max(), "min sep rel err", sep.min() / sep64.min() - 1)
+    assert np.abs(sep - sep64).max() <= 1e-3
+
+
 @pytest.mark.parametrize("group", GROUPS)
 def test_measure_matches_dense(group):
     """test_measures.py:46-92: Quey energy, separation and force RMS of 29 points."""

@@ -118,6 +118,46 @@ def test_near_grid_pair_set_is_brute_force(group, count, delta):
     assert stats["candidates"] >= stats["accepted"] > 0
 
 
+@pytest.mark.parametrize("group,count,delta", [("O", 3000, 4), ("I", 2000, 4), ("C2", 4000, 4), ("D6", 2500, 8),
+                                               ("T", 1500, 20)])
+def test_nearest_grid_is_brute_force_within_the_cutoff(group, count, delta):
+    """The spacing pass's grid search returns exactly what the brute-force scan of every
+    orbit point within the cutoff returns: ranks and FP32 chords, ties to the lowest rank."""
+    q = haar_folded(group, count, 21)
+    ops = OPS[group]
+    cut, r = dense.near_cut(count, len(ops), delta)
+    rank, chord = oracle.nearest_grid(q, ops, cut[0], r)
+    ref_rank, ref_chord = oracle.nearest(q, ops, cut[0])
+    np.testing.assert_array_equal(chord, ref_chord)
+    np.testing.assert_array_equal(rank, ref_rank)
+    # and a sub-range of sources gives the same answers
+    r2, c2 = oracle.nearest_grid(q, ops, cut[0], r, first=count // 3, count=count // 4)
+    np.testing.assert_array_equal(r2, ref_rank[count // 3:count // 3 + count // 4])
+    np.testing.assert_array_equal(c2, ref_chord[count // 3:count // 3 + count // 4])
+
+
+@pytest.mark.parametrize("group,count", [("O", 700), ("C2", 900), ("I", 400)])
+def test_near_grid64_is_the_dense_fp64_model(operator_fixture, group, count):
+    """FP64 pair terms over the grid's pair set agree with the dense FP64 near-field model
+    (tests/test_near_field.py's reference, restated in dense_f64.near_rows) to FP64
+    rounding, and the FP32 rows to FP32 rounding."""
+    q = haar_folded(group, count, 5)
+    ops = OPS[group]
+    ops64 = np.asarray(operator_fixture[group], np.float64)
+    cut, r = dense.near_cut(count, len(ops), 20)
+    rows64 = oracle.near_grid64(q, ops, ops64, cut, r)
+    ref = dense.near_rows(q, ops64, 20)  # energy, nearest, fx, fy, fz, curvature
+    # The dense model also normalises the identity images; a 1e-7 change of a point moves a
+    # close pair's chord by that much, so agreement is at that level, and FP64-tight in sums.
+    assert abs(rows64[:, 4].sum() - ref[:, 0].sum()) <= 1e-9 * ref[:, 0].sum()
+    np.testing.assert_allclose(rows64[:, 4], ref[:, 0], rtol=1e-6)
+    np.testing.assert_allclose(rows64[:, 3], ref[:, 5], rtol=1e-5)
+    assert np.linalg.norm(rows64[:, :3] - ref[:, 2:5]) <= 1e-6 * np.linalg.norm(ref[:, 2:5])
+    rows32, _ = oracle.near_grid(q, ops, cut, r)
+    np.testing.assert_allclose(rows32[:, 4], rows64[:, 4], rtol=2e-6)
+    assert np.linalg.norm(rows32[:, :3] - rows64[:, :3]) <= 2e-5 * np.linalg.norm(rows64[:, :3])
+
+
 @pytest.mark.parametrize("group,count", [("O", 300), ("I", 120), ("C2", 500), ("D3", 400)])
 def test_nearest_ranks_match_fp64(group, count):
     q = haar_folded(group, count, 9)
