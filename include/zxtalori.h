@@ -141,6 +141,9 @@ const uint32_t* zo_batch_point_indices(zxt_batch* b, uint64_t id);
 uint64_t zo_batch_bytes(zxt_batch* b, uint64_t id);
 int32_t zo_batch_remove(zxt_batch* b, uint64_t id);
 uint32_t zo_batch_running(zxt_batch* b);
+/* Blocks until at most `most` steps of this batch are unfinished on the device (a serving
+ * loop keeps each batch a step or two ahead of the device); in_flight receives how many are. */
+int32_t zo_batch_wait(zxt_batch* b, uint32_t most, uint32_t* in_flight);
 /* out2[0]: steps captured into a graph, out2[1]: steps replayed from one. */
 int32_t zo_batch_graphs(zxt_batch* b, uint64_t* out2);
 

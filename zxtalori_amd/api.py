@@ -83,6 +83,7 @@ def _library():
         "zo_batch_bytes": ([ptr, C.c_uint64], C.c_uint64),
         "zo_batch_remove": ([ptr, C.c_uint64], C.c_int),
         "zo_batch_running": ([ptr], C.c_uint32),
+        "zo_batch_wait": ([ptr, C.c_uint32, C.POINTER(C.c_uint32)], C.c_int),
         "zo_batch_graphs": ([ptr, C.POINTER(C.c_uint64)], C.c_int),
     }
     for name, (args, result) in signatures.items():

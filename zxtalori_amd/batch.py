@@ -94,6 +94,13 @@ class Batch:
             self.step()
         return self
 
+    def wait(self, most=1):
+        """Blocks until at most `most` steps are unfinished on the device; returns how many are."""
+        self._open()
+        left = C.c_uint32(0)
+        self._check(self._lib.zo_batch_wait(self._handle, int(most), C.byref(left)))
+        return int(left.value)
+
     @property
     def running(self):
         self._open()

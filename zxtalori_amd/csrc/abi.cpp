@@ -296,6 +296,13 @@ int32_t zo_batch_remove(zxt_batch* b, uint64_t id) {
 
 uint32_t zo_batch_running(zxt_batch* b) { return b->batch.running(); }
 
+int32_t zo_batch_wait(zxt_batch* b, uint32_t most, uint32_t* in_flight) {
+  return guarded_batch(b, "Batch wait", [&] {
+    b->batch.wait(most);
+    if (in_flight) *in_flight = b->batch.in_flight();
+  });
+}
+
 int32_t zo_batch_graphs(zxt_batch* b, uint64_t* out2) {
   out2[0] = b->batch.captures();
   out2[1] = b->batch.replays();

@@ -34,6 +34,8 @@ Batch::~Batch() {
     if (t->host_desc) (void)hipHostFree(t->host_desc);
     if (t->host_start) (void)hipHostFree(t->host_start);
   }
+  for (hipEvent_t e : inflight_) (void)hipEventDestroy(e);
+  for (hipEvent_t e : spare_) (void)hipEventDestroy(e);
   for (hipStream_t s : aux_) (void)hipStreamDestroy(s);
   for (hipEvent_t e : join_) (void)hipEventDestroy(e);
   if (fork_) (void)hipEventDestroy(fork_);
@@ -292,6 +294,7 @@ uint32_t Batch::step() {
     err = e.message;
     return completed + uint32_t(active.size());
   }
+  mark_step();
   for (Job* j : active) {
     j->engine->stepped();
     if (--j->remaining == 0) {
@@ -318,6 +321,35 @@ void Batch::remove(uint64_t id) {
     }
   }
   fail("UnknownJob");
+}
+
+// Steps in flight: an event recorded after each step's launches.
+void Batch::mark_step() {
+  hipEvent_t ev = nullptr;
+  if (!spare_.empty()) {
+    ev = spare_.back();
+    spare_.pop_back();
+  } else {
+    Engine::check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
+  }
+  Engine::check(hipEventRecord(ev, stream), "event");
+  inflight_.push_back(ev);
+}
+
+uint32_t Batch::in_flight() {
+  while (!inflight_.empty() && hipEventQuery(inflight_.front()) == hipSuccess) {
+    spare_.push_back(inflight_.front());
+    inflight_.pop_front();
+  }
+  return uint32_t(inflight_.size());
+}
+
+void Batch::wait(uint32_t most) {
+  while (in_flight() > most) {
+    Engine::check(hipEventSynchronize(inflight_.front()), "wait");
+    spare_.push_back(inflight_.front());
+    inflight_.pop_front();
+  }
 }
 
 uint32_t Batch::running() const {

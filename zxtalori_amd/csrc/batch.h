@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <deque>
 #include <memory>
 #include <string>
 #include <vector>
@@ -44,6 +45,10 @@ class Batch {
   void remove(uint64_t id);
   uint32_t running() const;
   Engine& engine(uint64_t id) const { return *find(id)->engine; }
+  // Steps whose launches have not finished on the device; wait() blocks until at most
+  // `most` are left (a serving loop keeps each batch at most a step or two ahead).
+  uint32_t in_flight();
+  void wait(uint32_t most);
   // Steps captured into a graph, and steps replayed from one.
   uint64_t captures() const { return captures_; }
   uint64_t replays() const { return replays_; }
@@ -71,6 +76,9 @@ class Batch {
     JobSet set;
   };
   Job* find(uint64_t id) const;
+  void mark_step();
+  std::deque<hipEvent_t> inflight_;
+  std::vector<hipEvent_t> spare_;
   JobSet upload(Table& t, const std::vector<JobDesc>& descs);
   void finish(Job& j, bool recorded);
 

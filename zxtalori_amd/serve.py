@@ -10,6 +10,7 @@ a request's arrival to the end of the step that finished it.
 """
 from __future__ import annotations
 
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -91,10 +92,10 @@ def mixed_relax_jobs(total=256, seed=193, count_range=(500, 200_000), steps=50, 
 
 
 def run_batched(jobs, *, graphs=True, keep_results=False, clock=time.perf_counter):
-    """All jobs admitted at once into one Batch and stepped to the end (no budget)."""
+    """All jobs admitted at once and stepped to the end (no memory budget)."""
     with Server(hbm_budget=1 << 62, graphs=graphs, keep_results=keep_results) as server:
         outcomes, stats = server.run(jobs, realtime=False, clock=clock)
-        stats["graph_captures"], stats["graph_replays"] = server.batch.graphs
+        stats["graph_captures"], stats["graph_replays"] = server.graphs()
     return outcomes, stats
 
 
@@ -115,18 +116,60 @@ def summary(outcomes, elapsed):
             "point_steps_per_s": points / elapsed if elapsed > 0 else None, "p50_ms": p50, "p99_ms": p99}
 
 
-class Server:
-    """Admits requests by HBM budget into one Batch and steps it until all are served."""
+class _Lane:
+    """One Batch and the host thread that steps it."""
 
-    def __init__(self, hbm_budget=64 << 30, *, graphs=True, keep_results=False):
+    def __init__(self, name, graphs, streams, limit=0):
+        self.name = name
+        self.limit = limit  # most jobs at once (0: no limit)
+        self.batch = Batch(graphs=graphs, streams=streams)
+        self.queue = []   # request indices, in arrival order
+        self.live = {}    # job id -> (request index, admitted time, bytes held)
+        self.max_live = 0
+        self.steps = 0
+
+
+class Server:
+    """Admits requests by HBM budget into batches and steps them until all are served.
+
+    Requests go to a lane by size: jobs whose steps cost more than `large_pairs` pair
+    evaluations (a near-field N of about 270,000 at neighbourhood 20) share the large lane,
+    the rest the small lane.  Each lane is a Batch stepped by its own host thread and kept
+    at most `ahead` steps ahead of the device, so the lanes' kernels overlap on the GPU
+    and a small job does not wait for a large job's steps.  Jobs within a lane advance
+    together, one BB step per lane step.  A large job fills the device by itself, so the
+    large lane runs one at a time, in arrival order: sharing steps among several would
+    only delay the first to finish.
+    """
+
+    def __init__(self, hbm_budget=64 << 30, *, graphs=True, keep_results=False, large_pairs=2e9, ahead=1):
         self.hbm_budget = int(hbm_budget)
         self.keep_results = keep_results
-        self.batch = Batch(graphs=graphs)
+        self.large_pairs = float(large_pairs)
+        self.ahead = int(ahead)
+        # The large lane's passes fill the device alone; the small lane spreads its jobs'
+        # near-field passes over two extra streams.
+        self.lanes = [_Lane("small", graphs, 2), _Lane("large", graphs, 0, limit=1)]
         self.peak_bytes = 0
         self.max_live = 0
+        self._lock = threading.Lock()
+        self._held = 0
+
+    @property
+    def batch(self):
+        """The small lane's batch (most jobs run there)."""
+        return self.lanes[0].batch
+
+    def graphs(self):
+        captured = replayed = 0
+        for lane in self.lanes:
+            c, r = lane.batch.graphs
+            captured, replayed = captured + c, replayed + r
+        return captured, replayed
 
     def close(self):
-        self.batch.close()
+        for lane in self.lanes:
+            lane.batch.close()
 
     def __enter__(self):
         return self
@@ -134,58 +177,96 @@ class Server:
     def __exit__(self, *exc):
         self.close()
 
+    def lane_of(self, req):
+        if req.kind == "relax":
+            per_step = req.count * 7500.0 if req.neighbourhood > 0 else req.count ** 2 * ORDERS[req.group.upper()]
+        else:
+            per_step = req.count * 7500.0
+        return self.lanes[1] if per_step > self.large_pairs else self.lanes[0]
+
     def run(self, requests, *, realtime=True, clock=time.perf_counter):
-        """Serves `requests` (sorted by arrival); with realtime=False every request counts
-        as arrived at the start.  Returns (outcomes in request order, summary)."""
-        pending = sorted(range(len(requests)), key=lambda k: requests[k].arrival)
+        """Serves `requests`; with realtime=False every request counts as arrived at the
+        start.  Returns (outcomes in request order, summary)."""
+        for lane in self.lanes:
+            lane.queue, lane.live = [], {}
+        for index in sorted(range(len(requests)), key=lambda k: requests[k].arrival):
+            self.lane_of(requests[index]).queue.append(index)
         outcomes = [None] * len(requests)
-        live = {}  # job id -> (request index, admitted time, bytes held)
         start = clock()
+        errors = []
+
+        def serve(lane):
+            try:
+                self._serve_lane(lane, requests, outcomes, realtime, clock, start)
+            except BaseException as e:  # surfaced after the join
+                errors.append(e)
+
+        threads = [threading.Thread(target=serve, args=(lane,), name=f"lane-{lane.name}") for lane in self.lanes]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+        self.max_live = max(lane.max_live for lane in self.lanes)
+        return outcomes, summary(outcomes, clock() - start)
+
+    def _serve_lane(self, lane, requests, outcomes, realtime, clock, start):
         head = 0
-        while head < len(pending) or live:
+        batch = lane.batch
+        while head < len(lane.queue) or lane.live:
             now = clock() - start
-            held = sum(b for _, _, b in live.values())
-            while head < len(pending):
-                req = requests[pending[head]]
+            while head < len(lane.queue):
+                req = requests[lane.queue[head]]
                 if realtime and req.arrival > now:
                     break
-                need = req.estimate_bytes()
-                if live and held + need > self.hbm_budget:
+                if lane.limit and len(lane.live) >= lane.limit:
                     break
-                job = self.batch.add(req.group, count=req.count, seed=req.seed, kind=req.kind, steps=req.steps,
-                                     neighbourhood=req.neighbourhood)
-                live[job] = (pending[head], now, need)
-                held += need
+                need = req.estimate_bytes()
+                with self._lock:
+                    if lane.live and self._held + need > self.hbm_budget:
+                        break
+                    self._held += need
+                    self.peak_bytes = max(self.peak_bytes, self._held)
+                job = batch.add(req.group, count=req.count, seed=req.seed, kind=req.kind, steps=req.steps,
+                                neighbourhood=req.neighbourhood)
+                lane.live[job] = (lane.queue[head], now, need)
                 head += 1
-            if not live:
-                # Idle until the next arrival.
-                wait = requests[pending[head]].arrival - (clock() - start)
-                if wait > 0:
-                    time.sleep(min(wait, 0.05))
+            if not lane.live:
+                # Idle until the next arrival (or until memory frees up).
+                wait = requests[lane.queue[head]].arrival - (clock() - start) if realtime else 0.0
+                time.sleep(min(max(wait, 0.0005), 0.02))
                 continue
-            self.max_live = max(self.max_live, len(live))
-            self.batch.step()
+            lane.max_live = max(lane.max_live, len(lane.live))
+            batch.wait(self.ahead)
+            batch.step()
+            lane.steps += 1
             t = clock() - start
-            for job in list(live):
-                state = self.batch.status(job)
-                index, admitted, need = live[job]
+            for job in list(lane.live):
+                state = batch.status(job)
+                index, admitted, need = lane.live[job]
                 if state == "running":
-                    live[job] = (index, admitted, max(need, self.batch.device_bytes(job)))
+                    held = max(need, batch.device_bytes(job))
+                    if held != need:
+                        with self._lock:
+                            self._held += held - need
+                            self.peak_bytes = max(self.peak_bytes, self._held)
+                        lane.live[job] = (index, admitted, held)
                     continue
                 req = requests[index]
-                out = Outcome(req, state, self.batch.error(job) if state == "failed" else "", admitted, t)
+                out = Outcome(req, state, batch.error(job) if state == "failed" else "", admitted, t)
                 if state == "done":
-                    out.stats = self.batch.stats(job)
+                    out.stats = batch.stats(job)
                     if self.keep_results:
-                        out.quaternions = self.batch.quaternions(job)
-                        out.separation = self.batch.point_values(job, "separation")
+                        out.quaternions = batch.quaternions(job)
+                        out.separation = batch.point_values(job, "separation")
                         if req.kind == "spacing":
-                            out.indices = self.batch.point_indices(job)
+                            out.indices = batch.point_indices(job)
                 outcomes[index] = out
-                self.batch.remove(job)
-                del live[job]
-            self.peak_bytes = max(self.peak_bytes, sum(b for _, _, b in live.values()))
-        return outcomes, summary(outcomes, clock() - start)
+                batch.remove(job)
+                del lane.live[job]
+                with self._lock:
+                    self._held -= need
 
 
 def serve_plain(requests, *, realtime=True, keep_results=False, clock=time.perf_counter):
