@@ -60,12 +60,40 @@ __device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
   return base + mbcnt(mask);
 }
 
+// Cross-lane steps by DPP (immediate lane patterns: no address registers, which the
+// ds_bpermute shuffles would keep live across the loops that call these).  row_shr:n
+// within each 16-lane row, then row_bcast:15 and row_bcast:31 carry row totals upward;
+// lanes without a source lane take 0.
+template <int CTRL, int ROWS>
+__device__ inline uint32_t dpp_u32(uint32_t v) {
+  return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), CTRL, ROWS, 0xf, false));
+}
+template <int CTRL, int ROWS>
+__device__ inline double dpp_f64(double v) {
+  const uint64_t b = uint64_t(__double_as_longlong(v));
+  const uint32_t lo = dpp_u32<CTRL, ROWS>(uint32_t(b)), hi = dpp_u32<CTRL, ROWS>(uint32_t(b >> 32));
+  return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+
+// Inclusive prefix sum over the wave (lane i: v_0 + ... + v_i).
 __device__ inline uint32_t wave_incl_scan(uint32_t v) {
-  const uint32_t lane = lane_id();
-  for (uint32_t d = 1; d < 64; d <<= 1) {
-    uint32_t o = __shfl_up(v, d);
-    if (lane >= d) v += o;
-  }
+  v += dpp_u32<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_u32<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_u32<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_u32<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_u32<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_u32<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+
+// The wave's FP64 sum, in lane 63 (the same order every run).
+__device__ inline double wave_sum_to_last(double v) {
+  v += dpp_f64<0x111, 0xf>(v);
+  v += dpp_f64<0x112, 0xf>(v);
+  v += dpp_f64<0x114, 0xf>(v);
+  v += dpp_f64<0x118, 0xf>(v);
+  v += dpp_f64<0x142, 0xa>(v);
+  v += dpp_f64<0x143, 0xc>(v);
   return v;
 }
 
@@ -73,6 +101,15 @@ template <typename T>
 __device__ inline T wave_sum(T v) {
   for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
   return v;
+}
+
+// acc + (sum of v over the wave) in FP64, returned as a wave-uniform value (scalar
+// registers).
+__device__ inline double uniform_add(double acc, float v) {
+  const uint64_t bits = uint64_t(__double_as_longlong(wave_sum_to_last(double(v))));
+  const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(bits), 63);
+  const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(bits >> 32), 63);
+  return acc + __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
 }
 
 __device__ inline float wave_min(float v) {
@@ -1130,8 +1167,8 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 
 // ---- near-field rows (pairs.wgsl:363-427) -------------------------------------------
 struct WaveLds {
-  float4 qy[128];
-  float qd[128];
+  float4 qe[128];  // x - y of each queued site
+  float qd[128];   // |x - y|^2
 };
 
 __device__ inline float row_gap(float c, int idx, uint32_t dim) {
@@ -1157,7 +1194,7 @@ __device__ inline float row_gap(float c, int idx, uint32_t dim) {
 // image is ever a candidate: then the candidate stage needs no site codes, the only site
 // to skip is the source's own identity image (its sorted position), and the nearest chord
 // is taken over the compacted pairs.
-template <int MODE, bool OWN>
+template <int MODE, bool OWN, int TERMS>
 __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint32_t n,
                                               const uint32_t* __restrict__ order, const float4* __restrict__ sq,
                                               const uint32_t* __restrict__ scode, const uint32_t* __restrict__ srank,
@@ -1169,7 +1206,9 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   __shared__ WaveLds lds[kBlock / 64];
   const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
   if (vb >= real_blocks) return;
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // The wave index is uniform: the source, its position and every per-source constant
+  // below then live in scalar registers, leaving the VGPRs to the candidate and pair work.
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const uint32_t slot = vb * (kBlock / 64) + wave;
   if (slot >= n) return;  // the whole wave leaves; no workgroup barrier follows
   WaveLds& L = lds[wave];
@@ -1194,8 +1233,8 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   const uint32_t self_code = src | 0x80000000u, self_rank = src * two_order;
   const uint32_t self_site = (MODE == 0 && !OWN) ? self_pos[src] : 0u;
 
-  PairSum ps{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  NearSum ps{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   float nearest = 4.0f;
   uint64_t best = ~uint64_t(0);
   uint32_t qhead = 0, qcount = 0, rounds = 0;
@@ -1204,16 +1243,18 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   auto process = [&](uint32_t e) {
     const float dm2 = L.qd[e];
     if (!OWN) nearest = fminf(nearest, dm2);
-    near_pair_terms(x, L.qy[e], dm2, cut.taper_start, cut.inv_width, ps);
+    near_pair_terms<TERMS>(L.qe[e], dm2, cut.taper_start, cut.inv_width, ps);
     ++pairs;
   };
+  // Every 16 rounds the lanes' FP32 sums (16 pairs each) go into the wave's FP64 sums.
   auto flush = [&]() {
-    acc[0] += ps.fx;
-    acc[1] += ps.fy;
-    acc[2] += ps.fz;
-    acc[3] += ps.curvature;
-    acc[4] += ps.energy;
-    ps = PairSum{0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    acc[0] = uniform_add(acc[0], ps.gx);
+    acc[1] = uniform_add(acc[1], ps.gy);
+    acc[2] = uniform_add(acc[2], ps.gz);
+    acc[3] = uniform_add(acc[3], ps.gw);
+    acc[4] = uniform_add(acc[4], ps.curvature);
+    acc[5] = uniform_add(acc[5], ps.energy);
+    ps = NearSum{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   };
 
   for (uint32_t rb = 0; rb < nrows; rb += 64) {
@@ -1252,12 +1293,13 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
       const bool valid = v < total;
       const uint32_t site = mstart + (v - mexcl);
       if (MODE == 0) {
-        float4 y = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float dm2 = 4.0f;
         bool accept = false;
         if (valid) {
-          y = sq[site];
-          dm2 = chord2(x, y);
+          const float4 y = sq[site];
+          d = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
+          dm2 = dot4(d, d);  // chord2(x, y), the same operations
           const bool inside = dm2 <= cut.chord2;
           if (OWN) {
             // Every orbit point but the source itself bounds the nearest chord; the
@@ -1273,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
         if (mask) {
           if (accept) {
             const uint32_t e = (qhead + qcount + mbcnt(mask)) & 127u;
-            L.qy[e] = y;
+            L.qe[e] = d;
             L.qd[e] = dm2;
           }
           qcount = __builtin_amdgcn_readfirstlane(qcount + uint32_t(__popcll(mask)));
@@ -1303,12 +1345,14 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   if (MODE == 0) {
     if (lane < qcount) process((qhead + lane) & 127u);
     flush();
-    double f0 = wave_sum(acc[0]), f1 = wave_sum(acc[1]), f2 = wave_sum(acc[2]), f3 = wave_sum(acc[3]);
-    double e = wave_sum(acc[4]);
+    const double g[4] = {acc[0], acc[1], acc[2], acc[3]};
+    const double curvature = acc[4], e = acc[5];
     nearest = wave_min(nearest);
     pairs = wave_sum(pairs);
     if (lane == 0) {
-      rows.force[src] = make_float4(float(f0), float(f1), float(f2), float(f3));
+      double f[3];
+      near_force_of(x, g, f);
+      rows.force[src] = make_float4(float(f[0]), float(f[1]), float(f[2]), float(curvature));
       rows.energy[src] = e;
       rows.nearest[src] = nearest;
       counts[2 * size_t(src)] = pairs;
@@ -1343,14 +1387,28 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
                        Rows out) {
   const uint32_t real = blocks_for(n, kBlock / 64);
   p.run("near_force", [&] {
-    if (cut.own)
-      hipLaunchKernelGGL((k_near<0, true>), dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode,
-                         b.srank, b.offsets, b.info, b.self_pos, b.sdim, cut, 0u, real, out, nullptr, nullptr,
-                         b.counts);
-    else
-      hipLaunchKernelGGL((k_near<0, false>), dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode,
-                         b.srank, b.offsets, b.info, b.self_pos, b.sdim, cut, 0u, real, out, nullptr, nullptr,
-                         b.counts);
+    auto go = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.srank,
+                         b.offsets, b.info, b.self_pos, b.sdim, cut, 0u, real, out, nullptr, nullptr, b.counts);
+    };
+    const int terms = atan_terms(cut.chord2);
+    if (cut.own) {
+      switch (terms) {
+        case 2: go(k_near<0, true, 2>); break;
+        case 3: go(k_near<0, true, 3>); break;
+        case 4: go(k_near<0, true, 4>); break;
+        case 5: go(k_near<0, true, 5>); break;
+        default: go(k_near<0, true, 0>); break;
+      }
+    } else {
+      switch (terms) {
+        case 2: go(k_near<0, false, 2>); break;
+        case 3: go(k_near<0, false, 3>); break;
+        case 4: go(k_near<0, false, 4>); break;
+        case 5: go(k_near<0, false, 5>); break;
+        default: go(k_near<0, false, 0>); break;
+      }
+    }
   });
   launch_fill_u32(p, s, reinterpret_cast<uint32_t*>(b.counters), 0u, 4);
   p.run("count_reduce", [&] {
@@ -1363,7 +1421,7 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
                          const NearCut& cut, float* chord_out, uint32_t* rank_out) {
   const uint32_t real = blocks_for(n, kBlock / 64);
   p.run("near_nearest", [&] {
-    hipLaunchKernelGGL((k_near<1, true>), dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode,
+    hipLaunchKernelGGL((k_near<1, true, 0>), dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode,
                        b.srank, b.offsets, b.info, b.self_pos, b.sdim, cut, 2 * order, real, Rows{}, chord_out,
                        rank_out, nullptr);
   });

@@ -16,6 +16,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include <cstdint>
 
 namespace zxt {
@@ -123,16 +125,44 @@ __device__ __forceinline__ void pair_terms(const float4 x, const float4 y, float
   s.energy += energy;
 }
 
-// The near-field pair for a site within the cutoff: dm2 <= chord2 < 2, so y is the nearer
-// of +-y (sign +) and dp2 = 4 - dm2 for unit quaternions; then near + far = 4 and
-// 1/sin a = 2*rsq(near*far).  Otherwise the arithmetic of pair_terms<true>.
-__device__ __forceinline__ void near_pair_terms(const float4 x, const float4 y, float dm2, float taper_start,
-                                                float inv_width, PairSum& s) {
-  const float nr = fmaxf(dm2, 1e-30f);
+// atan(t) for small t: t times the Taylor series 1 - z/3 + z^2/5 - ... in z = t^2 up to
+// z^TERMS.  Within a near-field cutoff z <= tan(r/2)^2, and the launcher picks the fewest
+// terms whose truncation, z^(TERMS+1)/(2 TERMS + 3), stays below 5e-9 there (atan_terms);
+// TERMS = 0 selects the minimax polynomial on [0, 1].
+template <int TERMS>
+__device__ __forceinline__ float atan_small(float t) {
+  if (TERMS == 0) return atan01(t);
+  const float z = t * t;
+  float p = 0.0f;
+  if (TERMS >= 5) p = fmaf(z, p, -0.09090909f);
+  if (TERMS >= 4) p = fmaf(z, p, 0.11111111f);
+  if (TERMS >= 3) p = fmaf(z, p, -0.14285715f);
+  if (TERMS >= 2) p = fmaf(z, p, 0.2f);
+  p = fmaf(z, p, -0.33333334f);
+  p = fmaf(z, p, 1.0f);
+  return t * p;
+}
+
+// Per-lane sums of near-field pairs.  The force direction of a pair is linear in the
+// site, M(x) y = x0 y_v - y0 x_v + x_v x y_v with M(x) x = 0, so it is also -M(x) (x - y):
+// the kernel sums g += coef (x - y) over the pairs and applies -M(x) once per source
+// (near_force_of), 4 FMAs per pair instead of 15.
+struct NearSum {
+  float gx, gy, gz, gw, curvature, energy;
+};
+
+// The near-field pair for a site y within the cutoff, from e = x - y and dm2 = |e|^2:
+// dm2 <= chord2 < 2, so y is the nearer of +-y (sign +) and dp2 = 4 - dm2 for unit
+// quaternions; then near + far = 4 and 1/sin a = 2 rsq(near far).  Otherwise the
+// arithmetic of pair_terms<true> (pairs.wgsl:246-262, :349-361).
+template <int TERMS>
+__device__ __forceinline__ void near_pair_terms(const float4 e, float dm2, float taper_start, float inv_width,
+                                                NearSum& s) {
+  const float nr = __builtin_amdgcn_fmed3f(dm2, 1e-30f, 4.0f);
   const float fr = 4.0f - dm2;
   const float R = __builtin_amdgcn_rsqf(nr * fr);
   const float t = nr * R;
-  const float a = fmaxf(2.0f * atan01(t), 1e-12f);
+  const float a = fmaxf(2.0f * atan_small<TERMS>(t), 1e-12f);
   const float b = kPiF - a;
   const float aa = a * a, bb = b * b;
   const float a2 = aa + 1e-9f, b2 = bb + 1e-9f;
@@ -147,14 +177,29 @@ __device__ __forceinline__ void near_pair_terms(const float4 x, const float4 y, 
   const float w = fmaf(u * u, fmaf(2.0f, u, -3.0f), 1.0f);
   const float slope = -6.0f * inv_width * (u * (1.0f - u));
   const float coef = fmaf(w, (a2 - b2) * iP1, energy * slope * (2.0f * R));
-  const float d1 = fmaf(x.x, y.y, -y.x * x.y) + fmaf(x.z, y.w, -x.w * y.z);
-  const float d2 = fmaf(x.x, y.z, -y.x * x.z) + fmaf(x.w, y.y, -x.y * y.w);
-  const float d3 = fmaf(x.x, y.w, -y.x * x.w) + fmaf(x.y, y.z, -x.z * y.y);
-  s.fx = fmaf(coef, d1, s.fx);
-  s.fy = fmaf(coef, d2, s.fy);
-  s.fz = fmaf(coef, d3, s.fz);
+  s.gx = fmaf(coef, e.x, s.gx);
+  s.gy = fmaf(coef, e.y, s.gy);
+  s.gz = fmaf(coef, e.z, s.gz);
+  s.gw = fmaf(coef, e.w, s.gw);
   s.curvature = fmaf((a3 + b3) * iP2, w, s.curvature);
   s.energy = fmaf(energy, w, s.energy);
+}
+
+// The force of a source x from the summed g = sum coef (x - y): -M(x) g, in FP64.
+__host__ __device__ inline void near_force_of(const float4 x, const double g[4], double f[3]) {
+  const double x0 = x.x, x1 = x.y, x2 = x.z, x3 = x.w;
+  f[0] = x1 * g[0] - x0 * g[1] + x3 * g[2] - x2 * g[3];
+  f[1] = x2 * g[0] - x3 * g[1] - x0 * g[2] + x1 * g[3];
+  f[2] = x3 * g[0] + x2 * g[1] - x1 * g[2] - x0 * g[3];
+}
+
+// Terms of atan_small for a cutoff chord^2: the fewest with truncation error below 5e-9
+// for z up to chord2 / (4 - chord2) = tan(r/2)^2 (0: the minimax polynomial).
+inline int atan_terms(float chord2) {
+  const double z = double(chord2) / (4.0 - double(chord2)) * 1.0001;
+  for (int k = 2; k <= 5; ++k)
+    if (std::pow(z, k + 1) / (2 * k + 3) <= 5e-9) return k;
+  return 0;
 }
 
 }  // namespace zxt
