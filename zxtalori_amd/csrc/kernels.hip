@@ -1285,10 +1285,11 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
       for (uint32_t rr = row + 1; rr < 64; ++rr) {
         const uint32_t e = uint32_t(__builtin_amdgcn_readlane(int(excl), int(rr)));
         if (e > cb + 63) break;
-        if (v >= e) {
-          mstart = uint32_t(__builtin_amdgcn_readlane(int(start), int(rr)));
-          mexcl = e;
-        }
+        // branchless: a row starting inside the chunk takes the lanes from its start on
+        const uint32_t s_rr = uint32_t(__builtin_amdgcn_readlane(int(start), int(rr)));
+        const bool take = v >= e;
+        mstart = take ? s_rr : mstart;
+        mexcl = take ? e : mexcl;
       }
       const bool valid = v < total;
       const uint32_t site = mstart + (v - mexcl);

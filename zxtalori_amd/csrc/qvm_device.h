@@ -132,14 +132,12 @@ __device__ __forceinline__ void pair_terms(const float4 x, const float4 y, float
 template <int TERMS>
 __device__ __forceinline__ float atan_small(float t) {
   if (TERMS == 0) return atan01(t);
+  // Horner from the highest kept term: coefficients (-1)^k / (2k + 1), k = TERMS..1.
   const float z = t * t;
-  float p = 0.0f;
-  if (TERMS >= 5) p = fmaf(z, p, -0.09090909f);
-  if (TERMS >= 4) p = fmaf(z, p, 0.11111111f);
-  if (TERMS >= 3) p = fmaf(z, p, -0.14285715f);
-  if (TERMS >= 2) p = fmaf(z, p, 0.2f);
-  p = fmaf(z, p, -0.33333334f);
-  p = fmaf(z, p, 1.0f);
+  constexpr float c[6] = {1.0f, -0.33333334f, 0.2f, -0.14285715f, 0.11111111f, -0.09090909f};
+  float p = c[TERMS];
+#pragma unroll
+  for (int k = TERMS - 1; k >= 0; --k) p = fmaf(z, p, c[k]);
   return t * p;
 }
 
