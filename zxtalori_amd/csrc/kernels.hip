@@ -116,6 +116,10 @@ __device__ inline float wave_min(float v) {
   for (int d = 32; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d));
   return v;
 }
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+  for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
+  return v;
+}
 __device__ inline uint32_t wave_max_u32(uint32_t v) {
   for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
   return v;
@@ -1235,16 +1239,17 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
 
   NearSum ps{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  float nearest = 4.0f;
+  // Chords are sums of squares (>= 0), so their bits order as unsigned integers: one
+  // integer min per update, no float canonicalisation.
+  uint32_t nearest_bits = __float_as_uint(4.0f);
   uint64_t best = ~uint64_t(0);
   uint32_t qhead = 0, qcount = 0, rounds = 0;
-  uint32_t pairs = 0, candidates = 0;  // accepted pair evaluations (per lane), sites tested
+  uint32_t candidates = 0;  // sites tested (uniform); accepted pairs are 64 rounds + the tail
 
   auto process = [&](uint32_t e) {
     const float dm2 = L.qd[e];
-    if (!OWN) nearest = fminf(nearest, dm2);
+    if (!OWN) nearest_bits = min(nearest_bits, __float_as_uint(dm2));
     near_pair_terms<TERMS>(L.qe[e], dm2, cut.taper_start, cut.inv_width, ps);
-    ++pairs;
   };
   // Every 16 rounds the lanes' FP32 sums (16 pairs each) go into the wave's FP64 sums.
   auto flush = [&]() {
@@ -1306,7 +1311,7 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
             // Every orbit point but the source itself bounds the nearest chord; the
             // source's own images exert no force (pairs.wgsl:404-408).
             const uint32_t code = scode[site];
-            if (inside && code != self_code) nearest = fminf(nearest, dm2);
+            if (inside && code != self_code) nearest_bits = min(nearest_bits, __float_as_uint(dm2));
             accept = inside && (code & 0x7fffffffu) != src;
           } else {
             accept = inside && site != self_site;
@@ -1348,8 +1353,8 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
     flush();
     const double g[4] = {acc[0], acc[1], acc[2], acc[3]};
     const double curvature = acc[4], e = acc[5];
-    nearest = wave_min(nearest);
-    pairs = wave_sum(pairs);
+    const float nearest = __uint_as_float(wave_min_u32(nearest_bits));
+    const uint32_t pairs = 64 * rounds + qcount;
     if (lane == 0) {
       double f[3];
       near_force_of(x, g, f);
