@@ -1277,36 +1277,23 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
         len = offsets[row + x1 + 1] - start;
       }
     }
-    const uint32_t incl = wave_incl_scan(len);
-    const uint32_t total = __shfl(incl, 63);
-    const uint32_t excl = incl - len;  // rows past nrows: len 0, excl = total
-    candidates += total;
-    uint32_t row = 0;  // uniform: the row holding the chunk's first candidate
-    for (uint32_t cb = 0; cb < total; cb += 64) {
-      while (row < 63 && uint32_t(__builtin_amdgcn_readlane(int(excl), int(row + 1))) <= cb) ++row;
-      const uint32_t v = cb + lane;
-      uint32_t mstart = uint32_t(__builtin_amdgcn_readlane(int(start), int(row)));
-      uint32_t mexcl = uint32_t(__builtin_amdgcn_readlane(int(excl), int(row)));
-      for (uint32_t rr = row + 1; rr < 64; ++rr) {
-        const uint32_t e = uint32_t(__builtin_amdgcn_readlane(int(excl), int(rr)));
-        if (e > cb + 63) break;
-        // branchless: a row starting inside the chunk takes the lanes from its start on
-        const uint32_t s_rr = uint32_t(__builtin_amdgcn_readlane(int(start), int(rr)));
-        const bool take = v >= e;
-        mstart = take ? s_rr : mstart;
-        mexcl = take ? e : mexcl;
-      }
-      const bool valid = v < total;
-      const uint32_t site = mstart + (v - mexcl);
-      if (MODE == 0) {
-        float4 d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        float dm2 = 4.0f;
-        bool accept = false;
-        if (valid) {
+    candidates += uint32_t(__builtin_amdgcn_readlane(int(wave_incl_scan(len)), 63));
+    // Row by row (uniform): a row's sites are contiguous, so a lane's site is the row start
+    // plus its offset; the last chunk of a row runs partly empty, which the LDS ring
+    // absorbs (the pair arithmetic still runs on full waves).
+    for (uint32_t rr = 0; rr < 64; ++rr) {
+      const uint32_t n_row = uint32_t(__builtin_amdgcn_readlane(int(len), int(rr)));
+      if (n_row == 0) continue;
+      const uint32_t s_row = uint32_t(__builtin_amdgcn_readlane(int(start), int(rr)));
+      for (uint32_t c = 0; c < n_row; c += 64) {
+        const bool valid = c + lane < n_row;
+        const uint32_t site = s_row + min(c + lane, n_row - 1);  // in range for every lane
+        if (MODE == 0) {
           const float4 y = sq[site];
-          d = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
-          dm2 = dot4(d, d);  // chord2(x, y), the same operations
-          const bool inside = dm2 <= cut.chord2;
+          const float4 d = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
+          const float dm2 = dot4(d, d);  // chord2(x, y), the same operations
+          const bool inside = valid && dm2 <= cut.chord2;
+          bool accept;
           if (OWN) {
             // Every orbit point but the source itself bounds the nearest chord; the
             // source's own images exert no force (pairs.wgsl:404-408).
@@ -1316,31 +1303,29 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
           } else {
             accept = inside && site != self_site;
           }
-        }
-        const uint64_t mask = __ballot(accept);
-        if (mask) {
-          if (accept) {
-            const uint32_t e = (qhead + qcount + mbcnt(mask)) & 127u;
-            L.qe[e] = d;
-            L.qd[e] = dm2;
-          }
-          qcount = __builtin_amdgcn_readfirstlane(qcount + uint32_t(__popcll(mask)));
-          wave_sync();
-          if (qcount >= 64) {
-            process((qhead + lane) & 127u);
-            qhead = __builtin_amdgcn_readfirstlane((qhead + 64) & 127u);
-            qcount -= 64;
-            rounds = __builtin_amdgcn_readfirstlane(rounds + 1);
-            if ((rounds & 15u) == 0) flush();
+          const uint64_t mask = __ballot(accept);
+          if (mask) {
+            if (accept) {
+              const uint32_t e = (qhead + qcount + mbcnt(mask)) & 127u;
+              L.qe[e] = d;
+              L.qd[e] = dm2;
+            }
+            qcount = __builtin_amdgcn_readfirstlane(qcount + uint32_t(__popcll(mask)));
             wave_sync();
+            if (qcount >= 64) {
+              process((qhead + lane) & 127u);
+              qhead = __builtin_amdgcn_readfirstlane((qhead + 64) & 127u);
+              qcount -= 64;
+              rounds = __builtin_amdgcn_readfirstlane(rounds + 1);
+              if ((rounds & 15u) == 0) flush();
+              wave_sync();
+            }
           }
-        }
-      } else {
-        if (valid) {
+        } else {
           const float4 y = sq[site];
           const uint32_t rank = srank[site];
           const float dm2 = chord2(x, y);
-          if (dm2 <= cut.chord2 && rank != self_rank) {
+          if (valid && dm2 <= cut.chord2 && rank != self_rank) {
             const uint64_t key = (uint64_t(__float_as_uint(dm2)) << 32) | rank;
             best = key < best ? key : best;
           }
