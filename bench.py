@@ -125,12 +125,19 @@ def fsin(a):  # v_sin takes revolutions: one scale, one transcendental
     return F(math.sin(F._val(a)))
 
 
-def fatan01(t):
-    """The build's atan on [0, 1] (t*P(t^2), degree 8): what the pair kernel executes."""
+def fatan(t, terms):
+    """The build's atan as the pair kernel executes it for this cutoff: t times a Taylor
+    series of `terms` terms in t^2, or (terms 0) the degree-8 minimax polynomial."""
     z = t * t
-    p = fma(z, 0.0028498897, -0.01606863)
-    for c in (0.04269152, -0.07504295, 0.10640934, -0.14203644, 0.1999262, -0.33333072, 1.0):
-        p = fma(z, p, c)
+    if terms == 0:
+        p = fma(z, 0.0028498897, -0.01606863)
+        for c in (0.04269152, -0.07504295, 0.10640934, -0.14203644, 0.1999262, -0.33333072, 1.0):
+            p = fma(z, p, c)
+    else:
+        coef = [1.0, -1 / 3, 1 / 5, -1 / 7, 1 / 9, -1 / 11]
+        p = F(coef[terms])
+        for c in reversed(coef[:terms]):
+            p = fma(z, p, c)
     return t * p
 
 
@@ -138,9 +145,9 @@ def dot4(a, b):
     return fma(a[3], b[3], fma(a[2], b[2], fma(a[1], b[1], a[0] * b[0])))
 
 
-def canonical_near_pair(x, y, taper_start, inv_width):
+def canonical_near_pair(x, y, taper_start, inv_width, terms=0):
     """The reference's near-field pair (candidate chord, quey_pair, near_pair, row sums),
-    operation for operation, for counting."""
+    operation for operation, for counting; atan as the build executes it."""
     m = [x[i] - y[i] for i in range(4)]
     dm2 = dot4(m, m)
     p = [x[i] + y[i] for i in range(4)]
@@ -148,7 +155,7 @@ def canonical_near_pair(x, y, taper_start, inv_width):
     near = fmax(fmin(dm2, dp2), 1e-30)
     far = fmax(fmax(dm2, dp2), 1e-30)
     t = fsqrt(near / far)
-    a = fmax(2.0 * fatan01(t), 1e-12)
+    a = fmax(2.0 * fatan(t, terms), 1e-12)
     b = math.pi - a
     a2, b2 = fma(a, a, 1e-9), fma(b, b, 1e-9)
     a3, b3 = fma(a * a, a, 1e-9), fma(b * b, b, 1e-9)
@@ -174,13 +181,19 @@ def canonical_near_pair(x, y, taper_start, inv_width):
     return acc
 
 
-def pair_op_counts():
-    """(VALU, transcendental) per accepted near-field pair in the windowed band."""
-    Count.reset()
+def pair_op_counts(terms=0):
+    """(VALU, transcendental) per accepted near-field pair, averaged over the cutoff ball:
+    the window branch runs for the pairs in its outer half radius, 1 - (1/2)^3 = 7/8 of
+    them at uniform density, and is skipped for the inner 1/8 (pairs.wgsl:349-361)."""
     x = [F(v) for v in (0.9, 0.1, -0.3, 0.3)]
     y = [F(v) for v in (0.89, 0.12, -0.31, 0.31)]
-    canonical_near_pair(x, y, F(0.0), F(1.0))  # taper from 0: the window branch runs
-    return Count.valu, Count.trans
+    Count.reset()
+    canonical_near_pair(x, y, F(0.0), F(1.0), terms)  # taper from 0: the window runs
+    windowed = (Count.valu, Count.trans)
+    Count.reset()
+    canonical_near_pair(x, y, F(10.0), F(1.0), terms)  # taper past the pair: no window
+    plain = (Count.valu, Count.trans)
+    return tuple(plain[k] + 7 / 8 * (windowed[k] - plain[k]) for k in range(2))
 
 
 # ---- measurement ------------------------------------------------------------------
@@ -241,7 +254,6 @@ def main():
         sys.exit(2)
 
     import zxtalori_amd as zo
-    from oracle import dense_f64 as dense
 
     print(f"[bench] sample {args.group} N={args.n} delta={args.delta}", file=sys.stderr, flush=True)
     with zo.haar(args.group, 10_000_000, 193) as big:
@@ -263,7 +275,7 @@ def main():
     pairs = info["pairs"]
     ms_per_step = dt * 1e3 / args.steps
     value = pairs / (dt / args.steps)
-    valu, trans = pair_op_counts()
+    valu, trans = pair_op_counts(info["atan_terms"])
     kernel_count, kernel_ms = report.get("near_force", (0, float("nan")))
     kernel_s = kernel_ms / max(kernel_count, 1) / 1e3
     flop_equiv = 2.0 * (valu + trans * PEAK_VALU / PEAK_TRANS)
@@ -297,6 +309,7 @@ def main():
     }
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        from oracle import dense_f64 as dense  # the checker, for the CPU baseline leg only
         cut, radius = dense.near_cut(len(q), len(ops), args.delta)
         out["cpu_baseline"] = cpu_baseline(q, ops, cut, radius, args.cpu_sample)
     if args.batched_jobs > 0:

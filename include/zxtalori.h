@@ -102,6 +102,9 @@ int32_t zo_force_rows(zxt_engine* e, float* force4, double* energy, float* neare
 int32_t zo_near_pairs(zxt_engine* e, uint64_t* out2);
 /* The last near-field pass's site count, shell count and cell grid dimension. */
 int32_t zo_near_info(zxt_engine* e, uint32_t* out3);
+/* The last near-field pass's cutoff: chord^2, S3 radius r, and the number of Taylor terms
+ * of the pair kernel's arctangent for it (0: the minimax polynomial). */
+int32_t zo_near_cut(zxt_engine* e, float* out3);
 
 /* ---- Batches: many samples relaxed together ------------------------------------------
  * No counterpart in the reference, which runs one sample per engine and one call at a time

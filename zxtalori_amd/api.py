@@ -67,6 +67,7 @@ def _library():
         "zo_near_info": ([ptr, C.POINTER(C.c_uint32)], C.c_int),
         "zo_force_rows": ([ptr, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
         "zo_near_pairs": ([ptr, C.POINTER(C.c_uint64)], C.c_int),
+        "zo_near_cut": ([ptr, C.POINTER(C.c_float)], C.c_int),
         # batches (no reference counterpart)
         "zo_batch_create": ([C.c_uint32], ptr), "zo_batch_destroy": ([ptr], None),
         "zo_batch_error": ([ptr], C.c_char_p),
@@ -342,7 +343,10 @@ class Grid:
         self._lib.zo_near_info(self._handle, out)
         pairs = (C.c_uint64 * 2)()
         self._lib.zo_near_pairs(self._handle, pairs)
-        return {"sites": out[0], "shell": out[1], "sdim": out[2], "pairs": pairs[0], "candidates": pairs[1]}
+        cut = (C.c_float * 3)()
+        self._lib.zo_near_cut(self._handle, cut)
+        return {"sites": out[0], "shell": out[1], "sdim": out[2], "pairs": pairs[0], "candidates": pairs[1],
+                "chord2": cut[0], "radius": cut[1], "atan_terms": int(cut[2])}
 
 
 def generate(group="O", *, h=None, count=2048, exact=False, identity=False, fixed_identity=False, kind=None):

@@ -10,6 +10,7 @@
 #include "batch.h"
 #include "engine.h"
 #include "groups_table.h"
+#include "qvm_device.h"
 
 struct zxt_engine {
   zxt::Engine engine;
@@ -198,6 +199,13 @@ int32_t zo_near_info(zxt_engine* e, uint32_t* out3) {
   out3[0] = e->engine.last_near.sites;
   out3[1] = e->engine.last_near.sites >= e->engine.count() ? e->engine.last_near.sites - e->engine.count() : 0;
   out3[2] = e->engine.last_near_sdim;
+  return 0;
+}
+
+int32_t zo_near_cut(zxt_engine* e, float* out3) {
+  out3[0] = e->engine.last_near_cut.chord2;
+  out3[1] = float(e->engine.last_near_radius);
+  out3[2] = float(zxt::atan_terms(e->engine.last_near_cut.chord2));
   return 0;
 }
 

@@ -210,6 +210,8 @@ bool NearField::overflowed(Engine& e) {
   e.sync();
   e.last_near = status;
   e.last_near_sdim = sdim;
+  e.last_near_cut = cut;
+  e.last_near_radius = radius;
   if (!status.overflow) return false;
   ready = false;
   return true;

@@ -133,6 +133,8 @@ class Engine {
   NearInfo last_near{};
   uint32_t last_near_sdim = 0;
   unsigned long long last_near_counts[2] = {0, 0};  // accepted pairs, sites tested
+  NearCut last_near_cut{};
+  double last_near_radius = 0;
 
   // Sends this engine's launches to s (returns the previous stream); a batch runs the
   // near-field passes of its jobs on several streams this way.

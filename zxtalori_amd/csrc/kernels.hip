@@ -1251,7 +1251,7 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
     if (!OWN) nearest_bits = min(nearest_bits, __float_as_uint(dm2));
     near_pair_terms<TERMS>(L.qe[e], dm2, cut.taper_start, cut.inv_width, ps);
   };
-  // Every 16 rounds the lanes' FP32 sums (16 pairs each) go into the wave's FP64 sums.
+  // Every 64 rounds the lanes' FP32 sums (64 pairs each) go into the wave's FP64 sums.
   auto flush = [&]() {
     acc[0] = uniform_add(acc[0], ps.gx);
     acc[1] = uniform_add(acc[1], ps.gy);
@@ -1317,7 +1317,7 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
               qhead = __builtin_amdgcn_readfirstlane((qhead + 64) & 127u);
               qcount -= 64;
               rounds = __builtin_amdgcn_readfirstlane(rounds + 1);
-              if ((rounds & 15u) == 0) flush();
+              if ((rounds & 63u) == 0) flush();
               wave_sync();
             }
           }
