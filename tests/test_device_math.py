@@ -154,3 +154,121 @@ def test_duplicate_pair_stays_finite():
     f, c, e = kernel_pair(x, x, 0.0, 4.0)
     assert np.isfinite(f).all() and np.isfinite(c) and np.isfinite(e)
     assert (f == 0).all()
+
+
+# ---- the near-field form of the pair (near_pair_terms, near_force_of) ---------------
+def series_coefficients():
+    body = SRC[SRC.index("__device__ __forceinline__ float atan_small"):SRC.index("struct NearSum")]
+    return [float(x) for x in re.findall(r"(-?\d+\.\d+)f", body[body.index("c[6]"):])][:6]
+
+
+def atan_series(t, terms):
+    c = series_coefficients()
+    z = t * t
+    p = c[terms]
+    for k in range(terms - 1, -1, -1):
+        p = p * z + c[k]
+    return t * p
+
+
+def test_atan_series_within_its_cutoff():
+    """Each series length is used only for z = t^2 up to the bound atan_terms grants it;
+    there its error against the exact arctangent stays below 1e-8 relative."""
+    c = series_coefficients()
+    assert c[:2] == [1.0, -0.33333334] and len(c) == 6
+    for terms in range(2, 6):
+        zmax = (5e-9 * (2 * terms + 3)) ** (1 / (terms + 1))
+        t = np.linspace(1e-6, math.sqrt(zmax), 20001)
+        approx = atan_series(t, terms)
+        rel = np.abs(approx - np.arctan(t)) / np.arctan(t)
+        assert rel.max() < 1e-8, (terms, rel.max())
+        # one term fewer would not do over the same range
+        if terms > 2:
+            assert (np.abs(atan_series(t, terms - 1) - np.arctan(t)) / np.arctan(t)).max() > 5e-9
+
+
+def direction_M(x, e):
+    """-M(x) e as near_force_of applies it (e = sum of coef (x - y))."""
+    x0, x1, x2, x3 = x
+    return np.array([x1 * e[0] - x0 * e[1] + x3 * e[2] - x2 * e[3],
+                     x2 * e[0] - x3 * e[1] - x0 * e[2] + x1 * e[3],
+                     x3 * e[0] + x2 * e[1] - x1 * e[2] - x0 * e[3]])
+
+
+def test_direction_is_linear_in_the_difference():
+    """The reference's direction x0 y_v - y0 x_v + x_v x y_v equals -M(x)(x - y) for unit x
+    (M(x) x = 0), so summing coef (x - y) and applying -M(x) once gives the same force."""
+    rng = np.random.default_rng(11)
+    for _ in range(2000):
+        x = rng.normal(size=4)
+        x /= np.linalg.norm(x)
+        y = x + rng.normal(size=4) * 10 ** rng.uniform(-4, 0)
+        y /= np.linalg.norm(y)
+        d = x[0] * y[1:] - y[0] * x[1:] + np.cross(x[1:], y[1:])
+        np.testing.assert_allclose(direction_M(x, x - y), d, rtol=0, atol=1e-14)
+    # and the kernel's M is the one in the header (signs as near_force_of writes them)
+    body = SRC[SRC.index("inline void near_force_of"):]
+    body = body[:body.index("}")]
+    assert body.count("x1 * g[0] - x0 * g[1] + x3 * g[2] - x2 * g[3]") == 1
+    assert body.count("x2 * g[0] - x3 * g[1] - x0 * g[2] + x1 * g[3]") == 1
+    assert body.count("x3 * g[0] + x2 * g[1] - x1 * g[2] - x0 * g[3]") == 1
+
+
+def kernel_near_pair(x, y, taper_start, inv_width, terms):
+    """near_pair_terms in FP64: returns (force contribution via -M(x) of coef (x - y), curv, energy)."""
+    e = x - y
+    dm2 = float(e @ e)
+    nr = min(max(dm2, 1e-30), 4.0)
+    fr = 4.0 - dm2
+    R = 1 / math.sqrt(nr * fr)
+    t = nr * R
+    a = max(2 * (atan_series(t, terms) if terms else atan_poly(t)), 1e-12)
+    b = math.pi - a
+    aa, bb = a * a, b * b
+    a2, b2 = aa + 1e-9, bb
+    a3, b3 = aa * a + 1e-9, bb * b + 1e-9
+    sine = 0.5 * t * fr + 1e-9
+    P1, P2, P3 = a2 * b2 * sine, a3 * b3, a * b
+    D = 1 / (P1 * P2 * P3)
+    iP1, iP2, iP3 = D * P2 * P3, D * P1 * P3, D * P1 * P2
+    energy = math.pi * iP3
+    u = min(max((a - taper_start) * inv_width, 0.0), 1.0)
+    w = (u * u) * (2 * u - 3) + 1
+    slope = -6 * inv_width * (u * (1 - u))
+    coef = w * (a2 - b2) * iP1 + energy * slope * (2 * R)
+    return direction_M(x, coef * e), (a3 + b3) * iP2 * w, energy * w
+
+
+def test_near_pair_form_is_the_reference_pair():
+    """Within a cutoff (near side, r up to 1.5 rad) the kernel's near-field pair equals the
+    reference's windowed pair (with the same arctangent) to FP64 rounding, and the series
+    arctangent moves it by less than the series bound."""
+    rng = np.random.default_rng(5)
+    for trial in range(3000):
+        r = 10 ** rng.uniform(-2, math.log10(1.5))
+        x = rng.normal(size=4)
+        x /= np.linalg.norm(x)
+        ang = r * rng.uniform(1e-3, 1.0)
+        axis = rng.normal(size=4)
+        axis -= axis.dot(x) * x
+        axis /= np.linalg.norm(axis)
+        y = math.cos(ang) * x + math.sin(ang) * axis
+        dm2, dp2 = ((x - y) ** 2).sum(), ((x + y) ** 2).sum()
+        window = (0.5 * r, 1 / (0.5 * r))
+        z = math.tan(r / 2) ** 2 * 1.0001
+        terms = next((k for k in range(2, 6) if z ** (k + 1) / (2 * k + 3) <= 5e-9), 0)
+        fk, ck, ek = kernel_near_pair(x, y, *window, terms)
+        atan = (lambda t: atan_series(t, terms)) if terms else atan_poly
+        fr, cr, er = reference_pair(x, y, dm2, dp2, *window, atan=atan, exact_sin=True)
+        f0, c0, e0 = reference_pair(x, y, dm2, dp2, atan=atan)
+        scale_f = abs(f0).max() + abs(e0) * 20 * np.abs(x).sum() * 2 / r
+        assert abs(ek - er) <= 1e-11 * e0
+        assert abs(ck - cr) <= 1e-11 * c0
+        assert np.abs(fk - fr).max() <= 1e-10 * scale_f
+        # against the exact arctangent: a relative error delta in a (1e-8 for the series,
+        # 6e-8 for the minimax polynomial in FP32 coefficients) moves E W by at most
+        # E delta (a |E'/E| + a |W'|) <= E delta (2 + 6 a / width)
+        fx, cx, ex = reference_pair(x, y, dm2, dp2, *window, exact_sin=True)
+        delta = 1e-8 if terms else 6e-8
+        a = 2 * math.atan(math.sqrt(dm2 / dp2))
+        assert abs(ek - ex) <= e0 * delta * (2 + 6 * a * window[1])
