@@ -231,6 +231,7 @@ def main():
     ap.add_argument("--delta", type=float, default=20.0)
     ap.add_argument("--cpu-sample", type=int, default=int(os.environ.get("ZXT_CPU_SAMPLE", 600000)))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--spacing-n", type=int, default=10_000_000, help="0 skips the spacing pass")
     ap.add_argument("--batched-jobs", type=int, default=256, help="0 skips the batched comparison")
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--serving-requests", type=int, default=40, help="0 skips the serving run")
@@ -312,11 +313,38 @@ def main():
         from oracle import dense_f64 as dense  # the checker, for the CPU baseline leg only
         cut, radius = dense.near_cut(len(q), len(ops), args.delta)
         out["cpu_baseline"] = cpu_baseline(q, ops, cut, radius, args.cpu_sample)
+    if args.spacing_n > 0:
+        out["spacing"] = spacing_section(args.spacing_n)
     if args.batched_jobs > 0:
         out["batched"] = batched_section(args.batched_jobs, args.batched_steps)
     if args.serving_requests > 0:
         out["serving"] = serving_section(args.serving_requests, args.serving_rate)
     print(json.dumps(out), flush=True)
+
+
+def spacing_section(n):
+    """Spacing of n Haar orientations of I (seed 193): nearest orbit point of every point
+    (index and angle) over the 4-mean-radius near field; the second call is timed (the
+    first sizes the near field).  Bytes: the algorithmic 76 B per point of a linear-time
+    pass (site sort 52, positions 16, index and angle 8)."""
+    import zxtalori_amd as zo
+    print(f"[bench] spacing I N={n}", file=sys.stderr, flush=True)
+    with zo.haar("I", n, 193) as grid:
+        grid.spacing()
+        grid.profile(True)
+        t0 = time.perf_counter()
+        stats = grid.spacing()
+        dt = time.perf_counter() - t0
+        report = grid.profile_report()
+        grid.profile(False)
+        info = grid.near_info()
+    kernels = {k: round(ms / c, 4) for k, (c, ms) in sorted(report.items(), key=lambda kv: -kv[1][1])}
+    device_ms = sum(ms for _, ms in report.values())
+    return {"workload": f"zo.haar('I', {n}, 193).spacing(): nearest orbit point of every point, "
+                        "neighbourhood 4 mean radii", "n": n, "ms": dt * 1e3, "device_ms": device_ms,
+            "points_per_s": n / dt, "alg_bytes_per_point": 76,
+            "alg_gbps_device": 76 * n / (device_ms * 1e-3) / 1e9, "kernels_ms": kernels,
+            "sites": info["sites"], "separation_deg": stats["separation_deg"]}
 
 
 def _same(a, b):

@@ -220,7 +220,7 @@ bool NearField::overflowed(Engine& e) {
 // ---- RelaxState --------------------------------------------------------------------
 void RelaxState::release() {
   for (DevBuf* b : {&force, &energy, &nearest, &prev, &state, &partials, &ap_partial, &ap_pnear, &nn_partial,
-                    &nn_chord, &nn_rank})
+                    &nn_chord, &nn_rank, &nn_degrees})
     b->release();
   release_near(near);
   n = 0;
@@ -566,19 +566,20 @@ double Engine::nearest_pass(RelaxState& rs) {
       launch_allpairs_nearest(prof, stream, q, n_, ops_device(), order(), rs.nn_partial.as<uint64_t>(),
                               rs.nn_chord.as<float>(), rs.nn_rank.as<uint32_t>());
     }
-    std::vector<float> chord;
-    download(chord, rs.nn_chord, n_);
+    rs.nn_degrees.ensure(size_t(n_) * 4 + 4);
+    float* degrees = rs.nn_degrees.as<float>();
+    uint32_t* least_bits = reinterpret_cast<uint32_t*>(degrees + n_);
+    launch_degrees(prof, stream, rs.nn_chord.as<float>(), n_, degrees, least_bits);
+    auto& out = separation_out();
+    uint32_t least_key = 0;
+    check(hipMemcpyAsync(out.data(), degrees, size_t(n_) * 4, hipMemcpyDeviceToHost, stream), "download");
+    check(hipMemcpyAsync(&least_key, least_bits, 4, hipMemcpyDeviceToHost, stream), "download");
     download(point_index_, rs.nn_rank, n_);
     sync();
     if (rs.has_near && rs.near.overflowed(*this)) continue;
-    auto& out = separation_out();
     point_index_iteration_ = long(iteration_);
-    float least = 180.0f;
-    const float k = 180.0f / float(kPi);
-    for (uint32_t i = 0; i < n_; ++i) {
-      out[i] = 4.0f * std::asin(std::min(1.0f, std::sqrt(chord[i]) * 0.5f)) * k;
-      least = std::min(least, out[i]);
-    }
+    float least;
+    std::memcpy(&least, &least_key, 4);
     if (!(least > 0)) fail("DuplicateOrientations");
     return least;
   }
@@ -703,7 +704,7 @@ size_t Engine::device_bytes() const {
   size_t b = q_.bytes + ops_.bytes + measure_partial_.bytes + measure_out_.bytes;
   for (const RelaxState* rs : {&relax_state_, &spacing_state_, &measure_state_}) {
     for (const DevBuf* d : {&rs->force, &rs->energy, &rs->nearest, &rs->prev, &rs->state, &rs->partials,
-                            &rs->ap_partial, &rs->ap_pnear, &rs->nn_partial, &rs->nn_chord, &rs->nn_rank})
+                            &rs->ap_partial, &rs->ap_pnear, &rs->nn_partial, &rs->nn_chord, &rs->nn_rank, &rs->nn_degrees})
       b += d->bytes;
     const NearField& f = rs->near;
     for (const DevBuf* d : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,

@@ -79,7 +79,8 @@ struct RelaxState {
   bool current = false;
   bool has_near = false;
   NearField near;
-  DevBuf force, energy, nearest, prev, state, partials, ap_partial, ap_pnear, nn_partial, nn_chord, nn_rank;
+  DevBuf force, energy, nearest, prev, state, partials, ap_partial, ap_pnear, nn_partial, nn_chord, nn_rank,
+      nn_degrees;
   Rows rows() const { return Rows{force.as<float4>(), energy.as<double>(), nearest.as<float>()}; }
   void release();
   void set_near(uint32_t group, const double* delta);

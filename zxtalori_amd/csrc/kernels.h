@@ -210,6 +210,9 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, uint32_t order, const NearBuffers& b,
                          const NearCut& cut, float* chord_out, uint32_t* rank_out);
 
+// SO(3) degrees of nearest chords (out) and their minimum's bits (*least).
+void launch_degrees(Prof& p, hipStream_t s, const float* chord, uint32_t n, float* out, uint32_t* least);
+
 // BB step (relax.wgsl) of every job in the set, resident: stats, sums, median, move.
 void launch_bb_step(Prof& p, hipStream_t s, const JobSet& jobs);
 uint32_t bb_partials(uint32_t n);

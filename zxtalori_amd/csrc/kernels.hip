@@ -1183,30 +1183,28 @@ __device__ inline float row_gap(float c, int idx, uint32_t dim) {
   return fmaxf(0.0f, fmaxf(lo - c, c - hi));
 }
 
-// MODE 0: force rows.  MODE 1: nearest orbit point (chord^2, rank), the identity image
-// of the source excluded.
+// Near-field force rows (pairs.wgsl:363-427): per source (one wave) the sum of the
+// windowed Quey pair over every orbit point within the cutoff, outside its own orbit.
 //
-// Per source (one wave): up to 64 cell rows at a time, one row per lane, each trimmed in x
-// to the chart ball of the source's cap; their site runs form one index space that the
-// wave walks 64 candidates at a time.  A lane finds its candidate's row by walking the row
-// boundaries of the chunk with readlane (uniform, ~1.5 boundaries per chunk).  Candidates
-// within the cutoff update the nearest chord; those outside the source's own orbit enter
-// an LDS ring, and each full 64 runs the pair kernel on a full wave.
+// The cutoff cap is a ball in the chart; up to 64 of the cell rows it crosses are set up
+// at once, one per lane, each trimmed in x to the ball.  The wave then walks them row by
+// row (uniform): a row's sites are contiguous, so lane l tests the site row start + c + l.
+// Candidates within the cutoff enter an LDS ring (ballot + mbcnt) and each full 64 runs
+// the pair arithmetic on a full wave; only a row's last chunk runs partly empty.  Per-lane
+// FP32 sums go into the wave's FP64 sums every 64 rounds.
 //
 // OWN: whether the source's own orbit can lie within the cutoff.  x.g is at S3 angle
 // acos|g.w| from x for every x, so below the smallest such angle (0.63 rad for I) no own
 // image is ever a candidate: then the candidate stage needs no site codes, the only site
 // to skip is the source's own identity image (its sorted position), and the nearest chord
-// is taken over the compacted pairs.
-template <int MODE, bool OWN, int TERMS>
+// is taken over the compacted pairs.  TERMS: the arctangent series (atan_small).
+template <bool OWN, int TERMS>
 __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint32_t n,
                                               const uint32_t* __restrict__ order, const float4* __restrict__ sq,
-                                              const uint32_t* __restrict__ scode, const uint32_t* __restrict__ srank,
+                                              const uint32_t* __restrict__ scode,
                                               const uint32_t* __restrict__ offsets, const NearInfo* __restrict__ info,
                                               const uint32_t* __restrict__ self_pos, uint32_t sdim, NearCut cut,
-                                              uint32_t two_order, uint32_t real_blocks, Rows rows,
-                                              float* __restrict__ chord_out, uint32_t* __restrict__ rank_out,
-                                              uint32_t* __restrict__ counts) {
+                                              uint32_t real_blocks, Rows rows, uint32_t* __restrict__ counts) {
   __shared__ WaveLds lds[kBlock / 64];
   const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
   if (vb >= real_blocks) return;
@@ -1234,15 +1232,14 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   const int zlo = clampc(cz - rc), zhi = clampc(cz + rc), ylo = clampc(cy - rc), yhi = clampc(cy + rc);
   const uint32_t ny = uint32_t(yhi - ylo + 1), nrows = ny * uint32_t(zhi - zlo + 1);
   const float rc2 = rc * rc;
-  const uint32_t self_code = src | 0x80000000u, self_rank = src * two_order;
-  const uint32_t self_site = (MODE == 0 && !OWN) ? self_pos[src] : 0u;
+  const uint32_t self_code = src | 0x80000000u;
+  const uint32_t self_site = OWN ? 0u : self_pos[src];
 
   NearSum ps{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   // Chords are sums of squares (>= 0), so their bits order as unsigned integers: one
   // integer min per update, no float canonicalisation.
   uint32_t nearest_bits = __float_as_uint(4.0f);
-  uint64_t best = ~uint64_t(0);
   uint32_t qhead = 0, qcount = 0, rounds = 0;
   uint32_t candidates = 0;  // sites tested (uniform); accepted pairs are 64 rounds + the tail
 
@@ -1278,9 +1275,6 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
       }
     }
     candidates += uint32_t(__builtin_amdgcn_readlane(int(wave_incl_scan(len)), 63));
-    // Row by row (uniform): a row's sites are contiguous, so a lane's site is the row start
-    // plus its offset; the last chunk of a row runs partly empty, which the LDS ring
-    // absorbs (the pair arithmetic still runs on full waves).
     for (uint32_t rr = 0; rr < 64; ++rr) {
       const uint32_t n_row = uint32_t(__builtin_amdgcn_readlane(int(len), int(rr)));
       if (n_row == 0) continue;
@@ -1288,73 +1282,55 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
       for (uint32_t c = 0; c < n_row; c += 64) {
         const bool valid = c + lane < n_row;
         const uint32_t site = s_row + min(c + lane, n_row - 1);  // in range for every lane
-        if (MODE == 0) {
-          const float4 y = sq[site];
-          const float4 d = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
-          const float dm2 = dot4(d, d);  // chord2(x, y), the same operations
-          const bool inside = valid && dm2 <= cut.chord2;
-          bool accept;
-          if (OWN) {
-            // Every orbit point but the source itself bounds the nearest chord; the
-            // source's own images exert no force (pairs.wgsl:404-408).
-            const uint32_t code = scode[site];
-            if (inside && code != self_code) nearest_bits = min(nearest_bits, __float_as_uint(dm2));
-            accept = inside && (code & 0x7fffffffu) != src;
-          } else {
-            accept = inside && site != self_site;
-          }
-          const uint64_t mask = __ballot(accept);
-          if (mask) {
-            if (accept) {
-              const uint32_t e = (qhead + qcount + mbcnt(mask)) & 127u;
-              L.qe[e] = d;
-              L.qd[e] = dm2;
-            }
-            qcount = __builtin_amdgcn_readfirstlane(qcount + uint32_t(__popcll(mask)));
-            wave_sync();
-            if (qcount >= 64) {
-              process((qhead + lane) & 127u);
-              qhead = __builtin_amdgcn_readfirstlane((qhead + 64) & 127u);
-              qcount -= 64;
-              rounds = __builtin_amdgcn_readfirstlane(rounds + 1);
-              if ((rounds & 63u) == 0) flush();
-              wave_sync();
-            }
-          }
+        const float4 y = sq[site];
+        const float4 d = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
+        const float dm2 = dot4(d, d);  // chord2(x, y), the same operations
+        const bool inside = valid && dm2 <= cut.chord2;
+        bool accept;
+        if (OWN) {
+          // Every orbit point but the source itself bounds the nearest chord; the
+          // source's own images exert no force (pairs.wgsl:404-408).
+          const uint32_t code = scode[site];
+          if (inside && code != self_code) nearest_bits = min(nearest_bits, __float_as_uint(dm2));
+          accept = inside && (code & 0x7fffffffu) != src;
         } else {
-          const float4 y = sq[site];
-          const uint32_t rank = srank[site];
-          const float dm2 = chord2(x, y);
-          if (valid && dm2 <= cut.chord2 && rank != self_rank) {
-            const uint64_t key = (uint64_t(__float_as_uint(dm2)) << 32) | rank;
-            best = key < best ? key : best;
+          accept = inside && site != self_site;
+        }
+        const uint64_t mask = __ballot(accept);
+        if (mask) {
+          if (accept) {
+            const uint32_t e = (qhead + qcount + mbcnt(mask)) & 127u;
+            L.qe[e] = d;
+            L.qd[e] = dm2;
+          }
+          qcount = __builtin_amdgcn_readfirstlane(qcount + uint32_t(__popcll(mask)));
+          wave_sync();
+          if (qcount >= 64) {
+            process((qhead + lane) & 127u);
+            qhead = __builtin_amdgcn_readfirstlane((qhead + 64) & 127u);
+            qcount -= 64;
+            rounds = __builtin_amdgcn_readfirstlane(rounds + 1);
+            if ((rounds & 63u) == 0) flush();
+            wave_sync();
           }
         }
       }
     }
   }
-  if (MODE == 0) {
-    if (lane < qcount) process((qhead + lane) & 127u);
-    flush();
-    const double g[4] = {acc[0], acc[1], acc[2], acc[3]};
-    const double curvature = acc[4], e = acc[5];
-    const float nearest = __uint_as_float(wave_min_u32(nearest_bits));
-    const uint32_t pairs = 64 * rounds + qcount;
-    if (lane == 0) {
-      double f[3];
-      near_force_of(x, g, f);
-      rows.force[src] = make_float4(float(f[0]), float(f[1]), float(f[2]), float(curvature));
-      rows.energy[src] = e;
-      rows.nearest[src] = nearest;
-      counts[2 * size_t(src)] = pairs;
-      counts[2 * size_t(src) + 1] = candidates;
-    }
-  } else {
-    best = wave_min_u64(best);
-    if (lane == 0) {
-      chord_out[src] = best == ~uint64_t(0) ? 4.0f : __uint_as_float(uint32_t(best >> 32));
-      rank_out[src] = best == ~uint64_t(0) ? 0xffffffffu : uint32_t(best);
-    }
+  if (lane < qcount) process((qhead + lane) & 127u);
+  flush();
+  const double g[4] = {acc[0], acc[1], acc[2], acc[3]};
+  const double curvature = acc[4], e = acc[5];
+  const float nearest = __uint_as_float(wave_min_u32(nearest_bits));
+  const uint32_t pairs = 64 * rounds + qcount;
+  if (lane == 0) {
+    double f[3];
+    near_force_of(x, g, f);
+    rows.force[src] = make_float4(float(f[0]), float(f[1]), float(f[2]), float(curvature));
+    rows.energy[src] = e;
+    rows.nearest[src] = nearest;
+    counts[2 * size_t(src)] = pairs;
+    counts[2 * size_t(src) + 1] = candidates;
   }
 }
 
@@ -1379,25 +1355,25 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
   const uint32_t real = blocks_for(n, kBlock / 64);
   p.run("near_force", [&] {
     auto go = [&](auto kernel) {
-      hipLaunchKernelGGL(kernel, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.srank,
-                         b.offsets, b.info, b.self_pos, b.sdim, cut, 0u, real, out, nullptr, nullptr, b.counts);
+      hipLaunchKernelGGL(kernel, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.offsets,
+                         b.info, b.self_pos, b.sdim, cut, real, out, b.counts);
     };
     const int terms = atan_terms(cut.chord2);
     if (cut.own) {
       switch (terms) {
-        case 2: go(k_near<0, true, 2>); break;
-        case 3: go(k_near<0, true, 3>); break;
-        case 4: go(k_near<0, true, 4>); break;
-        case 5: go(k_near<0, true, 5>); break;
-        default: go(k_near<0, true, 0>); break;
+        case 2: go(k_near<true, 2>); break;
+        case 3: go(k_near<true, 3>); break;
+        case 4: go(k_near<true, 4>); break;
+        case 5: go(k_near<true, 5>); break;
+        default: go(k_near<true, 0>); break;
       }
     } else {
       switch (terms) {
-        case 2: go(k_near<0, false, 2>); break;
-        case 3: go(k_near<0, false, 3>); break;
-        case 4: go(k_near<0, false, 4>); break;
-        case 5: go(k_near<0, false, 5>); break;
-        default: go(k_near<0, false, 0>); break;
+        case 2: go(k_near<false, 2>); break;
+        case 3: go(k_near<false, 3>); break;
+        case 4: go(k_near<false, 4>); break;
+        case 5: go(k_near<false, 5>); break;
+        default: go(k_near<false, 0>); break;
       }
     }
   });
@@ -1408,13 +1384,118 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
   });
 }
 
+// Nearest orbit point within the cutoff for spacing (pairs.zig:318-335): a small cap (4
+// mean radii, a few dozen candidates in a handful of short cell rows), so a group of
+// kSpacingLanes lanes owns a source: the group sets up up to kSpacingLanes rows at once,
+// one per lane, then walks them (group-uniform), its lanes testing consecutive sites of a
+// row; the key (chord^2 bits, rank) makes the minimum take the lowest rank on ties.  The
+// source's own identity image is skipped by its rank.
+constexpr uint32_t kSpacingLanes = 8;
+__global__ __launch_bounds__(256) void k_spacing(const float4* __restrict__ q, uint32_t n,
+                                                 const uint32_t* __restrict__ order, const float4* __restrict__ sq,
+                                                 const uint32_t* __restrict__ srank,
+                                                 const uint32_t* __restrict__ offsets,
+                                                 const NearInfo* __restrict__ info, uint32_t sdim, NearCut cut,
+                                                 uint32_t two_order, uint32_t real_blocks,
+                                                 float* __restrict__ chord_out, uint32_t* __restrict__ rank_out) {
+  constexpr uint32_t G = kSpacingLanes;
+  const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
+  if (vb >= real_blocks) return;
+  const uint32_t sub = threadIdx.x % G;
+  const uint32_t slot = vb * (kBlock / G) + threadIdx.x / G;
+  if (slot >= n) return;  // the whole group leaves (groups never straddle a wave)
+  const uint32_t src = order[slot];
+  const float4 x = q[src];
+  const float cpu = info->cpu;
+  const float fdim = float(sdim);
+  const float denom = cut.c0 + x.x;
+  const bool full = !(denom > 1e-3f);
+  float cx = 0.5f * fdim, cy = 0.5f * fdim, cz = 0.5f * fdim, rc = 3e18f;
+  if (!full) {
+    const float inv = 1.0f / denom;
+    cx = (x.y * inv - info->lower[0]) * cpu;
+    cy = (x.z * inv - info->lower[1]) * cpu;
+    cz = (x.w * inv - info->lower[2]) * cpu;
+    rc = (cut.s0 * inv * 1.001f + 1e-6f) * cpu + 1e-3f;
+  }
+  auto clampc = [&](float v) -> int { return int(fminf(fmaxf(floorf(v), 0.0f), fdim - 1.0f)); };
+  const int zlo = clampc(cz - rc), zhi = clampc(cz + rc), ylo = clampc(cy - rc), yhi = clampc(cy + rc);
+  const uint32_t ny = uint32_t(yhi - ylo + 1), nrows = ny * uint32_t(zhi - zlo + 1);
+  const float rc2 = rc * rc;
+  const uint32_t self_rank = src * two_order;
+  uint64_t best = ~uint64_t(0);
+  for (uint32_t rb = 0; rb < nrows; rb += G) {
+    const uint32_t r = rb + sub;
+    uint32_t start = 0, len = 0;
+    if (r < nrows) {
+      const int yy = ylo + int(r % ny), zz = zlo + int(r / ny);
+      const float gy = row_gap(cy, yy, sdim), gz = row_gap(cz, zz, sdim);
+      const float rest = rc2 - gy * gy - gz * gz;
+      if (rest >= 0.0f) {
+        const float span = full ? 3e18f : sqrtf(rest);
+        const int x0 = clampc(cx - span), x1 = clampc(cx + span);
+        const uint32_t row = sdim * (uint32_t(yy) + sdim * uint32_t(zz));
+        start = offsets[row + x0];
+        len = offsets[row + x1 + 1] - start;
+      }
+    }
+    const uint32_t rows_here = min(G, nrows - rb);
+    for (uint32_t k = 0; k < rows_here; ++k) {
+      const uint32_t n_row = uint32_t(__shfl(int(len), int(k), int(G)));
+      const uint32_t s_row = uint32_t(__shfl(int(start), int(k), int(G)));
+      for (uint32_t c = sub; c < n_row; c += G) {
+        const uint32_t site = s_row + c;
+        const float4 y = sq[site];
+        const uint32_t rank = srank[site];
+        const float dm2 = chord2(x, y);
+        if (dm2 <= cut.chord2 && rank != self_rank) {
+          const uint64_t key = (uint64_t(__float_as_uint(dm2)) << 32) | rank;
+          best = key < best ? key : best;
+        }
+      }
+    }
+  }
+  for (uint32_t d = G / 2; d > 0; d >>= 1) {
+    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(best)), int(d), int(G)));
+    const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(best >> 32)), int(d), int(G)));
+    const uint64_t o = (uint64_t(hi) << 32) | lo;
+    best = o < best ? o : best;
+  }
+  if (sub == 0) {
+    chord_out[src] = best == ~uint64_t(0) ? 4.0f : __uint_as_float(uint32_t(best >> 32));
+    rank_out[src] = best == ~uint64_t(0) ? 0xffffffffu : uint32_t(best);
+  }
+}
+
 void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, uint32_t order, const NearBuffers& b,
                          const NearCut& cut, float* chord_out, uint32_t* rank_out) {
-  const uint32_t real = blocks_for(n, kBlock / 64);
+  const uint32_t real = blocks_for(n, kBlock / kSpacingLanes);
   p.run("near_nearest", [&] {
-    hipLaunchKernelGGL((k_near<1, true, 0>), dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode,
-                       b.srank, b.offsets, b.info, b.self_pos, b.sdim, cut, 2 * order, real, Rows{}, chord_out,
-                       rank_out, nullptr);
+    hipLaunchKernelGGL(k_spacing, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.srank, b.offsets,
+                       b.info, b.sdim, cut, 2 * order, real, chord_out, rank_out);
+  });
+}
+
+// Nearest-neighbour angles in SO(3) degrees, 4 asin(min(1, sqrt(c)/2)) 180/pi in FP32
+// (nearest_degrees, pairs.wgsl:428-434), and their minimum (positive floats order as their
+// bits; 0xffffffff marks an empty set) into *least.
+__global__ void k_degrees(const float* __restrict__ chord, uint32_t n, float* __restrict__ out,
+                          uint32_t* __restrict__ least) {
+  uint32_t lo = 0xffffffffu;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float d = 4.0f * asinf(fminf(1.0f, sqrtf(chord[i]) * 0.5f)) * (180.0f / kPiF);
+    out[i] = d;
+    lo = min(lo, __float_as_uint(d));
+  }
+  lo = wave_min_u32(lo);
+  if ((threadIdx.x & 63) == 0) atomicMin(least, lo);
+}
+
+void launch_degrees(Prof& p, hipStream_t s, const float* chord, uint32_t n, float* out, uint32_t* least) {
+  launch_fill_u32(p, s, least, 0xffffffffu, 1);
+  p.run("degrees", [&] {
+    hipLaunchKernelGGL(k_degrees, dim3(std::min<uint32_t>(2048, blocks_for(n, kBlock))), dim3(kBlock), 0, s, chord, n,
+                       out, least);
   });
 }
 
