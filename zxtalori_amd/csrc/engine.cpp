@@ -144,6 +144,11 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
       pl.push_back(make_float4(float(v[0] / len), float(v[1] / len), float(v[2] / len), float(v[3] / len)));
     }
   }
+  // Nearest planes first (by their distance from the identity, the normal's scalar part):
+  // the rim and shell tests then stop as soon as the remaining planes are provably out of
+  // reach (k_core_rim, shell_image).  The tests are "any" and "all", so order only
+  // changes when they stop.
+  std::stable_sort(pl.begin(), pl.end(), [](const float4& a, const float4& b) { return a.x < b.x; });
   nplanes = uint32_t(pl.size());
   planes.ensure(pl.size() * sizeof(float4));
   Engine::check(hipMemcpyAsync(planes.p, pl.data(), pl.size() * sizeof(float4), hipMemcpyHostToDevice, e.stream),

@@ -753,6 +753,12 @@ __device__ inline void box_reduce_commit(NearInfo* info, uint32_t lo[3], uint32_
   }
 }
 
+// |q - e|, the chord from the identity, without the cancellation of sqrt(2 - 2 q.w).
+__device__ inline float identity_chord(const float4 q) {
+  const float d0 = q.x - 1.0f;
+  return sqrtf(fmaf(q.w, q.w, fmaf(q.z, q.z, fmaf(q.y, q.y, d0 * d0))));
+}
+
 __device__ inline void box_add(uint32_t lo[3], uint32_t hi[3], const float4 y) {
   const float3 p = chart(y);
   const uint32_t k0 = ordered_key(p.x), k1 = ordered_key(p.y), k2 = ordered_key(p.z);
@@ -783,9 +789,11 @@ __global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, 
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   uint32_t wmin = 0xffffffffu;
   const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t rounds = (n + stride - 1) / stride;
+  const uint32_t rounds = (n + stride - 1) / stride;  // at most 32 (core_rim_blocks)
+  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t rim_mask = 0;  // bit r: source r * stride + first is a rim source
   for (uint32_t r = 0; r < rounds; ++r) {
-    const uint32_t i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = r * stride + first;
     bool rim = false;
     if (i < n) {
       const float4 x = q[i];
@@ -794,18 +802,41 @@ __global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, 
       b.urank[i] = i * two_order;
       box_add(lo, hi, x);
       wmin = min(wmin, ordered_key(x.x));
+      // x.p >= p.w - |x - e| (e the identity, |p| = 1) and the planes come nearest first:
+      // once that bound clears the band (with 1e-5 to spare for FP32 rounding), no later
+      // plane can hold x within it.
+      const float de = identity_chord(x);
       for (uint32_t k = 0; k < nplanes; ++k) {
-        if (dot4(x, planes[k]) <= band + 2e-6f) {
+        const float4 pl = planes[k];
+        if (pl.x - de > band + 2e-6f + 1e-5f) break;
+        if (dot4(x, pl) <= band + 2e-6f) {
           rim = true;
           break;
         }
       }
     }
-    const uint32_t slot = wave_append(&b.info->rim, rim);
-    if (rim) b.rim[slot] = i;
+    rim_mask |= uint32_t(rim) << r;
   }
+  // One append per wave for all its rounds (a counter shared by every wave would
+  // otherwise serialise one atomic per wave per round).
+  const uint32_t mine = uint32_t(__popc(rim_mask));
+  const uint32_t incl = wave_incl_scan(mine);
+  const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
+  uint32_t base = 0;
+  if (total) {
+    if ((threadIdx.x & 63) == 63) base = atomicAdd(&b.info->rim, total);
+    base = uint32_t(__builtin_amdgcn_readlane(int(base), 63));
+  }
+  uint32_t at = base + incl - mine;
+  for (uint32_t m = rim_mask; m; m &= m - 1) b.rim[at++] = uint32_t(__ffs(m) - 1) * stride + first;
   box_reduce_commit(b.info, lo, hi);
   block_min_commit(&b.info->wmin_key, wmin);
+}
+
+// Workgroups of k_core_rim: at least 2048, and enough that no thread takes more than 32
+// sources (its rim flags fit one word).
+static uint32_t core_rim_blocks(uint32_t n) {
+  return std::max<uint32_t>(2048, blocks_for(n, kBlock * 32));
 }
 
 // Whether image img (1..2|K|-1: operator img/2, negated when odd) of x is a shell site;
@@ -823,8 +854,14 @@ __device__ inline bool shell_image(const float4 x, uint32_t img, const float4* _
   y = image(x, op, k);
   if (img & 1u) y = neg4(y);
   if (!(y.x > -0.9999999f)) return false;
-  for (uint32_t pk = 0; pk < nplanes; ++pk)
-    if (dot4(y, planes[pk]) < -band - 2e-6f) return false;
+  // Planes nearest first: once y.p >= p.w - |y - e| clears -band (with 1e-5 to spare),
+  // every later plane holds y too.
+  const float de = identity_chord(y);
+  for (uint32_t pk = 0; pk < nplanes; ++pk) {
+    const float4 pl = planes[pk];
+    if (pl.x - de >= -band - 2e-6f + 1e-5f) break;
+    if (dot4(y, pl) < -band - 2e-6f) return false;
+  }
   return true;
 }
 
@@ -1124,7 +1161,7 @@ void launch_near_sites(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
                        const NearBuffers& b) {
   p.run("near_reset", [&] { hipLaunchKernelGGL(k_near_reset, dim3(1), dim3(1), 0, s, b.info, n); });
   p.run("core_rim", [&] {
-    hipLaunchKernelGGL(k_core_rim, dim3(std::min<uint32_t>(2048, blocks_for(n, kBlock))), dim3(kBlock), 0, s, q, n,
+    hipLaunchKernelGGL(k_core_rim, dim3(std::min(core_rim_blocks(n), blocks_for(n, kBlock))), dim3(kBlock), 0, s, q, n,
                        b.planes, b.nplanes, b.band, 2 * order, b);
   });
   p.run("shell", [&] {
