@@ -11,6 +11,7 @@ namespace {
 }  // namespace
 
 Batch::Batch(bool use_graphs, uint32_t streams) : use_graphs_(use_graphs) {
+  CaptureLock lock(false);
   Engine::check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
   prof.set_stream(stream);
   if (streams > 0) {
@@ -30,6 +31,7 @@ Batch::~Batch() {
   (void)hipStreamSynchronize(stream);
   if (graph_) (void)hipGraphExecDestroy(graph_);
   jobs_.clear();
+  CaptureLock lock(false);
   for (Table* t : {&ap_table_, &step_table_}) {
     if (t->host_desc) (void)hipHostFree(t->host_desc);
     if (t->host_start) (void)hipHostFree(t->host_start);
@@ -94,6 +96,7 @@ JobSet Batch::upload(Table& t, const std::vector<JobDesc>& descs) {
   }
   if (count > t.capacity) {
     Engine::check(hipStreamSynchronize(stream), "wait");
+    CaptureLock lock(false);
     if (t.host_desc) (void)hipHostFree(t.host_desc);
     if (t.host_start) (void)hipHostFree(t.host_start);
     t.capacity = std::max<size_t>(count, 2 * t.capacity);
@@ -254,21 +257,25 @@ uint32_t Batch::step() {
       }
       graph_key_.clear();
       hipGraph_t g = nullptr;
-      Engine::check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
-      capturing_ = true;
       bool changed = false;
-      try {
-        record();
-      } catch (const GraphTableChanged&) {
-        changed = true;
-      } catch (...) {
+      hipError_t ended;
+      {
+        CaptureLock lock(true);
+        Engine::check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+        capturing_ = true;
+        try {
+          record();
+        } catch (const GraphTableChanged&) {
+          changed = true;
+        } catch (...) {
+          capturing_ = false;
+          (void)hipStreamEndCapture(stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
         capturing_ = false;
-        (void)hipStreamEndCapture(stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
+        ended = hipStreamEndCapture(stream, &g);
       }
-      capturing_ = false;
-      const hipError_t ended = hipStreamEndCapture(stream, &g);
       if (changed) {
         // Nothing captured has run: record this step eagerly, capture the next one.
         if (g) (void)hipGraphDestroy(g);

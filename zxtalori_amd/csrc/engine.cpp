@@ -6,6 +6,8 @@
 // measure, spacing, the near field's one-time sizing pass).
 #include "engine.h"
 
+#include <shared_mutex>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -41,14 +43,40 @@ double mean_radius(uint64_t count) {
   return 0.25 * (lo + hi);
 }
 
+// ---- CaptureLock -----------------------------------------------------------------
+namespace {
+std::shared_mutex capture_mutex;
+thread_local int capture_holds = 0;
+}  // namespace
+
+CaptureLock::CaptureLock(bool capture) : mode_(0) {
+  if (capture_holds++ > 0) return;
+  if (capture) {
+    capture_mutex.lock_shared();
+    mode_ = 1;
+  } else {
+    capture_mutex.lock();
+    mode_ = 2;
+  }
+}
+CaptureLock::~CaptureLock() {
+  --capture_holds;
+  if (mode_ == 1) capture_mutex.unlock_shared();
+  if (mode_ == 2) capture_mutex.unlock();
+}
+
 // ---- DevBuf ----------------------------------------------------------------------
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p) {
+    CaptureLock lock(false);
+    (void)hipFree(p);
+  }
   p = nullptr;
   bytes = 0;
 }
 void DevBuf::ensure(size_t b) {
   if (p && b <= bytes) return;
+  CaptureLock lock(false);
   release();
   const size_t want = std::max<size_t>(b, 256);
   if (hipMalloc(&p, want) != hipSuccess) {
@@ -308,6 +336,7 @@ JobDesc RelaxState::step_desc(Engine& e, double radius, uint32_t fixed) {
 
 // ---- Engine ------------------------------------------------------------------------
 Engine::Engine(hipStream_t shared) {
+  CaptureLock lock(false);
   if (shared) {
     stream = shared;
   } else {
@@ -327,7 +356,10 @@ Engine::~Engine() {
   measure_state_.release();
   q_.release();
   ops_.release();
-  if (owns_stream_) (void)hipStreamDestroy(stream);
+  if (owns_stream_) {
+    CaptureLock lock(false);
+    (void)hipStreamDestroy(stream);
+  }
 }
 
 void Engine::sync() { check(hipStreamSynchronize(stream), "wait"); }
@@ -437,14 +469,14 @@ void Engine::sync_neighbourhood() {
   measure_neighbourhood_ = neighbourhood_;
 }
 
-std::vector<float>& Engine::separation_out() {
+PinnedVec<float>& Engine::separation_out() {
   point_separation_.resize(n_);
   point_separation_iteration_ = long(iteration_);
   return point_separation_;
 }
 
-Measures Engine::quick_measures(const std::vector<float4>& f, const std::vector<double>& energy,
-                                const std::vector<float>& nearest) const {
+Measures Engine::quick_measures(const PinnedVec<float4>& f, const PinnedVec<double>& energy,
+                                const PinnedVec<float>& nearest) const {
   const uint32_t n = n_;
   double nearv = 4.0, e = 0.0, sq = 0.0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -464,14 +496,14 @@ Measures Engine::quick_measures(const std::vector<float4>& f, const std::vector<
 
 // One force pass of the current positions, read back (retried when the near field's
 // shell outgrew its buffers); the forces are kept for the next step.
-void Engine::read_forces(RelaxState& rs, std::vector<float4>& f, std::vector<double>& en, std::vector<float>& nr) {
+void Engine::read_forces(RelaxState& rs) {
   for (int attempt = 0; attempt < 4; ++attempt) {
     rs.prepare(*this, n_);
     rs.current = false;
     rs.record_forces(*this, n_, order());
-    download(f, rs.force, n_);
-    download(en, rs.energy, n_);
-    download(nr, rs.nearest, n_);
+    download(rows_force_, rs.force, n_);
+    download(rows_energy_, rs.energy, n_);
+    download(rows_nearest_, rs.nearest, n_);
     sync();
     if (rs.has_near && rs.near.overflowed(*this)) continue;
     rs.current = true;
@@ -482,14 +514,11 @@ void Engine::read_forces(RelaxState& rs, std::vector<float4>& f, std::vector<dou
 
 // Measures of the current positions (pairs.zig:302-316).
 Measures Engine::relax_measure(RelaxState& rs, bool keep_separation) {
-  std::vector<float4> f;
-  std::vector<double> en;
-  std::vector<float> nr;
-  read_forces(rs, f, en, nr);
-  Measures m = quick_measures(f, en, nr);
+  read_forces(rs);
+  Measures m = quick_measures(rows_force_, rows_energy_, rows_nearest_);
   if (keep_separation) {
     auto& out = separation_out();
-    for (uint32_t i = 0; i < n_; ++i) out[i] = float(nearest_degrees(nr[i]));
+    for (uint32_t i = 0; i < n_; ++i) out[i] = float(nearest_degrees(rows_nearest_[i]));
   }
   return m;
 }
@@ -497,13 +526,10 @@ Measures Engine::relax_measure(RelaxState& rs, bool keep_separation) {
 void Engine::force_rows(float* force4, double* energy, float* nearest) {
   if (n_ < 2) fail("TooFewOrientations");
   sync_neighbourhood();
-  std::vector<float4> f;
-  std::vector<double> en;
-  std::vector<float> nr;
-  read_forces(relax_state_, f, en, nr);
-  std::memcpy(force4, f.data(), size_t(n_) * 16);
-  std::memcpy(energy, en.data(), size_t(n_) * 8);
-  std::memcpy(nearest, nr.data(), size_t(n_) * 4);
+  read_forces(relax_state_);
+  std::memcpy(force4, rows_force_.data(), size_t(n_) * 16);
+  std::memcpy(energy, rows_energy_.data(), size_t(n_) * 8);
+  std::memcpy(nearest, rows_nearest_.data(), size_t(n_) * 4);
 }
 
 // Riesz, Green and log ratios from the FP64 row sums, normalised as pairs.zig:142-173.
@@ -677,12 +703,9 @@ void Engine::finish_relax(bool recorded) {
     if (recorded) check(hipMemcpyAsync(&st, rs.state.p, sizeof(BBState), hipMemcpyDeviceToHost, stream), "state");
     host_.resize(size_t(n_) * 4);
     check(hipMemcpyAsync(host_.data(), q_.p, size_t(n_) * 16, hipMemcpyDeviceToHost, stream), "positions");
-    std::vector<float4> f;
-    std::vector<double> en;
-    std::vector<float> nr;
-    download(f, rs.force, n_);
-    download(en, rs.energy, n_);
-    download(nr, rs.nearest, n_);
+    download(rows_force_, rs.force, n_);
+    download(rows_energy_, rs.energy, n_);
+    download(rows_nearest_, rs.nearest, n_);
     sync();
     if (recorded) {
       if (st.flags & 1u) fail("NonFiniteForce");
@@ -694,8 +717,8 @@ void Engine::finish_relax(bool recorded) {
       recorded = false;
     }
     if (rs.has_near && rs.near.overflowed(*this)) continue;
-    measures_ = quick_measures(f, en, nr);
-    for (uint32_t i = 0; i < n_; ++i) sep[i] = float(nearest_degrees(nr[i]));
+    measures_ = quick_measures(rows_force_, rows_energy_, rows_nearest_);
+    for (uint32_t i = 0; i < n_; ++i) sep[i] = float(nearest_degrees(rows_nearest_[i]));
     full_measures_ = false;
     measured_ = true;
     measure_iteration_ = iteration_;
@@ -726,7 +749,7 @@ void Engine::identity(bool add, bool fixed) {
   if (add && index < 0) {
     const std::string generator = generator_;
     const uint32_t base = base_count_, seed = seed_;
-    std::vector<float> values(host_);
+    std::vector<float> values(host_.begin(), host_.end());
     values.insert(values.end(), {1.0f, 0.0f, 0.0f, 0.0f});
     set(group_, values.data(), values.size());
     generator_ = generator;

@@ -4,15 +4,33 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
 #include "kernels.h"
 
 namespace zxt {
+
+// Graph captures and the host calls that can end another thread's capture (device and
+// pinned host allocation and free, stream creation and destruction, synchronous copies)
+// exclude each other process-wide: a capture holds this lock shared, such a call holds it
+// exclusively.  A thread that already holds it (inside its own capture, or inside an
+// enclosing call) does not take it again.
+class CaptureLock {
+ public:
+  explicit CaptureLock(bool capture);
+  ~CaptureLock();
+  CaptureLock(const CaptureLock&) = delete;
+  CaptureLock& operator=(const CaptureLock&) = delete;
+
+ private:
+  int mode_;  // 0 not taken, 1 shared, 2 exclusive
+};
 
 // A grown-only device allocation.
 struct DevBuf {
@@ -33,6 +51,48 @@ struct DevBuf {
 
 struct EngineError {
   std::string message;
+};
+
+// A grown-only host array in pinned memory (device-to-host copies at full PCIe rate), with
+// the few vector operations the engine uses.  resize keeps the first min(old, new) items.
+template <typename T>
+class PinnedVec {
+ public:
+  PinnedVec() = default;
+  PinnedVec(const PinnedVec&) = delete;
+  PinnedVec& operator=(const PinnedVec&) = delete;
+  ~PinnedVec() {
+    if (p_) {
+      CaptureLock lock(false);
+      (void)hipHostFree(p_);
+    }
+  }
+  void resize(size_t n) {
+    if (n > cap_) {
+      CaptureLock lock(false);
+      void* q = nullptr;
+      if (hipHostMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) throw EngineError{"OutOfHostMemory"};
+      if (p_) {
+        std::memcpy(q, p_, n_ * sizeof(T));
+        (void)hipHostFree(p_);
+      }
+      p_ = static_cast<T*>(q);
+      cap_ = n;
+    }
+    n_ = n;
+  }
+  size_t size() const { return n_; }
+  size_t bytes() const { return cap_ * sizeof(T); }
+  T* data() { return p_; }
+  const T* data() const { return p_; }
+  T& operator[](size_t i) { return p_[i]; }
+  const T& operator[](size_t i) const { return p_[i]; }
+  const T* begin() const { return p_; }
+  const T* end() const { return p_ + n_; }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0, cap_ = 0;
 };
 
 // measure() results (pairs.zig:17-30).
@@ -150,10 +210,10 @@ class Engine {
   // index + 1 of the held identity, or 0.
   uint32_t fixed_slot() const { return fixed_index_ >= 0 ? uint32_t(fixed_index_ + 1) : 0u; }
   void sync();
-  template <typename T>
-  void download(std::vector<T>& dst, const DevBuf& src, size_t count) {
+  template <typename V>
+  void download(V& dst, const DevBuf& src, size_t count) {
     dst.resize(count);
-    check(hipMemcpyAsync(dst.data(), src.p, count * sizeof(T), hipMemcpyDeviceToHost, stream), "download");
+    check(hipMemcpyAsync(dst.data(), src.p, count * sizeof(dst[0]), hipMemcpyDeviceToHost, stream), "download");
   }
   static void check(hipError_t e, const char* what);
 
@@ -168,11 +228,15 @@ class Engine {
   Measures relax_measure(RelaxState& rs, bool keep_separation);
   void full_measure(Measures& m);
   DevBuf measure_partial_, measure_out_;
-  void read_forces(RelaxState& rs, std::vector<float4>& f, std::vector<double>& en, std::vector<float>& nr);
+  // One force pass read back into rows_force_, rows_energy_, rows_nearest_.
+  void read_forces(RelaxState& rs);
+  PinnedVec<float4> rows_force_;
+  PinnedVec<double> rows_energy_;
+  PinnedVec<float> rows_nearest_;
   double nearest_pass(RelaxState& rs);
-  Measures quick_measures(const std::vector<float4>& f, const std::vector<double>& energy,
-                          const std::vector<float>& nearest) const;
-  std::vector<float>& separation_out();
+  Measures quick_measures(const PinnedVec<float4>& f, const PinnedVec<double>& energy,
+                          const PinnedVec<float>& nearest) const;
+  PinnedVec<float>& separation_out();
   // relax() in parts, so that a batch runs the same steps for many samples.
   void begin_relax();
   void prepare_step();
@@ -185,7 +249,7 @@ class Engine {
 
   uint32_t group_ = 9;
   DevBuf ops_, q_;
-  std::vector<float> host_;
+  PinnedVec<float> host_;
   uint32_t n_ = 0;
   uint32_t iteration_ = 0;
   uint32_t seed_ = 0;
@@ -197,8 +261,8 @@ class Engine {
   bool full_measures_ = false;
   RelaxState relax_state_, spacing_state_, measure_state_;
   double neighbourhood_ = 0, measure_neighbourhood_ = 0;
-  std::vector<float> point_separation_;
-  std::vector<uint32_t> point_index_;
+  PinnedVec<float> point_separation_;
+  PinnedVec<uint32_t> point_index_;
   long point_separation_iteration_ = -1;
   long point_index_iteration_ = -1;
   double step_deg_ = 0, force_rms_ = 0;
