@@ -38,8 +38,11 @@ void zo_destroy(zxt_engine* e);
 int32_t zo_random(zxt_engine* e, uint32_t group, uint32_t count, uint32_t seed);
 /* src/exports.zig:23-28: import n quaternions (scalar first, copied), folded. */
 int32_t zo_set(zxt_engine* e, uint32_t group, const float* q, uint32_t n);
-/* src/exports.zig:9-15: inverse Hopf / FCC generation.  Not part of this build's path:
- * returns -1 with "Generation failed: NotImplemented". */
+/* src/exports.zig:9-15 (src/engine.zig:242-282): kind 0 or 1, the inverse Hopf product grid
+ * at frequency h (1..512), or with h = 0 the frequency whose count is nearest `target`
+ * (2048 when 0), folded into the zone.  Errors: "Generation failed: GridTooLarge" (more than
+ * 10,000,000 points), "...: InvalidGridResolution".  kind 2 (FCC transports) returns
+ * "...: NotImplemented" for T, O, I and "...: TransportNeedsTOrOOrI" otherwise. */
 int32_t zo_generate(zxt_engine* e, uint32_t group, uint32_t h, uint32_t target, uint32_t seed, uint32_t kind);
 
 /* src/exports.zig:29-40 */
@@ -102,6 +105,13 @@ int32_t zo_force_rows(zxt_engine* e, float* force4, double* energy, float* neare
 int32_t zo_near_pairs(zxt_engine* e, uint64_t* out2);
 /* The last near-field pass's site count, shell count and cell grid dimension. */
 int32_t zo_near_info(zxt_engine* e, uint32_t* out3);
+/* Host only, no device: the inverse Hopf grid's rows before the zone fold (src/hopf.zig:516-604
+ * points()), 4 floats each, in the reference's row order.  Returns the row count and writes the
+ * rows when `capacity` rows fit; returns 0 for an unknown group, f outside 1..512 or more than
+ * 10,000,000 rows. */
+uint32_t zo_hopf_points(uint32_t group, uint32_t f, float* out, uint32_t capacity);
+/* Host only: the frequency zo_generate takes for h = 0 and `target` (src/hopf.zig:82-97). */
+uint32_t zo_hopf_frequency(uint32_t group, uint32_t target);
 /* The last near-field pass's cutoff: chord^2, S3 radius r, and the number of Taylor terms
  * of the pair kernel's arctangent for it (0: the minimax polynomial). */
 int32_t zo_near_cut(zxt_engine* e, float* out3);

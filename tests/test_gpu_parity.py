@@ -330,7 +330,7 @@ def test_error_paths():
         with pytest.raises(RuntimeError, match="TooFewOrientations"):
             grid.relax(1)
     with pytest.raises(RuntimeError, match="NotImplemented"):
-        zo.generate("O", count=100)
+        zo.generate("O", count=100, exact=True)
 
 
 def test_fixed_identity_stays_exact(operator_fixture):

@@ -68,6 +68,8 @@ def _library():
         "zo_force_rows": ([ptr, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
         "zo_near_pairs": ([ptr, C.POINTER(C.c_uint64)], C.c_int),
         "zo_near_cut": ([ptr, C.POINTER(C.c_float)], C.c_int),
+        "zo_hopf_points": ([C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32], C.c_uint32),
+        "zo_hopf_frequency": ([C.c_uint32, C.c_uint32], C.c_uint32),
         # batches (no reference counterpart)
         "zo_batch_create": ([C.c_uint32], ptr), "zo_batch_destroy": ([ptr], None),
         "zo_batch_error": ([ptr], C.c_char_p),
@@ -115,8 +117,8 @@ class Grid:
     """A GPU-resident sample of orientation orbits on SO(3)/K. Close it to release the device.
 
     Quaternions have shape (N,4), scalar first, and right action q*g.  Construct samples
-    with haar() or from_quaternions(); inverse Hopf and FCC generation are not part of
-    this build (RuntimeError "Generation failed: NotImplemented").
+    with haar(), from_quaternions() or generate() (inverse Hopf product grids; FCC
+    transports and exact counts raise RuntimeError "...: NotImplemented").
     """
 
     def __init__(self, group="O", *, h=None, count=2048, quaternions=None, exact=False, identity=False,
@@ -347,6 +349,25 @@ class Grid:
         self._lib.zo_near_cut(self._handle, cut)
         return {"sites": out[0], "shell": out[1], "sdim": out[2], "pairs": pairs[0], "candidates": pairs[1],
                 "chord2": cut[0], "radius": cut[1], "atan_terms": int(cut[2])}
+
+
+def hopf_points(group, f):
+    """The frequency-f inverse Hopf grid of `group` before the zone fold, (N,4) float32 in the
+    reference's row order, computed on the host (no device)."""
+    lib = _library()
+    gid = _group_id(group)
+    n = lib.zo_hopf_points(gid, int(f), None, 0)
+    if n == 0:
+        raise ValueError(f"no inverse Hopf grid for {group} at frequency {f} (1-512, at most 10,000,000 points)")
+    out = np.empty((n, 4), np.float32)
+    if lib.zo_hopf_points(gid, int(f), out.ctypes.data, n) != n:
+        raise RuntimeError("inverse Hopf grid construction failed")
+    return out
+
+
+def hopf_frequency(group, count=2048):
+    """The frequency generate() takes for `count` when h is not given."""
+    return int(_library().zo_hopf_frequency(_group_id(group), int(count)))
 
 
 def generate(group="O", *, h=None, count=2048, exact=False, identity=False, fixed_identity=False, kind=None):

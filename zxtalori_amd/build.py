@@ -2,9 +2,9 @@
 
     python -m zxtalori_amd.build         # or python zxtalori_amd/build.py
 
-Device code is compiled with -ffp-contract=off (the exact-order image/chord arithmetic
-depends on it; the pair kernel writes its fmas explicitly), correctly rounded FP32
-division and sqrt, and without packed-FP32 instructions.
+Everything is compiled with -ffp-contract=off (the exact-order image/chord arithmetic
+depends on it; the pair kernel writes its fmas explicitly); device code with correctly
+rounded FP32 division and sqrt and without packed-FP32 instructions.
 """
 import os
 import shutil
@@ -18,10 +18,10 @@ CSRC = HERE / "csrc"
 LIB = HERE / "_lib" / "libzxtalori.so"
 ARCH = os.environ.get("ZXTALORI_ARCH", "gfx950")
 
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-ffp-contract=off"]
 DEVICE = [f"--offload-arch={ARCH}", "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["kernels.hip", "engine.cpp", "batch.cpp", "abi.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "batch.cpp", "abi.cpp", "hopf.cpp"]
 
 
 def hipcc():

@@ -10,6 +10,7 @@
 #include "batch.h"
 #include "engine.h"
 #include "groups_table.h"
+#include "hopf.h"
 #include "qvm_device.h"
 
 struct zxt_engine {
@@ -107,7 +108,7 @@ int32_t zo_set(zxt_engine* e, uint32_t group, const float* q, uint32_t n) {
   return guarded(e, "Import failed", [&] { e->engine.set(group, q, size_t(n) * 4); });
 }
 
-int32_t zo_generate(zxt_engine* e, uint32_t group, uint32_t, uint32_t, uint32_t, uint32_t kind) {
+int32_t zo_generate(zxt_engine* e, uint32_t group, uint32_t h, uint32_t target, uint32_t seed, uint32_t kind) {
   if (!valid_group(group)) {
     e->engine.err = "Unknown group";
     return -1;
@@ -116,7 +117,26 @@ int32_t zo_generate(zxt_engine* e, uint32_t group, uint32_t, uint32_t, uint32_t,
     e->engine.err = "Unknown grid kind";
     return -1;
   }
-  return not_implemented(e, "Generation failed");
+  return guarded(e, "Generation failed", [&] { e->engine.generate(group, h, target, seed, kind); });
+}
+
+uint32_t zo_hopf_points(uint32_t group, uint32_t f, float* out, uint32_t capacity) {
+  if (!valid_group(group) || f < 1 || f > zxt::hopf::kMaxFrequency) return 0;
+  if (zxt::hopf::raw_count(group, f) > zxt::hopf::kMaxCount) return 0;
+  const uint32_t n = uint32_t(zxt::hopf::raw_count(group, f));
+  if (!out || capacity < n) return n;
+  try {
+    const std::vector<float> rows = zxt::hopf::points(group, f);
+    std::memcpy(out, rows.data(), rows.size() * sizeof(float));
+    return n;
+  } catch (...) {
+    return 0;
+  }
+}
+
+uint32_t zo_hopf_frequency(uint32_t group, uint32_t target) {
+  if (!valid_group(group)) return 0;
+  return zxt::hopf::nearest_frequency(group, target == 0 ? 2048 : target);
 }
 
 uint32_t zo_count(zxt_engine* e) { return e->engine.count(); }

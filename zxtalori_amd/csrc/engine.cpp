@@ -14,6 +14,7 @@
 #include <limits>
 
 #include "groups_table.h"
+#include "hopf.h"
 
 namespace zxt {
 
@@ -437,6 +438,8 @@ void Engine::set(uint32_t group, const float* values, size_t count4) {
     throw;
   }
   generator_ = "Imported quaternions";
+  grid_kind_.clear();
+  h_ = 0;
   seed_ = 0;
   base_count_ = n_;
 }
@@ -454,9 +457,36 @@ void Engine::generate_random(uint32_t group, uint32_t count, uint32_t seed) {
     group_ = old;
     throw;
   }
+  grid_kind_.clear();
+  h_ = 0;
   generator_ = "Haar random";
   seed_ = seed;
   base_count_ = count;
+}
+
+void Engine::generate(uint32_t group, uint32_t h, uint32_t target, uint32_t seed, uint32_t kind) {
+  if (target > hopf::kMaxCount) fail("GridTooLarge");
+  if (kind == 2) fail(group < 8 ? "TransportNeedsTOrOOrI" : "NotImplemented");
+  const uint32_t f = h == 0 ? hopf::nearest_frequency(group, target == 0 ? 2048 : target) : h;
+  const std::vector<float> rows = hopf::points(group, f);
+  const uint32_t n = uint32_t(rows.size() / 4);
+  const uint32_t old = group_;
+  set_group(group);
+  try {
+    q_.ensure(size_t(n) * 16);
+    check(hipMemcpyAsync(q_.p, rows.data(), size_t(n) * 16, hipMemcpyHostToDevice, stream), "upload");
+    // Grid points are orbit images: the fold picks each one's zone representative.
+    fold(n);
+    adopt(n);
+  } catch (...) {
+    group_ = old;
+    throw;
+  }
+  generator_ = hopf::description(group);
+  grid_kind_ = "hopf";
+  h_ = f;
+  seed_ = seed;
+  base_count_ = n;
 }
 
 void Engine::set_neighbourhood(double delta) {
@@ -747,12 +777,14 @@ void Engine::identity(bool add, bool fixed) {
   if (n_ == 0) fail("NoOrientations");
   long index = identity_index();
   if (add && index < 0) {
-    const std::string generator = generator_;
-    const uint32_t base = base_count_, seed = seed_;
+    const std::string generator = generator_, grid = grid_kind_;
+    const uint32_t base = base_count_, seed = seed_, h = h_;
     std::vector<float> values(host_.begin(), host_.end());
     values.insert(values.end(), {1.0f, 0.0f, 0.0f, 0.0f});
     set(group_, values.data(), values.size());
     generator_ = generator;
+    grid_kind_ = grid;
+    h_ = h;
     base_count_ = base;
     seed_ = seed;
     index = long(n_) - 1;
@@ -853,8 +885,11 @@ std::string Engine::stats() {
   j.integer("count", n_);
   j.integer("count_extended", 2ll * ord * n_);
   j.str("generator", generator_);
-  j.null("grid");
-  j.integer("h", 0);
+  if (grid_kind_.empty())
+    j.null("grid");
+  else
+    j.str("grid", grid_kind_);
+  j.integer("h", h_);
   j.integer("seed", seed_);
   j.integer("iteration", iteration_);
   j.boolean("fixed_identity", fix_identity_);

@@ -166,6 +166,9 @@ class Engine {
   // Public operations (src/engine.zig).  They throw EngineError.
   void set(uint32_t group, const float* values, size_t count4);
   void generate_random(uint32_t group, uint32_t count, uint32_t seed);
+  // Inverse Hopf grid at frequency h, or with h = 0 the frequency whose count is nearest
+  // `target` (engine.zig:242-282); kind 0 or 1 (2, FCC transports, is not built).
+  void generate(uint32_t group, uint32_t h, uint32_t target, uint32_t seed, uint32_t kind);
   void measure();
   void spacing();
   void relax(uint32_t steps);
@@ -255,6 +258,8 @@ class Engine {
   uint32_t seed_ = 0;
   uint32_t base_count_ = 0;
   std::string generator_ = "none";
+  std::string grid_kind_;  // "hopf", or empty for samples that are not grids
+  uint32_t h_ = 0;
   bool measured_ = false;
   Measures measures_;
   uint32_t measure_iteration_ = 0;
