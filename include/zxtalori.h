@@ -41,8 +41,9 @@ int32_t zo_set(zxt_engine* e, uint32_t group, const float* q, uint32_t n);
 /* src/exports.zig:9-15 (src/engine.zig:242-282): kind 0 or 1, the inverse Hopf product grid
  * at frequency h (1..512), or with h = 0 the frequency whose count is nearest `target`
  * (2048 when 0), folded into the zone.  Errors: "Generation failed: GridTooLarge" (more than
- * 10,000,000 points), "...: InvalidGridResolution".  kind 2 (FCC transports) returns
- * "...: NotImplemented" for T, O, I and "...: TransportNeedsTOrOOrI" otherwise. */
+ * 10,000,000 points), "...: InvalidGridResolution".  kind 2: the FCC transport of T, O or I
+ * at semi-edge max(1, h) <= 128 in lattice order, not folded (src/grids.zig:40-77), or
+ * "...: TransportNeedsTOrOOrI" for other groups. */
 int32_t zo_generate(zxt_engine* e, uint32_t group, uint32_t h, uint32_t target, uint32_t seed, uint32_t kind);
 
 /* src/exports.zig:29-40 */

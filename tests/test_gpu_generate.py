@@ -88,8 +88,8 @@ def test_transport_kind():
         zo.generate("D4", kind="transport")
     with pytest.raises(ValueError):
         zo.generate("O", kind="lattice")
-    with pytest.raises(RuntimeError, match="NotImplemented"):
-        zo.generate("O", h=6, kind="transport")
+    with zo.generate("O", h=6, kind="transport") as grid:
+        assert len(grid) == 1043 and grid.stats["grid"] == "transport"
 
 
 def test_generated_grid_relaxes_and_measures():

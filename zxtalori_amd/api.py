@@ -2,8 +2,8 @@
 
 Mirrors the reference's Python adapter (python/zxtalori/api.py): the same names,
 argument meanings, error behaviour (RuntimeError carrying zo_error's text) and JSON stats.
-Only the QVM relaxation and spacing path runs here; generation (zo_generate) and covering
-raise RuntimeError("...: NotImplemented").
+The QVM relaxation and spacing path, inverse Hopf and FCC transport generation run here;
+exact-count generation and covering raise RuntimeError("...: NotImplemented").
 """
 from __future__ import annotations
 
@@ -117,8 +117,9 @@ class Grid:
     """A GPU-resident sample of orientation orbits on SO(3)/K. Close it to release the device.
 
     Quaternions have shape (N,4), scalar first, and right action q*g.  Construct samples
-    with haar(), from_quaternions() or generate() (inverse Hopf product grids; FCC
-    transports and exact counts raise RuntimeError "...: NotImplemented").
+    with haar(), from_quaternions() or generate() (inverse Hopf product grids, or the FCC
+    transports of T, O and I with kind="transport"; exact=True raises RuntimeError
+    "...: NotImplemented").
     """
 
     def __init__(self, group="O", *, h=None, count=2048, quaternions=None, exact=False, identity=False,

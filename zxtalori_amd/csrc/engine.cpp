@@ -15,6 +15,7 @@
 
 #include "groups_table.h"
 #include "hopf.h"
+#include "transport.h"
 
 namespace zxt {
 
@@ -466,7 +467,10 @@ void Engine::generate_random(uint32_t group, uint32_t count, uint32_t seed) {
 
 void Engine::generate(uint32_t group, uint32_t h, uint32_t target, uint32_t seed, uint32_t kind) {
   if (target > hopf::kMaxCount) fail("GridTooLarge");
-  if (kind == 2) fail(group < 8 ? "TransportNeedsTOrOOrI" : "NotImplemented");
+  if (kind == 2) {
+    generate_transport(group, std::max<uint32_t>(1, h), seed);
+    return;
+  }
   const uint32_t f = h == 0 ? hopf::nearest_frequency(group, target == 0 ? 2048 : target) : h;
   const std::vector<float> rows = hopf::points(group, f);
   const uint32_t n = uint32_t(rows.size() / 4);
@@ -485,6 +489,35 @@ void Engine::generate(uint32_t group, uint32_t h, uint32_t target, uint32_t seed
   generator_ = hopf::description(group);
   grid_kind_ = "hopf";
   h_ = f;
+  seed_ = seed;
+  base_count_ = n;
+}
+
+// FCC transport of T, O or I at semi-edge h (engine.zig:247-282 with grids.zig:40-77):
+// lattice order, not folded (the map lands in the zone).
+void Engine::generate_transport(uint32_t group, uint32_t h, uint32_t seed) {
+  const transport::Plan plan = transport::plan(group, h);
+  const uint32_t n = plan.job.count;
+  DevBuf tables, offsets;
+  tables.ensure(sizeof(fcc::Tables));
+  offsets.ensure(plan.offsets.size() * sizeof(uint32_t));
+  check(hipMemcpyAsync(tables.p, &transport::tables(), sizeof(fcc::Tables), hipMemcpyHostToDevice, stream), "upload");
+  check(hipMemcpyAsync(offsets.p, plan.offsets.data(), plan.offsets.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                       stream),
+        "upload");
+  const uint32_t old = group_;
+  set_group(group);
+  try {
+    q_.ensure(size_t(n) * 16);
+    launch_transport(prof, stream, tables.as<fcc::Tables>(), plan.job, offsets.as<uint32_t>(), q_.as<float4>());
+    adopt(n);
+  } catch (...) {
+    group_ = old;
+    throw;
+  }
+  generator_ = transport::description(group);
+  grid_kind_ = "transport";
+  h_ = h;
   seed_ = seed;
   base_count_ = n;
 }

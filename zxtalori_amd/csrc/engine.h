@@ -167,8 +167,10 @@ class Engine {
   void set(uint32_t group, const float* values, size_t count4);
   void generate_random(uint32_t group, uint32_t count, uint32_t seed);
   // Inverse Hopf grid at frequency h, or with h = 0 the frequency whose count is nearest
-  // `target` (engine.zig:242-282); kind 0 or 1 (2, FCC transports, is not built).
+  // `target` (engine.zig:242-282), for kind 0 or 1; kind 2 is the FCC transport of T, O or I
+  // at semi-edge max(1, h).
   void generate(uint32_t group, uint32_t h, uint32_t target, uint32_t seed, uint32_t kind);
+  void generate_transport(uint32_t group, uint32_t h, uint32_t seed);
   void measure();
   void spacing();
   void relax(uint32_t steps);
@@ -258,7 +260,7 @@ class Engine {
   uint32_t seed_ = 0;
   uint32_t base_count_ = 0;
   std::string generator_ = "none";
-  std::string grid_kind_;  // "hopf", or empty for samples that are not grids
+  std::string grid_kind_;  // "hopf" or "transport", empty for samples that are not grids
   uint32_t h_ = 0;
   bool measured_ = false;
   Measures measures_;

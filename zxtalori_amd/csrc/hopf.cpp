@@ -75,44 +75,6 @@ Quat turn(const Vec& axis, double k) {
   return {std::cos(kPi / k), a[0] * s, a[1] * s, a[2] * s};
 }
 
-// The group's elements in FP64: the closure of its generators (the frames of the operator
-// tables), breadth first with products element * generator, then scalar part made
-// nonnegative and ordered by decreasing scalar part, stably (hopf.zig:161-213).
-std::vector<Quat> elements(uint32_t group) {
-  const Vec z{0, 0, 1}, x{1, 0, 0}, d{1, 1, 1};
-  std::vector<Quat> gens;
-  switch (group) {
-    case C2: gens = {turn(z, 2)}; break;
-    case C3: gens = {turn(z, 3)}; break;
-    case C4: gens = {turn(z, 4)}; break;
-    case C6: gens = {turn(z, 6)}; break;
-    case D2: gens = {turn(z, 2), turn(x, 2)}; break;
-    case D3: gens = {turn(z, 3), turn(x, 2)}; break;
-    case D4: gens = {turn(z, 4), turn(x, 2)}; break;
-    case D6: gens = {turn(z, 6), turn(x, 2)}; break;
-    case T: gens = {turn(z, 2), turn(x, 2), turn(d, 3)}; break;
-    case O: gens = {turn(z, 4), turn(d, 3)}; break;
-    case I: gens = {turn(z, 5), turn({2, 0, 1}, 5)}; break;
-    default: break;  // C1
-  }
-  std::vector<Quat> e{{1, 0, 0, 0}};
-  for (size_t i = 0; i < e.size(); ++i) {
-    for (const Quat& g : gens) {
-      const Quat p = e[i] * g;
-      const bool known =
-          std::any_of(e.begin(), e.end(), [&](const Quat& k) { return std::fabs(dot(k, p)) > 1 - 1e-9; });
-      if (!known) e.push_back(p);
-    }
-  }
-  if (e.size() != kOrder[group]) fail("GroupClosure");
-  for (Quat& q : e) {
-    q = unit(q);
-    if (q[0] < 0) q = {-q[0], -q[1], -q[2], -q[3]};
-  }
-  std::stable_sort(e.begin(), e.end(), [](const Quat& a, const Quat& b) { return a[0] > b[0]; });
-  return e;
-}
-
 // R with R H R^-1 inside the root (so3k.product.embedding; hopf.zig:38-44).
 Quat embedding(uint32_t group) {
   switch (group) {
@@ -460,6 +422,41 @@ Quat section(const Vec& v) {
 }
 
 }  // namespace
+
+std::vector<std::array<double, 4>> elements(uint32_t group) {
+  const Vec z{0, 0, 1}, x{1, 0, 0}, d{1, 1, 1};
+  std::vector<Quat> gens;
+  switch (group) {
+    case C2: gens = {turn(z, 2)}; break;
+    case C3: gens = {turn(z, 3)}; break;
+    case C4: gens = {turn(z, 4)}; break;
+    case C6: gens = {turn(z, 6)}; break;
+    case D2: gens = {turn(z, 2), turn(x, 2)}; break;
+    case D3: gens = {turn(z, 3), turn(x, 2)}; break;
+    case D4: gens = {turn(z, 4), turn(x, 2)}; break;
+    case D6: gens = {turn(z, 6), turn(x, 2)}; break;
+    case T: gens = {turn(z, 2), turn(x, 2), turn(d, 3)}; break;
+    case O: gens = {turn(z, 4), turn(d, 3)}; break;
+    case I: gens = {turn(z, 5), turn({2, 0, 1}, 5)}; break;
+    default: break;  // C1
+  }
+  std::vector<Quat> e{{1, 0, 0, 0}};
+  for (size_t i = 0; i < e.size(); ++i) {
+    for (const Quat& g : gens) {
+      const Quat p = e[i] * g;
+      const bool known =
+          std::any_of(e.begin(), e.end(), [&](const Quat& k) { return std::fabs(dot(k, p)) > 1 - 1e-9; });
+      if (!known) e.push_back(p);
+    }
+  }
+  if (e.size() != kOrder[group]) fail("GroupClosure");
+  for (Quat& q : e) {
+    q = unit(q);
+    if (q[0] < 0) q = {-q[0], -q[1], -q[2], -q[3]};
+  }
+  std::stable_sort(e.begin(), e.end(), [](const Quat& a, const Quat& b) { return a[0] > b[0]; });
+  return e;
+}
 
 uint32_t root_of(uint32_t group) {
   switch (group) {

@@ -2,6 +2,7 @@
 // grid is built on the host in FP64, rounded once to FP32, and the engine folds it into
 // the zone on the GPU.
 #pragma once
+#include <array>
 #include <cstdint>
 #include <vector>
 
@@ -26,6 +27,11 @@ uint32_t nearest_frequency(uint32_t group, uint32_t target);
 
 // The generator text of the stats record (engine.zig:271-275).
 const char* description(uint32_t group);
+
+// The group's elements in FP64, scalar first: the closure of its generators (the frames
+// of the operator tables), breadth first, scalar part made nonnegative and ordered by
+// decreasing scalar part, stably (hopf.zig:161-213).
+std::vector<std::array<double, 4>> elements(uint32_t group);
 
 // The frequency-f grid as FP32 rows, scalar first, 4 floats per point; each point is one
 // image of its orbit (the engine's fold picks the zone representative).  Throws

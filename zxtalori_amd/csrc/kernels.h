@@ -8,6 +8,8 @@
 #include <string>
 #include <vector>
 
+#include "transport_math.h"
+
 namespace zxt {
 
 // Per-source force rows of one pass: (fx, fy, fz, curvature) in FP32, the energy sum in
@@ -141,6 +143,9 @@ JobSet single_job(const JobDesc& d);
 
 // ---- launchers (kernels.hip) ----------------------------------------------------
 void launch_haar(Prof& p, hipStream_t s, float4* q, uint32_t n, uint32_t seed);
+// FCC transport grid (job.count points, rows described by offsets) into out.
+void launch_transport(Prof& p, hipStream_t s, const fcc::Tables* tables, const fcc::Job& job, const uint32_t* offsets,
+                      float4* out);
 void launch_fold(Prof& p, hipStream_t s, const JobSet& jobs);
 void launch_fold(Prof& p, hipStream_t s, float4* q, uint32_t n, const float4* ops, uint32_t order, uint32_t group);
 void launch_fill_u32(Prof& p, hipStream_t s, uint32_t* dst, uint32_t value, size_t count);

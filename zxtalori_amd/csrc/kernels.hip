@@ -28,6 +28,7 @@
 
 #include "kernels.h"
 #include "qvm_device.h"
+#include "transport_math.h"
 
 namespace zxt {
 
@@ -252,6 +253,37 @@ void launch_haar(Prof& p, hipStream_t s, float4* q, uint32_t n, uint32_t seed) {
   x *= 0x846ca68bu;
   x ^= x >> 16;
   p.run("haar", [&] { hipLaunchKernelGGL(k_haar, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, q, n, x); });
+}
+
+// ---- FCC transports (grids.zig:40-77, grids.wgsl:165-174) ------------------------------
+// One thread per lattice point, in FP64: the point's (i, j) row by binary search of the row
+// offsets, then the volume-preserving map onto the zone (transport_math.h) and the
+// homochoric-to-quaternion solve; the result is rounded once to FP32.
+__global__ __launch_bounds__(256) void k_transport(const fcc::Tables* __restrict__ tb, fcc::Job job,
+                                                   const uint32_t* __restrict__ offsets, float4* __restrict__ out) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= job.count) return;
+  uint32_t lo = 0, hi = job.side * job.side;  // offsets[lo] <= n < offsets[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (offsets[mid] <= n)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int i = int(lo / job.side) - job.bound, j = int(lo % job.side) - job.bound;
+  int k0, kmax;
+  fcc::row_span(job, i, j, k0, kmax);
+  double q[4];
+  fcc::transport_point(*tb, job, i, j, k0 + 2 * int(n - offsets[lo]), q);
+  out[n] = make_float4(float(q[0]), float(q[1]), float(q[2]), float(q[3]));
+}
+
+void launch_transport(Prof& p, hipStream_t s, const fcc::Tables* tables, const fcc::Job& job, const uint32_t* offsets,
+                      float4* out) {
+  p.run("transport", [&] {
+    hipLaunchKernelGGL(k_transport, dim3(blocks_for(job.count, 256)), dim3(256), 0, s, tables, job, offsets, out);
+  });
 }
 
 // ---- fold (state.wgsl:1-51) ---------------------------------------------------------
