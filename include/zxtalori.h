@@ -89,6 +89,11 @@ bool zo_profile(zxt_engine* e, bool enabled);
 size_t zo_profile_report(zxt_engine* e, char* output, size_t capacity);
 /* Scales the all-pairs work split (tests shrink it: only rounding may change). */
 void zo_pair_budget(zxt_engine* e, float scale);
+/* No counterpart in the reference: the autotuner's hook (zxtalori_amd/tune.py).  Sets, for
+ * every engine of the process, the all-pairs force pass's tile plan: about `target`
+ * workgroups, `ops_per_split` operators per workgroup (0: automatic); target 0 returns to the
+ * compiled-in tuned table. */
+void zo_tune_allpairs(uint32_t target, uint32_t ops_per_split);
 
 /* Additions of this build (no reference counterpart). */
 /* Per point, the rank j*2*order + 2*k + sign of its nearest orbit point (source j,

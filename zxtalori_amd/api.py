@@ -62,6 +62,7 @@ def _library():
         "zo_profile": ([ptr, C.c_bool], C.c_bool),
         "zo_profile_report": ([ptr, C.c_char_p, C.c_size_t], C.c_size_t),
         "zo_pair_budget": ([ptr, C.c_float], None),
+        "zo_tune_allpairs": ([C.c_uint32, C.c_uint32], None),
         "zo_point_indices": ([ptr], C.POINTER(C.c_uint32)),
         "zo_force_passes": ([ptr, C.c_uint32], C.c_int),
         "zo_near_info": ([ptr, C.POINTER(C.c_uint32)], C.c_int),

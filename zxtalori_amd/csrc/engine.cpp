@@ -315,7 +315,7 @@ void RelaxState::record_forces(Engine& e, uint32_t n_, uint32_t order_) {
 JobDesc RelaxState::allpairs_desc(Engine& e, uint32_t n_, uint32_t order_) {
   ensure_rows(n_);
   JobDesc d = describe_job(static_cast<float4*>(e.q_device()), e.ops_device(), n_, order_, e.group());
-  d.plan = plan_allpairs(n_, order_, e.pair_budget_scale);
+  d.plan = plan_allpairs(n_, order_, e.pair_budget_scale, allpairs_shape(n_, order_));
   ap_partial.ensure(size_t(d.plan.splits()) * n_ * 5 * sizeof(double));
   ap_pnear.ensure(size_t(d.plan.splits()) * n_ * sizeof(float));
   d.ap_partial = ap_partial.as<double>();

@@ -150,7 +150,18 @@ void launch_fold(Prof& p, hipStream_t s, const JobSet& jobs);
 void launch_fold(Prof& p, hipStream_t s, float4* q, uint32_t n, const float4* ops, uint32_t order, uint32_t group);
 void launch_fill_u32(Prof& p, hipStream_t s, uint32_t* dst, uint32_t value, size_t count);
 
-AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale);
+// Tile plan of an all-pairs pass: about `target` workgroups (site chunks x operator
+// batches x 256-source blocks), operators batched `ops_per_split` to a workgroup (0: only
+// when the operator count alone would exceed 4 x target).
+struct AllPairsShape {
+  uint32_t target = 2048;
+  uint32_t ops_per_split = 0;
+};
+// The force pass's shape for n points and |K| = order: the tuned table (allpairs_tuned.h,
+// written by zxtalori_amd/tune.py) unless set_allpairs_override holds a shape.
+AllPairsShape allpairs_shape(uint32_t n, uint32_t order);
+void set_allpairs_override(uint32_t target, uint32_t ops_per_split);
+AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale, AllPairsShape shape = {});
 // All-pairs rows of every job in the set (each job's plan, partial rows and rows).
 void launch_allpairs(Prof& p, hipStream_t s, const JobSet& jobs);
 void launch_allpairs_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,

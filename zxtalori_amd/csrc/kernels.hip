@@ -24,11 +24,15 @@
 //    in-cell sort by a unique key), so every pass is deterministic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
+#include <cmath>
 #include <cstdio>
 
 #include "kernels.h"
 #include "qvm_device.h"
 #include "transport_math.h"
+#include "allpairs_tuned.h"
 
 namespace zxt {
 
@@ -484,15 +488,37 @@ __global__ void k_allpairs_reduce(JobSet s) {
   d.rows.nearest[i] = nearest;
 }
 
-AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale) {
+namespace {
+std::atomic<uint64_t> g_allpairs_override{0};
+}  // namespace
+
+void set_allpairs_override(uint32_t target, uint32_t ops_per_split) {
+  g_allpairs_override.store(uint64_t(target) << 32 | ops_per_split);
+}
+
+AllPairsShape allpairs_shape(uint32_t n, uint32_t order) {
+  const uint64_t o = g_allpairs_override.load();
+  if (o >> 32) return {uint32_t(o >> 32), uint32_t(o)};
+  int row = -1;
+  for (int r = 0; r < kTunedOrderCount; ++r)
+    if (kTunedOrders[r] == order) row = r;
+  if (row < 0) return {};
+  int lg = int(std::lround(std::log2(double(std::max<uint32_t>(n, 1)))));
+  lg = std::min(std::max(lg, kTunedLogMin), kTunedLogMin + kTunedSizeCount - 1);
+  return {kTunedTarget[row][lg - kTunedLogMin], kTunedOps[row][lg - kTunedLogMin]};
+}
+
+AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale, AllPairsShape shape) {
   AllPairsPlan p;
   const uint32_t sb = blocks_for(n, kBlock);
-  // Enough workgroups to fill 256 CUs several times; at most ~4096 splits of rows.
-  const uint32_t target = 2048;
+  const uint32_t target = std::max(shape.target, 1u);
   p.ops_per_split = 1;
   p.op_splits = order;
-  if (uint64_t(sb) * order > 4 * target) {
-    p.ops_per_split = uint32_t((uint64_t(sb) * order + 4 * target - 1) / (4 * target));
+  if (shape.ops_per_split) {
+    p.ops_per_split = std::min(order, shape.ops_per_split);
+    p.op_splits = (order + p.ops_per_split - 1) / p.ops_per_split;
+  } else if (uint64_t(sb) * order > 4 * uint64_t(target)) {
+    p.ops_per_split = uint32_t((uint64_t(sb) * order + 4 * uint64_t(target) - 1) / (4 * uint64_t(target)));
     p.op_splits = (order + p.ops_per_split - 1) / p.ops_per_split;
   }
   uint32_t chunks = 1;
