@@ -69,10 +69,15 @@ const char* zo_stats(zxt_engine* e);
 const char* zo_error(zxt_engine* e);
 void zo_cancel(zxt_engine* e);
 
-/* Out of this build's path (covering radius, Delaunay mesh, exact counts), present so the
- * reference's loader resolves every name (src/exports.zig:88-120).  They return -1 (or
- * 0/NULL) with a "NotImplemented" message. */
+/* src/exports.zig:45-48 (engine.zig:378-398): the covering radius of the sample, the
+ * largest empty cap over every signed orbit image, into the stats record
+ * (covering_radius_deg, covering_witness, ...) and point values 1 (cap radius, degrees) and
+ * 2 (Voronoi faces); reused while the sample is unchanged.  Errors: "Covering radius:
+ * TooFewOrientations" (fewer than 4), "...: CoveringDomainNotEnclosed". */
 int32_t zo_cover(zxt_engine* e);
+/* Out of this build's path (covering optimisation, Delaunay mesh, exact counts), present so
+ * the reference's loader resolves every name (src/exports.zig:88-120).  They return -1 (or
+ * 0/NULL) with a "NotImplemented" message. */
 int32_t zo_cover_optimize(zxt_engine* e, uint32_t steps, uint32_t mode, uint32_t finish, float floor_deg);
 int32_t zo_cover_kicks(zxt_engine* e, uint32_t patience, float progress, float size, float trust);
 int32_t zo_mesh(zxt_engine* e);

@@ -2,8 +2,9 @@
 
 Mirrors the reference's Python adapter (python/zxtalori/api.py): the same names,
 argument meanings, error behaviour (RuntimeError carrying zo_error's text) and JSON stats.
-The QVM relaxation and spacing path, inverse Hopf and FCC transport generation run here;
-exact-count generation and covering raise RuntimeError("...: NotImplemented").
+The QVM relaxation and spacing path, inverse Hopf and FCC transport generation and the
+covering radius run here; exact-count generation, covering optimisation and the Delaunay
+mesh raise RuntimeError("...: NotImplemented").
 """
 from __future__ import annotations
 
@@ -196,8 +197,8 @@ class Grid:
         """Per-orientation values, or None before the pass that makes them.
 
         "separation": angle to the nearest other orbit point (SO(3) degrees), from every
-        energy pass and spacing().  "covering" and "faces" come from covering passes,
-        which this build does not run (always None).
+        energy pass and spacing().  "covering": radius of the largest empty cap the point's
+        Voronoi cell reaches (degrees) and "faces": faces of that cell, from covering().
         """
         self._open()
         kinds = {"separation": 0, "covering": 1, "faces": 2}
@@ -260,6 +261,9 @@ class Grid:
         return self
 
     def covering(self):
+        """Covering radius (stats covering_radius_deg): the largest angle from any
+        orientation to the sample's nearest orbit point.  Reused while the sample is
+        unchanged (stats covering_cached)."""
         self._open()
         self._check(self._lib.zo_cover(self._handle))
         return self.stats

@@ -171,7 +171,9 @@ const char* zo_stats(zxt_engine* e) {
 const char* zo_error(zxt_engine* e) { return e->engine.err.c_str(); }
 void zo_cancel(zxt_engine* e) { e->engine.cancelled.store(true); }
 
-int32_t zo_cover(zxt_engine* e) { return not_implemented(e, "Covering radius"); }
+int32_t zo_cover(zxt_engine* e) {
+  return guarded(e, "Covering radius", [&] { e->engine.cover(); });
+}
 int32_t zo_cover_optimize(zxt_engine* e, uint32_t, uint32_t, uint32_t, float) {
   return not_implemented(e, "Covering optimization");
 }

@@ -388,6 +388,8 @@ void Engine::fetch() {
 void Engine::reset_run() {
   point_separation_iteration_ = -1;
   point_index_iteration_ = -1;
+  point_covering_iteration_ = -1;
+  cover_ = CoverResult{};
   relax_state_.release();
   spacing_state_.release();
   measure_state_.release();
@@ -829,6 +831,8 @@ void Engine::identity(bool add, bool fixed) {
   if (fixed) {
     const float one[4] = {1.0f, 0.0f, 0.0f, 0.0f};
     std::memcpy(&host_[size_t(index) * 4], one, 16);
+    cover_ = CoverResult{};
+    point_covering_iteration_ = -1;
     check(hipMemcpyAsync(static_cast<char*>(q_.p) + size_t(index) * 16, one, 16, hipMemcpyHostToDevice, stream),
           "identity");
     sync();
@@ -852,8 +856,98 @@ void Engine::force_passes(uint32_t passes) {
 }
 
 const float* Engine::point_values(uint32_t kind) const {
-  if (kind != 0 || point_separation_iteration_ < 0 || point_separation_.size() != n_) return nullptr;
-  return point_separation_.data();
+  if (kind == 0) {
+    if (point_separation_iteration_ < 0 || point_separation_.size() != n_) return nullptr;
+    return point_separation_.data();
+  }
+  if (kind == 1 || kind == 2) {
+    const PinnedVec<float>& v = kind == 1 ? point_covering_ : point_faces_;
+    if (point_covering_iteration_ < 0 || v.size() != n_) return nullptr;
+    return v.data();
+  }
+  return nullptr;
+}
+
+// ---- covering radius ----------------------------------------------------------------
+// Per source, the farthest vertex of its Voronoi cell among all signed orbit images
+// (k_cover); the sample's covering radius is the largest.  Clouds of at most 131,072
+// images take every image as a candidate; larger ones the near-field site grid built at
+// the cutoff c_max, which grows (x1.6) until every cell is certified within it.
+void Engine::cover() {
+  if (n_ < 4) fail("TooFewOrientations");
+  if (cover_.valid && cover_.iteration == long(iteration_) && point_covering_iteration_ == long(iteration_)) {
+    cover_.cached = true;
+    return;
+  }
+  const uint32_t ord = order();
+  const double density = double(n_) * ord;
+  const double spacing = std::cbrt(kPi * kPi / density);  // S3 angle per cloud point
+  const bool all_images = density <= 131072.0;
+  const double limit = all_images ? 3.0 : 1.2;
+  double c_max = std::min(limit, 5.0 * spacing);
+  const float c_start = float(std::min(c_max, 2.4 * spacing));
+  cover_theta_.ensure(size_t(n_) * 4);
+  cover_witness_.ensure(size_t(n_) * 16);
+  cover_flags_.ensure(size_t(n_) * 4);
+  cover_faces_.ensure(size_t(n_) * 4);
+  float4* q = q_.as<float4>();
+  uint32_t rounds = 0, rebuilt = 0;
+  while (rounds < 8) {
+    ++rounds;
+    NearBuffers view;
+    const NearBuffers* grid = nullptr;
+    NearCut cut{};
+    if (!all_images) {
+      const double mr = mean_radius(uint64_t(n_) * ord);
+      const double delta = c_max / mr;
+      if (cover_near_.delta != delta) {
+        cover_near_.delta = delta;
+        cover_near_.ready = false;
+      }
+      cover_near_.build(*this, q, n_, ord, group_, mr);
+      view = cover_near_.view();
+      grid = &view;
+      cut = cover_near_.cut;
+    }
+    launch_cover(prof, stream, q, n_, ops_device(), ord, grid, cut, c_start, float(c_max), cover_theta_.as<float>(),
+                 cover_witness_.as<float4>(), cover_flags_.as<uint32_t>(), cover_faces_.as<float>());
+    download(point_covering_, cover_theta_, n_);
+    download(point_faces_, cover_faces_, n_);
+    download(cover_flags_host_, cover_flags_, n_);
+    sync();
+    if (!all_images && cover_near_.overflowed(*this)) {
+      if (++rebuilt > 4) fail("NearShellTooLarge");
+      --rounds;
+      continue;
+    }
+    uint32_t open = 0;
+    for (uint32_t i = 0; i < n_; ++i) {
+      if (cover_flags_host_[i] == 2) fail("CoveringNeighbourOverflow");
+      open += cover_flags_host_[i] != 0 ? 1u : 0u;
+    }
+    if (open) {
+      if (c_max >= limit) fail("CoveringDomainNotEnclosed");
+      c_max = std::min(limit, c_max * 1.6);
+      continue;
+    }
+    uint32_t worst = 0;
+    for (uint32_t i = 1; i < n_; ++i)
+      if (point_covering_[i] > point_covering_[worst]) worst = i;
+    CoverResult r;
+    r.valid = true;
+    r.iteration = long(iteration_);
+    r.radius_deg = double(point_covering_[worst]) * 360.0 / kPi;
+    r.domain_deg = c_max * 360.0 / kPi;
+    check(hipMemcpy(r.witness, cover_witness_.as<float4>() + worst, 16, hipMemcpyDeviceToHost), "download");
+    r.rounds = rounds;
+    r.probes = n_ * rounds;
+    r.sites = all_images ? uint32_t(std::min<double>(4e9, 2.0 * density)) : cover_near_.status.sites;
+    for (uint32_t i = 0; i < n_; ++i) point_covering_[i] = float(double(point_covering_[i]) * 360.0 / kPi);
+    cover_ = r;
+    point_covering_iteration_ = long(iteration_);
+    return;
+  }
+  fail("CoveringRetryLimit");
 }
 
 const uint32_t* Engine::point_indices() const {
@@ -895,6 +989,16 @@ struct Json {
     key(k);
     s += v ? "true" : "false";
   }
+  void floats(const char* k, const float* v, int count) {
+    key(k);
+    s += "[";
+    for (int i = 0; i < count; ++i) {
+      char b[64];
+      snprintf(b, sizeof(b), "%s%.9g", i ? "," : "", double(v[i]));
+      s += b;
+    }
+    s += "]";
+  }
   void str(const char* k, const std::string& v) {
     key(k);
     s += "\"";
@@ -933,15 +1037,53 @@ std::string Engine::stats() {
     j.null("identity_index");
   j.integer("base_count", base_count_);
   j.integer("inserted_count", 0);
-  j.str("triangulation_backend", "none");
-  for (const char* k : {"simplex_bound_deg", "covering_excess", "mesh_ratio", "covering_radius_deg"}) j.null(k);
-  j.boolean("covering_current", false);
-  j.boolean("covering_cached", false);
-  for (const char* k : {"covering_unresolved_caps", "covering_iteration", "covering_witness",
-                        "covering_domain_bound_deg", "covering_shell_points", "covering_fence_points",
-                        "covering_tetrahedra", "covering_certificate_probes", "covering_exact_predicates",
-                        "covering_rounds"})
-    j.null(k);
+  j.str("triangulation_backend", "hip");
+  // Simplex bound (SO3K_Transports so3kt/measures/reference.py; covering_reference.zig:4-15):
+  // sum_i c_i count^-(2i+1)/3 in SO(3) degrees, count = 2 |K| N.
+  double bound = NAN;
+  if (n_ > 0) {
+    static const double c[8] = {1.889832101454757, 0.573930763512449, 0.581069217539205, 0.859789414116394,
+                                1.490131881944242, 2.791922498626376, 5.481645765756282, 11.12163594341394};
+    const double t = std::cbrt(1.0 / (2.0 * ord * double(n_)));
+    double v = 0.0;
+    for (int i = 7; i >= 0; --i) v = c[i] + t * t * v;
+    bound = t * v * 360.0 / kPi;
+  }
+  const bool cover_current = cover_.valid && cover_.iteration == long(iteration_);
+  j.num("simplex_bound_deg", bound);
+  if (cover_.valid) {
+    j.num("covering_excess", cover_.radius_deg / bound - 1.0);
+  } else {
+    j.null("covering_excess");
+  }
+  if (cover_current && current && measures_.separation_deg > 0)
+    j.num("mesh_ratio", cover_.radius_deg / measures_.separation_deg);
+  else
+    j.null("mesh_ratio");
+  if (cover_.valid)
+    j.num("covering_radius_deg", cover_.radius_deg);
+  else
+    j.null("covering_radius_deg");
+  j.boolean("covering_current", cover_current);
+  j.boolean("covering_cached", cover_.valid && cover_.cached);
+  if (cover_.valid) {
+    j.integer("covering_unresolved_caps", 0);
+    j.integer("covering_iteration", cover_.iteration);
+    j.floats("covering_witness", cover_.witness, 4);
+    j.num("covering_domain_bound_deg", cover_.domain_deg);
+    j.integer("covering_shell_points", cover_.sites);
+    j.integer("covering_fence_points", 0);
+    j.null("covering_tetrahedra");
+    j.integer("covering_certificate_probes", cover_.probes);
+    j.integer("covering_exact_predicates", 0);
+    j.integer("covering_rounds", cover_.rounds);
+  } else {
+    for (const char* k : {"covering_unresolved_caps", "covering_iteration", "covering_witness",
+                          "covering_domain_bound_deg", "covering_shell_points", "covering_fence_points",
+                          "covering_tetrahedra", "covering_certificate_probes", "covering_exact_predicates",
+                          "covering_rounds"})
+      j.null(k);
+  }
   j.boolean("measured", m);
   j.boolean("measures_current", current);
   if (m)
@@ -957,7 +1099,10 @@ std::string Engine::stats() {
     j.integer("point_separation_iteration", point_separation_iteration_);
   else
     j.null("point_separation_iteration");
-  j.null("point_covering_iteration");
+  if (point_covering_iteration_ >= 0)
+    j.integer("point_covering_iteration", point_covering_iteration_);
+  else
+    j.null("point_covering_iteration");
   if (m && measure_neighbourhood_ > 0)
     j.num("energy_neighbourhood", measure_neighbourhood_);
   else

@@ -171,6 +171,9 @@ class Engine {
   // at semi-edge max(1, h).
   void generate(uint32_t group, uint32_t h, uint32_t target, uint32_t seed, uint32_t kind);
   void generate_transport(uint32_t group, uint32_t h, uint32_t seed);
+  // Covering radius of the sample (engine.zig:378-398, covering.zig:555-577): measured
+  // again only when the sample changed since the last measurement.
+  void cover();
   void measure();
   void spacing();
   void relax(uint32_t steps);
@@ -269,6 +272,20 @@ class Engine {
   RelaxState relax_state_, spacing_state_, measure_state_;
   double neighbourhood_ = 0, measure_neighbourhood_ = 0;
   PinnedVec<float> point_separation_;
+  // Covering: the last result, per-point cap radii (SO(3) degrees) and Voronoi face counts.
+  struct CoverResult {
+    bool valid = false, cached = false;
+    long iteration = -1;
+    double radius_deg = 0, domain_deg = 0;
+    float witness[4] = {0, 0, 0, 0};
+    uint32_t rounds = 0, probes = 0, sites = 0;
+  };
+  CoverResult cover_;
+  NearField cover_near_;
+  DevBuf cover_theta_, cover_witness_, cover_flags_, cover_faces_;
+  PinnedVec<float> point_covering_, point_faces_;
+  PinnedVec<uint32_t> cover_flags_host_;
+  long point_covering_iteration_ = -1;
   PinnedVec<uint32_t> point_index_;
   long point_separation_iteration_ = -1;
   long point_index_iteration_ = -1;

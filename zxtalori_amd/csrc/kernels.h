@@ -227,6 +227,13 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
                          const NearCut& cut, float* chord_out, uint32_t* rank_out);
 
 // SO(3) degrees of nearest chords (out) and their minimum's bits (*least).
+// Covering radius per source (S3 angle of its Voronoi cell's farthest vertex), the vertex,
+// a flag (0 certified, 1 needs a cutoff beyond c_max, 2 too many neighbours) and the cell's
+// face count; grid: the near-field buffers built at cutoff c_max, or null to take every
+// signed image.
+void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                  const NearBuffers* grid, const NearCut& cut, float c_start, float c_max, float* theta,
+                  float4* witness, uint32_t* flags, float* faces);
 void launch_degrees(Prof& p, hipStream_t s, const float* chord, uint32_t n, float* out, uint32_t* least);
 
 // BB step (relax.wgsl) of every job in the set, resident: stats, sums, median, move.

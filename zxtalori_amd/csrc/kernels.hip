@@ -1571,6 +1571,392 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
   });
 }
 
+// ---- covering radius (covering.zig:177-280 and :555-577: the largest empty cap) ---------
+// The covering radius of the sample is the largest angle from any rotation to its nearest
+// orbit point: the farthest vertex of any orbit point's Voronoi cell on S3 (the cloud of all
+// signed images).  One wave per source p.  In the gnomonic chart at p, t in the tangent
+// space spanned by p.i, p.j, p.k stands for the direction p + t, at angle atan|t| from p;
+// the bisector with a neighbour y is the plane n.t <= d, n = (y.(p i), y.(p j), y.(p k)),
+// d = 1 - y.p.  The cell's vertices are the points where three planes meet and no plane is
+// violated; the kernel enumerates plane triples in FP64 (64 lanes split them), which stays
+// exact in the cospherical configurations of lattice grids, where only the largest vertex
+// distance matters and not the cell's combinatorics.
+//   The cell starts as the chart box |t_k| <= kCoverBox and is clipped by the neighbours in
+// distance windows (at most kCoverWindow new planes each): the window's planes join the
+// current faces, a first pass over the faces and the 32 nearest new planes bounds the
+// cell's radius by theta1 (new planes at angle >= 2 theta1 cannot cut it), a second pass
+// gives the cell, and only its faces are kept (the cell only shrinks, so a plane that
+// misses it now misses it for good).  The cell is certified when 2 theta is below the
+// distance of every unseen neighbour; the windows grow up to c_max.
+constexpr uint32_t kCoverGather = 512;  // candidates held while a window is chosen
+constexpr uint32_t kCoverWindow = 64;   // new planes per window
+constexpr uint32_t kCoverPlanes = 320;  // faces kept plus one window
+constexpr uint32_t kCoverFirst = 32;
+constexpr double kCoverBox = 8.0;       // |t_k| <= 8: cells within ~82 degrees (S3) of p
+
+struct CoverLds {
+  float4 y[kCoverGather];
+  float d2[kCoverGather];
+  double pl[kCoverPlanes][4];  // n.x, n.y, n.z, d: current faces, then the window nearest first
+  float pd2[kCoverPlanes];     // chord^2 of the plane's neighbour
+  uint8_t face[kCoverPlanes];  // the plane bounds the cell (some vertex lies on it)
+  float cut;
+};
+
+struct CoverVertex {
+  double r2;
+  double t[3];
+};
+
+// Plane i of a pass over m neighbour planes: i < m a neighbour, else one of the six faces of
+// the box.
+__device__ inline void cover_plane(const CoverLds& L, uint32_t m, uint32_t i, double* n, double& d) {
+  if (i < m) {
+    n[0] = L.pl[i][0];
+    n[1] = L.pl[i][1];
+    n[2] = L.pl[i][2];
+    d = L.pl[i][3];
+    return;
+  }
+  const uint32_t f = i - m;
+  n[0] = n[1] = n[2] = 0.0;
+  n[f >> 1] = (f & 1) ? -1.0 : 1.0;
+  d = kCoverBox;
+}
+
+// The farthest vertex of the cell of m neighbour planes and the box, over the wave; with
+// `faces`, every vertex is checked and the planes through one are marked in L.face.
+__device__ CoverVertex cover_cell(CoverLds& L, uint32_t m, bool faces) {
+  const uint32_t lane = lane_id();
+  const uint32_t total = m + 6;
+  CoverVertex best{-1.0, {0.0, 0.0, 0.0}};
+  for (uint32_t a = 0; a + 2 < total; ++a) {
+    double na[3], da;
+    cover_plane(L, m, a, na, da);
+    const uint32_t r = total - a - 1;
+    const uint32_t pairs = r * (r - 1) / 2;
+    for (uint32_t mm = lane; mm < pairs; mm += 64) {
+      // (u, v), u < v < r, the mm-th pair in lexicographic order.
+      const double w = double(2 * r - 1);
+      uint32_t u = uint32_t((w - sqrt(fmax(w * w - 8.0 * double(mm), 0.0))) * 0.5);
+      auto first = [&](uint32_t k) { return k * (2 * r - k - 1) / 2; };
+      while (u + 1 < r && first(u + 1) <= mm) ++u;
+      while (u > 0 && first(u) > mm) --u;
+      const uint32_t v = mm - first(u) + u + 1;
+      const uint32_t b = a + 1 + u, c = a + 1 + v;
+      double nb[3], db, nc[3], dc;
+      cover_plane(L, m, b, nb, db);
+      cover_plane(L, m, c, nc, dc);
+      const double bc[3] = {nb[1] * nc[2] - nb[2] * nc[1], nb[2] * nc[0] - nb[0] * nc[2], nb[0] * nc[1] - nb[1] * nc[0]};
+      const double ca[3] = {nc[1] * na[2] - nc[2] * na[1], nc[2] * na[0] - nc[0] * na[2], nc[0] * na[1] - nc[1] * na[0]};
+      const double ab[3] = {na[1] * nb[2] - na[2] * nb[1], na[2] * nb[0] - na[0] * nb[2], na[0] * nb[1] - na[1] * nb[0]};
+      const double det = na[0] * bc[0] + na[1] * bc[1] + na[2] * bc[2];
+      const double scale = sqrt((na[0] * na[0] + na[1] * na[1] + na[2] * na[2]) *
+                                (nb[0] * nb[0] + nb[1] * nb[1] + nb[2] * nb[2]) *
+                                (nc[0] * nc[0] + nc[1] * nc[1] + nc[2] * nc[2]));
+      if (!(fabs(det) > 1e-13 * scale)) continue;
+      const double inv = 1.0 / det;
+      const double t[3] = {(da * bc[0] + db * ca[0] + dc * ab[0]) * inv, (da * bc[1] + db * ca[1] + dc * ab[1]) * inv,
+                           (da * bc[2] + db * ca[2] + dc * ab[2]) * inv};
+      const double r2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+      if (!faces && r2 <= best.r2) continue;  // cannot raise this lane's maximum
+      const double tl = sqrt(r2);
+      bool inside = true;
+      for (uint32_t j = 0; j < m && inside; ++j) {
+        if (j == a || j == b || j == c) continue;
+        const double sj = L.pl[j][0] * t[0] + L.pl[j][1] * t[1] + L.pl[j][2] * t[2];
+        const double nl = sqrt(L.pl[j][0] * L.pl[j][0] + L.pl[j][1] * L.pl[j][1] + L.pl[j][2] * L.pl[j][2]);
+        inside = sj <= L.pl[j][3] + 1e-11 * (nl * tl + L.pl[j][3]);
+      }
+      const double slack = kCoverBox * (1.0 + 1e-12);
+      inside = inside && fabs(t[0]) <= slack && fabs(t[1]) <= slack && fabs(t[2]) <= slack;
+      if (!inside) continue;
+      if (r2 > best.r2) best = {r2, {t[0], t[1], t[2]}};
+      if (faces) {
+        if (a < m) L.face[a] = 1;
+        if (b < m) L.face[b] = 1;
+        if (c < m) L.face[c] = 1;
+      }
+    }
+  }
+  // The wave's farthest vertex (lowest lane on a tie).
+  double r2 = best.r2;
+  uint32_t who = lane;
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(r2, off);
+    const uint32_t ow = uint32_t(__shfl_xor(int(who), off));
+    if (o > r2 || (o == r2 && ow < who)) {
+      r2 = o;
+      who = ow;
+    }
+  }
+  CoverVertex out;
+  out.r2 = r2;
+  for (int k = 0; k < 3; ++k) out.t[k] = __shfl(best.t[k], int(who));
+  return out;
+}
+
+struct CoverArgs {
+  const float4* q;
+  uint32_t n;
+  const float4* ops;  // all-images mode: every signed image is a candidate
+  uint32_t order;
+  uint32_t grid;      // grid mode: the near-field sites within the cutoff
+  NearBuffers b;
+  NearCut cut;        // the near field's cutoff (c_max)
+  float c_start, c_max;
+  float* theta;
+  float4* witness;
+  uint32_t* flags;    // 0 certified, 1 needs a cutoff beyond c_max, 2 too many neighbours
+  float* faces;       // faces of the cell
+};
+
+__device__ inline float chord2_of_angle(float a) {
+  const float h = sinf(0.5f * fminf(a, 3.1415f));
+  return 4.0f * h * h;
+}
+__device__ inline double angle_of_chord2(float d2) { return 2.0 * asin(fmin(1.0, 0.5 * sqrt(double(d2)))); }
+
+__global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
+  __shared__ CoverLds L;
+  const uint32_t slot = blockIdx.x;
+  if (slot >= a.n) return;
+  const uint32_t lane = lane_id();
+  const uint32_t src = a.grid ? a.b.order[slot] : slot;
+  const float4 x = a.q[src];
+  const double xn = 1.0 / sqrt(double(x.x) * x.x + double(x.y) * x.y + double(x.z) * x.z + double(x.w) * x.w);
+  const double p[4] = {x.x * xn, x.y * xn, x.z * xn, x.w * xn};
+  // Tangent basis p i, p j, p k.
+  const double e[3][4] = {{-p[1], p[0], p[3], -p[2]}, {-p[2], -p[3], p[0], p[1]}, {-p[3], p[2], -p[1], p[0]}};
+  // Grid-mode row walk setup: every row within the field's cutoff (as in k_spacing).
+  const NearBuffers& b = a.b;
+  float cx = 0, cy = 0, cz = 0, rc = 0;
+  bool full = true;
+  int zlo = 0, ylo = 0;
+  uint32_t ny = 0, nrows = 0;
+  if (a.grid) {
+    const float fdim = float(b.sdim);
+    const float cpu = b.info->cpu;
+    const float denom = a.cut.c0 + x.x;
+    full = !(denom > 1e-3f);
+    cx = cy = cz = 0.5f * fdim;
+    rc = 3e18f;
+    if (!full) {
+      const float inv = 1.0f / denom;
+      cx = (x.y * inv - b.info->lower[0]) * cpu;
+      cy = (x.z * inv - b.info->lower[1]) * cpu;
+      cz = (x.w * inv - b.info->lower[2]) * cpu;
+      rc = (a.cut.s0 * inv * 1.001f + 1e-6f) * cpu + 1e-3f;
+    }
+    auto clampc = [&](float v) -> int { return int(fminf(fmaxf(floorf(v), 0.0f), fdim - 1.0f)); };
+    zlo = clampc(cz - rc);
+    ylo = clampc(cy - rc);
+    const int zhi = clampc(cz + rc), yhi = clampc(cy + rc);
+    ny = uint32_t(yhi - ylo + 1);
+    nrows = ny * uint32_t(zhi - zlo + 1);
+  }
+  uint32_t faces = 0, flag = 1, nfaces = 0;
+  float lo2 = -1.0f, hi = fminf(a.c_start, a.c_max);
+  CoverVertex cell{0.0, {0.0, 0.0, 0.0}};
+  for (int round = 0; round < 256; ++round) {
+    float hi2 = chord2_of_angle(hi);
+    // Gather the window (lo2, hi2].
+    uint32_t count = 0;
+    auto take = [&](bool ok, const float4 y, float d2) {
+      const uint64_t mask = __ballot(ok);
+      const uint32_t pos = count + mbcnt(mask);
+      if (ok && pos < kCoverGather) {
+        L.y[pos] = y;
+        L.d2[pos] = d2;
+      }
+      count += uint32_t(__popcll(mask));
+    };
+    if (a.grid) {
+      const uint32_t sdim = b.sdim;
+      const float fdim = float(sdim);
+      auto clampc = [&](float v) -> int { return int(fminf(fmaxf(floorf(v), 0.0f), fdim - 1.0f)); };
+      const float rc2 = rc * rc;
+      const uint32_t self_rank = src * 2 * a.order;
+      for (uint32_t rb = 0; rb < nrows; rb += 64) {
+        const uint32_t r = rb + lane;
+        uint32_t start = 0, len = 0;
+        if (r < nrows) {
+          const int yy = ylo + int(r % ny), zz = zlo + int(r / ny);
+          const float gy = row_gap(cy, yy, sdim), gz = row_gap(cz, zz, sdim);
+          const float rest = rc2 - gy * gy - gz * gz;
+          if (rest >= 0.0f) {
+            const float span = full ? 3e18f : sqrtf(rest);
+            const int x0 = clampc(cx - span), x1 = clampc(cx + span);
+            const uint32_t row = sdim * (uint32_t(yy) + sdim * uint32_t(zz));
+            start = b.offsets[row + x0];
+            len = b.offsets[row + x1 + 1] - start;
+          }
+        }
+        const uint32_t rows_here = min(64u, nrows - rb);
+        for (uint32_t k = 0; k < rows_here; ++k) {
+          const uint32_t n_row = uint32_t(__shfl(int(len), int(k)));
+          const uint32_t s_row = uint32_t(__shfl(int(start), int(k)));
+          for (uint32_t c0 = 0; c0 < n_row; c0 += 64) {
+            const uint32_t c = c0 + lane;
+            bool ok = false;
+            float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
+            float d2 = 0.f;
+            if (c < n_row) {
+              y = b.sq[s_row + c];
+              d2 = chord2(x, y);
+              ok = d2 > lo2 && d2 <= hi2 && b.srank[s_row + c] != self_rank;
+            }
+            take(ok, y, d2);
+          }
+        }
+      }
+    } else {
+      const uint32_t images = a.n * a.order;
+      for (uint32_t base = 0; base < images; base += 64) {
+        const uint32_t idx = base + lane;
+        bool ok = false;
+        float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
+        float d2 = 0.f;
+        if (idx < images) {
+          const uint32_t j = idx / a.order, k = idx % a.order;
+          y = image(a.q[j], a.ops[k], k);
+          if (dot4(x, y) < 0.0f) y = neg4(y);
+          d2 = chord2(x, y);
+          ok = d2 > lo2 && d2 <= hi2 && !(j == src && k == 0);
+        }
+        take(ok, y, d2);
+      }
+    }
+    if (count > kCoverGather) {
+      // Too many in the window: narrow it (a third of the volume) and gather again.
+      const float lo = lo2 < 0.0f ? 0.0f : float(angle_of_chord2(lo2));
+      const float narrower = lo + (hi - lo) * 0.6f;
+      if (!(narrower > lo * (1.0f + 1e-6f) + 1e-9f)) {
+        flag = 2;
+        break;
+      }
+      hi = narrower;
+      continue;
+    }
+    __syncthreads();
+    // Ranks by chord^2 (ties by gather order); the window keeps the nearest kCoverWindow and
+    // everything as near as the last of them.
+    uint32_t my_rank[kCoverGather / 64];
+    for (uint32_t s = 0; s < kCoverGather / 64; ++s) {
+      const uint32_t i = lane + 64 * s;
+      my_rank[s] = 0xffffffffu;
+      if (i < count) {
+        const float di = L.d2[i];
+        uint32_t rank = 0;
+        for (uint32_t j = 0; j < count; ++j) rank += (L.d2[j] < di || (L.d2[j] == di && j < i)) ? 1u : 0u;
+        my_rank[s] = rank;
+        if (rank == kCoverWindow - 1) L.cut = di;
+      }
+    }
+    __syncthreads();
+    uint32_t take_n = count;
+    if (count > kCoverWindow) {
+      const float cut = L.cut;
+      uint32_t sel = 0;
+      for (uint32_t i = lane; i < count; i += 64) sel += L.d2[i] <= cut ? 1u : 0u;
+      for (int off = 32; off > 0; off >>= 1) sel += uint32_t(__shfl_xor(int(sel), off));
+      take_n = sel;
+      hi2 = cut;
+    }
+    if (faces + take_n > kCoverPlanes) {
+      flag = 2;
+      break;
+    }
+    for (uint32_t s = 0; s < kCoverGather / 64; ++s) {
+      const uint32_t i = lane + 64 * s;
+      if (i < count && my_rank[s] < take_n) {
+        const uint32_t at = faces + my_rank[s];
+        const float4 yf = L.y[i];
+        const double yn = 1.0 / sqrt(double(yf.x) * yf.x + double(yf.y) * yf.y + double(yf.z) * yf.z + double(yf.w) * yf.w);
+        const double y[4] = {yf.x * yn, yf.y * yn, yf.z * yn, yf.w * yn};
+        for (int k = 0; k < 3; ++k) L.pl[at][k] = y[0] * e[k][0] + y[1] * e[k][1] + y[2] * e[k][2] + y[3] * e[k][3];
+        L.pl[at][3] = 1.0 - (y[0] * p[0] + y[1] * p[1] + y[2] * p[2] + y[3] * p[3]);
+        L.pd2[at] = L.d2[i];
+      }
+    }
+    __syncthreads();
+    // Bound, prune the window, then the cell with its faces marked.
+    const uint32_t total = faces + take_n;
+    cell = cover_cell(L, min(total, faces + kCoverFirst), false);
+    const double limit = 2.0 * atan(sqrt(fmax(cell.r2, 0.0))) * (1.0 + 1e-5) + 1e-7;
+    uint32_t keep = 0;
+    for (uint32_t i = faces + lane; i < total; i += 64) keep += angle_of_chord2(L.pd2[i]) < limit ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) keep += uint32_t(__shfl_xor(int(keep), off));
+    keep += faces;
+    for (uint32_t i = lane; i < keep; i += 64) L.face[i] = 0;
+    __syncthreads();
+    cell = cover_cell(L, keep, true);
+    __syncthreads();
+    const double theta = atan(sqrt(fmax(cell.r2, 0.0)));
+    const double reach = angle_of_chord2(hi2);
+    // Keep only the faces (stable compaction; a chunk is read before it is written).
+    uint32_t out = 0;
+    for (uint32_t base = 0; base < keep; base += 64) {
+      const uint32_t i = base + lane;
+      const bool f = i < keep && L.face[i] != 0;
+      double v[4] = {0, 0, 0, 0};
+      float d = 0.f;
+      if (f) {
+        for (int k = 0; k < 4; ++k) v[k] = L.pl[i][k];
+        d = L.pd2[i];
+      }
+      const uint64_t mask = __ballot(f);
+      const uint32_t pos = out + mbcnt(mask);
+      __syncthreads();
+      if (f) {
+        for (int k = 0; k < 4; ++k) L.pl[pos][k] = v[k];
+        L.pd2[pos] = d;
+      }
+      out += uint32_t(__popcll(mask));
+      __syncthreads();
+    }
+    faces = out;
+    nfaces = out;
+    if (2.0 * theta < reach * (1.0 - 1e-6) - 1e-9) {
+      flag = 0;
+      break;
+    }
+    if (reach >= double(a.c_max) * (1.0 - 1e-6)) break;
+    lo2 = hi2;
+    hi = fminf(a.c_max, fmaxf(1.5f * float(reach), float(2.1 * theta)));
+  }
+  if (lane == 0) {
+    const double theta = atan(sqrt(fmax(cell.r2, 0.0)));
+    a.theta[src] = float(theta);
+    double v[4];
+    for (int k = 0; k < 4; ++k) v[k] = p[k] + cell.t[0] * e[0][k] + cell.t[1] * e[1][k] + cell.t[2] * e[2][k];
+    const double vn = 1.0 / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+    a.witness[src] = make_float4(float(v[0] * vn), float(v[1] * vn), float(v[2] * vn), float(v[3] * vn));
+    a.flags[src] = flag;
+    a.faces[src] = float(nfaces);
+  }
+}
+
+void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
+                  const NearBuffers* grid, const NearCut& cut, float c_start, float c_max, float* theta,
+                  float4* witness, uint32_t* flags, float* faces) {
+  CoverArgs a{};
+  a.q = q;
+  a.n = n;
+  a.ops = ops;
+  a.order = order;
+  a.grid = grid ? 1u : 0u;
+  if (grid) a.b = *grid;
+  a.cut = cut;
+  a.c_start = c_start;
+  a.c_max = c_max;
+  a.theta = theta;
+  a.witness = witness;
+  a.flags = flags;
+  a.faces = faces;
+  p.run("cover", [&] { hipLaunchKernelGGL(k_cover, dim3(n), dim3(64), 0, s, a); });
+}
+
 // Nearest-neighbour angles in SO(3) degrees, 4 asin(min(1, sqrt(c)/2)) 180/pi in FP32
 // (nearest_degrees, pairs.wgsl:428-434), and their minimum (positive floats order as their
 // bits; 0xffffffff marks an empty set) into *least.
