@@ -75,8 +75,8 @@ void zo_cancel(zxt_engine* e);
  * 2 (Voronoi faces); reused while the sample is unchanged.  Errors: "Covering radius:
  * TooFewOrientations" (fewer than 4), "...: CoveringDomainNotEnclosed". */
 int32_t zo_cover(zxt_engine* e);
-/* Out of this build's path (covering optimisation, Delaunay mesh, exact counts), present so
- * the reference's loader resolves every name (src/exports.zig:88-120).  They return -1 (or
+/* Out of this build's path (covering optimisation, Delaunay mesh), present so the
+ * reference's loader resolves every name (src/exports.zig:88-120).  They return -1 (or
  * 0/NULL) with a "NotImplemented" message. */
 int32_t zo_cover_optimize(zxt_engine* e, uint32_t steps, uint32_t mode, uint32_t finish, float floor_deg);
 int32_t zo_cover_kicks(zxt_engine* e, uint32_t patience, float progress, float size, float trust);
@@ -85,6 +85,10 @@ const float* zo_mesh_vertices(zxt_engine* e);
 const uint32_t* zo_mesh_tets(zxt_engine* e);
 uint32_t zo_mesh_vertex_count(zxt_engine* e);
 uint32_t zo_mesh_tet_count(zxt_engine* e);
+/* src/exports.zig:16-22 (engine.zig:199-232): exactly `target` (32..1,000,000) orientations,
+ * the largest base grid of `kind` (flags bits 8-15) that fits and then centres of the largest
+ * empty caps; flags bit 0 adds the identity.  Errors: "Exact-count generation:
+ * ExactCountRange32To1000000", "...: BaseGridExceedsTarget", "...: InsufficientDelaunayCentres". */
 int32_t zo_generate_exact(zxt_engine* e, uint32_t group, uint32_t target, uint32_t flags);
 
 /* Development hooks (src/native_api.zig:31-43). */

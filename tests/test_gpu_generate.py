@@ -59,9 +59,9 @@ def test_fixture_grids_fold_like_the_oracle(operator_fixture):
 def test_counts_and_selection(group):
     note(f"generate: counts {group}")
     sizes = {f: count(group, f) for f in range(1, 513) if count(group, f) <= 10_000_000}
-    with zo.generate(group, count=1000) as grid:
-        assert grid.stats["grid"] == "hopf"
-        assert grid.stats["h"] == zo.hopf_frequency(group, 1000)
+    with zo.generate(group, count=1000, exact=True, identity=True) as grid:
+        assert len(grid) == 1000 and grid.stats["grid"] == "hopf"
+        assert grid.stats["base_count"] == max(n for n in sizes.values() if n <= 999)
         for f in [f for f, n in sizes.items() if n <= 1100]:
             regenerate(grid, group, h=f)
             assert len(grid) == count(group, f) and grid.stats["h"] == f

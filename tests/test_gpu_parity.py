@@ -329,8 +329,9 @@ def test_error_paths():
     with zo.from_quaternions(np.array([[1, 0, 0, 0]], np.float32), "O") as grid:
         with pytest.raises(RuntimeError, match="TooFewOrientations"):
             grid.relax(1)
-    with pytest.raises(RuntimeError, match="NotImplemented"):
-        zo.generate("O", count=100, exact=True)
+    with zo.haar("O", 200, 3) as grid:
+        with pytest.raises(RuntimeError, match="NotImplemented"):
+            grid.optimize_covering(10)
 
 
 def test_fixed_identity_stays_exact(operator_fixture):

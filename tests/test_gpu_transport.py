@@ -116,8 +116,8 @@ def test_polyhedral_only():
         zo.generate("D4", kind="transport")
     with pytest.raises(ValueError):
         zo.generate("O", kind="lattice")
-    with pytest.raises(RuntimeError, match="NotImplemented"):
-        zo.generate("O", count=1000, exact=True, kind="transport")
+    with zo.generate("O", count=1000, exact=True, kind="transport") as grid:
+        assert len(grid) == 1000 and grid.stats["grid"] == "transport"
     with zo.haar("O", 10, 1) as grid:
         with pytest.raises(RuntimeError, match="TransportNeedsTOrOOrI"):
             regenerate(grid, "D4", 3)

@@ -2,9 +2,9 @@
 
 Mirrors the reference's Python adapter (python/zxtalori/api.py): the same names,
 argument meanings, error behaviour (RuntimeError carrying zo_error's text) and JSON stats.
-The QVM relaxation and spacing path, inverse Hopf and FCC transport generation and the
-covering radius run here; exact-count generation, covering optimisation and the Delaunay
-mesh raise RuntimeError("...: NotImplemented").
+The QVM relaxation and spacing path, inverse Hopf and FCC transport generation (also with
+exact counts) and the covering radius run here; covering optimisation and the Delaunay mesh
+raise RuntimeError("...: NotImplemented").
 """
 from __future__ import annotations
 
@@ -120,8 +120,8 @@ class Grid:
 
     Quaternions have shape (N,4), scalar first, and right action q*g.  Construct samples
     with haar(), from_quaternions() or generate() (inverse Hopf product grids, or the FCC
-    transports of T, O and I with kind="transport"; exact=True raises RuntimeError
-    "...: NotImplemented").
+    transports of T, O and I with kind="transport"; exact=True fills the base grid up to
+    the count with centres of its largest empty caps).
     """
 
     def __init__(self, group="O", *, h=None, count=2048, quaternions=None, exact=False, identity=False,

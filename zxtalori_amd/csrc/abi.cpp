@@ -183,12 +183,18 @@ const float* zo_mesh_vertices(zxt_engine*) { return nullptr; }
 const uint32_t* zo_mesh_tets(zxt_engine*) { return nullptr; }
 uint32_t zo_mesh_vertex_count(zxt_engine*) { return 0; }
 uint32_t zo_mesh_tet_count(zxt_engine*) { return 0; }
-int32_t zo_generate_exact(zxt_engine* e, uint32_t group, uint32_t, uint32_t) {
+int32_t zo_generate_exact(zxt_engine* e, uint32_t group, uint32_t target, uint32_t flags) {
   if (!valid_group(group)) {
     e->engine.err = "Unknown group";
     return -1;
   }
-  return not_implemented(e, "Exact-count generation");
+  const uint32_t kind = (flags >> 8) & 0xffu;
+  if (kind > 2) {
+    e->engine.err = "Unknown grid kind";
+    return -1;
+  }
+  return guarded(e, "Exact-count generation",
+                 [&] { e->engine.generate_exact(group, target, (flags & 1u) != 0, kind); });
 }
 
 bool zo_profile(zxt_engine* e, bool enabled) {

@@ -9,9 +9,11 @@
 #include <shared_mutex>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <unordered_map>
 
 #include "groups_table.h"
 #include "hopf.h"
@@ -441,6 +443,7 @@ void Engine::set(uint32_t group, const float* values, size_t count4) {
     throw;
   }
   generator_ = "Imported quaternions";
+  inserted_count_ = 0;
   grid_kind_.clear();
   h_ = 0;
   seed_ = 0;
@@ -462,6 +465,7 @@ void Engine::generate_random(uint32_t group, uint32_t count, uint32_t seed) {
   }
   grid_kind_.clear();
   h_ = 0;
+  inserted_count_ = 0;
   generator_ = "Haar random";
   seed_ = seed;
   base_count_ = count;
@@ -489,6 +493,7 @@ void Engine::generate(uint32_t group, uint32_t h, uint32_t target, uint32_t seed
     throw;
   }
   generator_ = hopf::description(group);
+  inserted_count_ = 0;
   grid_kind_ = "hopf";
   h_ = f;
   seed_ = seed;
@@ -518,6 +523,7 @@ void Engine::generate_transport(uint32_t group, uint32_t h, uint32_t seed) {
     throw;
   }
   generator_ = transport::description(group);
+  inserted_count_ = 0;
   grid_kind_ = "transport";
   h_ = h;
   seed_ = seed;
@@ -813,13 +819,14 @@ void Engine::identity(bool add, bool fixed) {
   long index = identity_index();
   if (add && index < 0) {
     const std::string generator = generator_, grid = grid_kind_;
-    const uint32_t base = base_count_, seed = seed_, h = h_;
+    const uint32_t base = base_count_, seed = seed_, h = h_, inserted = inserted_count_;
     std::vector<float> values(host_.begin(), host_.end());
     values.insert(values.end(), {1.0f, 0.0f, 0.0f, 0.0f});
     set(group_, values.data(), values.size());
     generator_ = generator;
     grid_kind_ = grid;
     h_ = h;
+    inserted_count_ = inserted;
     base_count_ = base;
     seed_ = seed;
     index = long(n_) - 1;
@@ -879,6 +886,11 @@ void Engine::cover() {
     cover_.cached = true;
     return;
   }
+  cover_measure(false);
+}
+
+void Engine::cover_measure(bool vertices) {
+  if (n_ < 4) fail("TooFewOrientations");
   const uint32_t ord = order();
   const double density = double(n_) * ord;
   const double spacing = std::cbrt(kPi * kPi / density);  // S3 angle per cloud point
@@ -890,6 +902,12 @@ void Engine::cover() {
   cover_witness_.ensure(size_t(n_) * 16);
   cover_flags_.ensure(size_t(n_) * 4);
   cover_faces_.ensure(size_t(n_) * 4);
+  const uint32_t vcap = vertices ? uint32_t(std::min<uint64_t>(uint64_t(n_) * 64, 1ull << 26)) : 0;
+  if (vertices) {
+    cover_vq_.ensure(size_t(vcap) * 16);
+    cover_vr_.ensure(size_t(vcap) * 4);
+    cover_vcount_.ensure(4);
+  }
   float4* q = q_.as<float4>();
   uint32_t rounds = 0, rebuilt = 0;
   while (rounds < 8) {
@@ -909,8 +927,11 @@ void Engine::cover() {
       grid = &view;
       cut = cover_near_.cut;
     }
+    if (vertices) launch_fill_u32(prof, stream, cover_vcount_.as<uint32_t>(), 0u, 1);
     launch_cover(prof, stream, q, n_, ops_device(), ord, grid, cut, c_start, float(c_max), cover_theta_.as<float>(),
-                 cover_witness_.as<float4>(), cover_flags_.as<uint32_t>(), cover_faces_.as<float>());
+                 cover_witness_.as<float4>(), cover_flags_.as<uint32_t>(), cover_faces_.as<float>(),
+                 vertices ? cover_vq_.as<float4>() : nullptr, vertices ? cover_vr_.as<float>() : nullptr,
+                 vertices ? cover_vcount_.as<uint32_t>() : nullptr, vcap);
     download(point_covering_, cover_theta_, n_);
     download(point_faces_, cover_faces_, n_);
     download(cover_flags_host_, cover_flags_, n_);
@@ -945,9 +966,229 @@ void Engine::cover() {
     for (uint32_t i = 0; i < n_; ++i) point_covering_[i] = float(double(point_covering_[i]) * 360.0 / kPi);
     cover_ = r;
     point_covering_iteration_ = long(iteration_);
+    if (vertices) {
+      uint32_t count = 0;
+      check(hipMemcpy(&count, cover_vcount_.p, 4, hipMemcpyDeviceToHost), "download");
+      count = std::min(count, vcap);
+      download(cover_vq_host_, cover_vq_, count);
+      download(cover_vr_host_, cover_vr_, count);
+      sync();
+    }
     return;
   }
   fail("CoveringRetryLimit");
+}
+
+// ---- exact counts (engine.zig:199-232, insert.zig, dedup.zig) --------------------------
+namespace {
+
+using Q4 = std::array<double, 4>;
+
+Q4 mul4(const Q4& a, const Q4& b) {
+  return {a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3], a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2],
+          a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1], a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0]};
+}
+double dot4d(const Q4& a, const Q4& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; }
+Q4 unit4(Q4 q) {
+  const double s = 1.0 / std::sqrt(dot4d(q, q));
+  for (double& x : q) x *= s;
+  return q;
+}
+
+// Points of S3 hashed by their vector part on a uniform grid; both signs of every point.
+class SphereHash {
+ public:
+  explicit SphereHash(double cell) : cell_(std::max(cell, 1e-9)) {}
+  void add(const Q4& q) {
+    for (double sign : {1.0, -1.0}) {
+      const Q4 p{sign * q[0], sign * q[1], sign * q[2], sign * q[3]};
+      cells_[key(p[1], p[2], p[3])].push_back(uint32_t(points_.size()));
+      points_.push_back(p);
+    }
+  }
+  // Whether some stored point has dot(q, p) > threshold.
+  bool near(const Q4& q, double threshold) const {
+    const double chord = std::sqrt(std::max(0.0, 2.0 - 2.0 * threshold)) * (1 + 1e-9) + 1e-12;
+    const auto lo = key(q[1] - chord, q[2] - chord, q[3] - chord), hi = key(q[1] + chord, q[2] + chord, q[3] + chord);
+    for (int64_t x = lo[0]; x <= hi[0]; ++x)
+      for (int64_t y = lo[1]; y <= hi[1]; ++y)
+        for (int64_t z = lo[2]; z <= hi[2]; ++z) {
+          auto it = cells_.find({x, y, z});
+          if (it == cells_.end()) continue;
+          for (uint32_t i : it->second)
+            if (dot4d(q, points_[i]) > threshold) return true;
+        }
+    return false;
+  }
+  // Index of a stored point within `tolerance` (4D distance) of q, or -1.
+  long within(const Q4& q, double tolerance) const {
+    const auto lo = key(q[1] - tolerance, q[2] - tolerance, q[3] - tolerance);
+    const auto hi = key(q[1] + tolerance, q[2] + tolerance, q[3] + tolerance);
+    for (int64_t x = lo[0]; x <= hi[0]; ++x)
+      for (int64_t y = lo[1]; y <= hi[1]; ++y)
+        for (int64_t z = lo[2]; z <= hi[2]; ++z) {
+          auto it = cells_.find({x, y, z});
+          if (it == cells_.end()) continue;
+          for (uint32_t i : it->second) {
+            double d = 0.0;
+            for (int k = 0; k < 4; ++k) d += (points_[i][k] - q[k]) * (points_[i][k] - q[k]);
+            if (d < tolerance * tolerance) return long(i);
+          }
+        }
+    return -1;
+  }
+
+ private:
+  using Key = std::array<int64_t, 3>;
+  struct KeyHash {
+    size_t operator()(const Key& k) const {
+      return size_t(uint64_t(k[0]) * 0x9E3779B97F4A7C15ull ^ uint64_t(k[1]) * 0xC2B2AE3D27D4EB4Full ^ uint64_t(k[2]));
+    }
+  };
+  Key key(double x, double y, double z) const {
+    return {int64_t(std::floor(x / cell_)), int64_t(std::floor(y / cell_)), int64_t(std::floor(z / cell_))};
+  }
+  double cell_;
+  std::vector<Q4> points_;
+  std::unordered_map<Key, std::vector<uint32_t>, KeyHash> cells_;
+};
+
+}  // namespace
+
+// Up to `wanted` centres of the largest empty caps, greedily (insert.zig:150-240): every
+// Voronoi vertex of the sample (k_cover), carried into the zone, largest first; a centre is
+// taken when its cap holds no orbit image of the sample or of a centre already taken (3e-7
+// of slack on the cap's height), and an identical centre only once.
+std::vector<float> Engine::cap_centres(uint32_t wanted) {
+  cover_measure(true);
+  const uint32_t ord = order();
+  std::vector<Q4> ops(ord);
+  for (uint32_t k = 0; k < ord; ++k) {
+    const float* o = kOperators[kGroupOffset[group_] + k];
+    ops[k] = unit4({o[0], o[1], o[2], o[3]});
+  }
+  struct Candidate {
+    Q4 q;
+    double radius;
+  };
+  std::vector<Candidate> cands;
+  cands.reserve(cover_vq_host_.size());
+  double largest = 0.0;
+  for (size_t i = 0; i < cover_vq_host_.size(); ++i) {
+    const float4 v = cover_vq_host_[i];
+    const Q4 c = unit4({v.x, v.y, v.z, v.w});
+    // The zone's image: c g^-1 with the largest |scalar part| = |c . g|.
+    uint32_t best = 0;
+    double top = -1.0;
+    for (uint32_t k = 0; k < ord; ++k) {
+      const double w = std::fabs(dot4d(c, ops[k]));
+      if (w > top) {
+        top = w;
+        best = k;
+      }
+    }
+    const Q4& g = ops[best];
+    Q4 r = mul4(c, {g[0], -g[1], -g[2], -g[3]});
+    if (r[0] < 0)
+      for (double& x : r) x = -x;
+    cands.push_back({unit4(r), double(cover_vr_host_[i])});
+    largest = std::max(largest, double(cover_vr_host_[i]));
+  }
+  std::stable_sort(cands.begin(), cands.end(), [](const Candidate& a, const Candidate& b) { return a.radius > b.radius; });
+  SphereHash occupied(2.0 * std::sin(0.5 * largest));
+  for (uint32_t i = 0; i < n_; ++i) occupied.add(unit4({host_[4 * i], host_[4 * i + 1], host_[4 * i + 2], host_[4 * i + 3]}));
+  std::vector<float> out;
+  for (const Candidate& c : cands) {
+    if (out.size() / 4 >= wanted) break;
+    if (occupied.near(c.q, 1.0 - 1e-12)) continue;  // the same centre from another cell
+    const double height = std::cos(c.radius);
+    bool empty = true;
+    for (const Q4& g : ops) {
+      if (occupied.near(mul4(c.q, {g[0], -g[1], -g[2], -g[3]}), height + 3e-7)) {
+        empty = false;
+        break;
+      }
+    }
+    if (!empty) continue;
+    occupied.add(c.q);
+    for (double x : c.q) out.push_back(float(x));
+  }
+  if (out.empty()) fail("NoEmptyDelaunayCentres");
+  return out;
+}
+
+// Keeps the first of every group of rows equal up to the orbit (2e-6), in place; the first
+// `unique` rows are known distinct (dedup.zig:5-60).  Returns the count kept.
+uint32_t Engine::orbit_compact(std::vector<float>& q, uint32_t unique) const {
+  const double tolerance = 2e-6;
+  const uint32_t ord = order();
+  std::vector<Q4> ops(ord);
+  for (uint32_t k = 0; k < ord; ++k) {
+    const float* o = kOperators[kGroupOffset[group_] + k];
+    ops[k] = {o[0], o[1], o[2], o[3]};
+  }
+  SphereHash kept(2.0 * tolerance);
+  uint32_t count = 0;
+  const uint32_t rows = uint32_t(q.size() / 4);
+  for (uint32_t i = 0; i < rows; ++i) {
+    const Q4 src{q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
+    bool duplicate = false;
+    if (i >= unique) {
+      for (const Q4& g : ops) {
+        if (kept.within(unit4(mul4(src, g)), tolerance) >= 0) {
+          duplicate = true;
+          break;
+        }
+      }
+    }
+    if (duplicate) continue;
+    for (int k = 0; k < 4; ++k) q[4 * count + k] = float(src[k]);
+    kept.add(src);
+    ++count;
+  }
+  return count;
+}
+
+void Engine::generate_exact(uint32_t group, uint32_t target, bool add_identity, uint32_t kind) {
+  if (target < 32 || target > 1000000) fail("ExactCountRange32To1000000");
+  const uint32_t budget = target - (add_identity ? 1u : 0u);
+  uint32_t h = 1;
+  if (kind == 2) {
+    if (group < 8 || group > 10) fail("TransportNeedsTOrOOrI");
+    while (h < 64 && transport::count(group, h + 1) <= budget) ++h;
+  } else {
+    uint32_t best = 0;
+    uint64_t most = 0;
+    for (uint32_t f = 1; f <= hopf::kMaxFrequency; ++f) {
+      const uint64_t n = hopf::raw_count(group, f);
+      if (n <= budget && n > most) {
+        most = n;
+        best = f;
+      }
+    }
+    if (!best) fail("BaseGridExceedsTarget");
+    h = best;
+  }
+  generate(group, h, target, 0, kind);
+  const uint32_t base = n_;
+  if (add_identity) identity(true, fix_identity_);
+  const std::string generator = generator_, grid = grid_kind_;
+  for (int round = 0; round < 8 && n_ != target; ++round) {
+    if (n_ > target) fail("BaseGridExceedsTarget");
+    const std::vector<float> additions = cap_centres(target - n_);
+    std::vector<float> values(host_.begin(), host_.end());
+    values.insert(values.end(), additions.begin(), additions.end());
+    const uint32_t keep = orbit_compact(values, n_);
+    if (keep <= n_) fail("NoDistinctDelaunayCentres");
+    values.resize(size_t(keep) * 4);
+    set(group, values.data(), values.size());
+  }
+  if (n_ != target) fail("InsufficientDelaunayCentres");
+  generator_ = generator;
+  grid_kind_ = grid;
+  h_ = h;
+  base_count_ = base;
+  inserted_count_ = target - base;
 }
 
 const uint32_t* Engine::point_indices() const {
@@ -1036,7 +1277,7 @@ std::string Engine::stats() {
   else
     j.null("identity_index");
   j.integer("base_count", base_count_);
-  j.integer("inserted_count", 0);
+  j.integer("inserted_count", inserted_count_);
   j.str("triangulation_backend", "hip");
   // Simplex bound (SO3K_Transports so3kt/measures/reference.py; covering_reference.zig:4-15):
   // sum_i c_i count^-(2i+1)/3 in SO(3) degrees, count = 2 |K| N.

@@ -174,6 +174,9 @@ class Engine {
   // Covering radius of the sample (engine.zig:378-398, covering.zig:555-577): measured
   // again only when the sample changed since the last measurement.
   void cover();
+  // Exactly `target` orientations (engine.zig:199-232): the largest base grid that fits,
+  // then centres of the largest empty caps until the count is reached.
+  void generate_exact(uint32_t group, uint32_t target, bool add_identity, uint32_t kind);
   void measure();
   void spacing();
   void relax(uint32_t steps);
@@ -286,6 +289,14 @@ class Engine {
   PinnedVec<float> point_covering_, point_faces_;
   PinnedVec<uint32_t> cover_flags_host_;
   long point_covering_iteration_ = -1;
+  DevBuf cover_vq_, cover_vr_, cover_vcount_;
+  PinnedVec<float4> cover_vq_host_;
+  PinnedVec<float> cover_vr_host_;
+  uint32_t inserted_count_ = 0;
+  // One covering measurement (with every certified cell's vertices when asked).
+  void cover_measure(bool vertices);
+  std::vector<float> cap_centres(uint32_t wanted);
+  uint32_t orbit_compact(std::vector<float>& q, uint32_t unique) const;
   PinnedVec<uint32_t> point_index_;
   long point_separation_iteration_ = -1;
   long point_index_iteration_ = -1;

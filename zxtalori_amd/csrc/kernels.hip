@@ -1594,6 +1594,8 @@ constexpr uint32_t kCoverPlanes = 320;  // faces kept plus one window
 constexpr uint32_t kCoverFirst = 32;
 constexpr double kCoverBox = 8.0;       // |t_k| <= 8: cells within ~82 degrees (S3) of p
 
+constexpr uint32_t kCoverVertices = 192;  // vertices of the last cell kept for output
+
 struct CoverLds {
   float4 y[kCoverGather];
   float d2[kCoverGather];
@@ -1601,6 +1603,8 @@ struct CoverLds {
   float pd2[kCoverPlanes];     // chord^2 of the plane's neighbour
   uint8_t face[kCoverPlanes];  // the plane bounds the cell (some vertex lies on it)
   float cut;
+  double vt[kCoverVertices][4];  // the cell's vertices (t, |t|^2), with `faces`
+  uint32_t nv;
 };
 
 struct CoverVertex {
@@ -1676,6 +1680,13 @@ __device__ CoverVertex cover_cell(CoverLds& L, uint32_t m, bool faces) {
         if (a < m) L.face[a] = 1;
         if (b < m) L.face[b] = 1;
         if (c < m) L.face[c] = 1;
+        const uint32_t at = atomicAdd(&L.nv, 1u);
+        if (at < kCoverVertices) {
+          L.vt[at][0] = t[0];
+          L.vt[at][1] = t[1];
+          L.vt[at][2] = t[2];
+          L.vt[at][3] = r2;
+        }
       }
     }
   }
@@ -1709,6 +1720,12 @@ struct CoverArgs {
   float4* witness;
   uint32_t* flags;    // 0 certified, 1 needs a cutoff beyond c_max, 2 too many neighbours
   float* faces;       // faces of the cell
+  // Optional: every vertex of every certified cell (direction with w >= 0, and its S3
+  // radius), appended up to vcap; *vcount counts all of them.
+  float4* vq;
+  float* vr;
+  uint32_t* vcount;
+  uint32_t vcap;
 };
 
 __device__ inline float chord2_of_angle(float a) {
@@ -1889,6 +1906,7 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
     for (int off = 32; off > 0; off >>= 1) keep += uint32_t(__shfl_xor(int(keep), off));
     keep += faces;
     for (uint32_t i = lane; i < keep; i += 64) L.face[i] = 0;
+    if (lane == 0) L.nv = 0;
     __syncthreads();
     cell = cover_cell(L, keep, true);
     __syncthreads();
@@ -1925,6 +1943,22 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
     lo2 = hi2;
     hi = fminf(a.c_max, fmaxf(1.5f * float(reach), float(2.1 * theta)));
   }
+  if (a.vq && flag == 0) {
+    // The certified cell's vertices (its last pass listed them; a cell with more than
+    // kCoverVertices lists the first found).
+    const uint32_t nv = min(L.nv, kCoverVertices);
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(a.vcount, nv);
+    base = uint32_t(__shfl(int(base), 0));
+    for (uint32_t i = lane; i < nv; i += 64) {
+      if (base + i >= a.vcap) break;
+      double v[4];
+      for (int k = 0; k < 4; ++k) v[k] = p[k] + L.vt[i][0] * e[0][k] + L.vt[i][1] * e[1][k] + L.vt[i][2] * e[2][k];
+      const double vn = (v[0] < 0.0 ? -1.0 : 1.0) / sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+      a.vq[base + i] = make_float4(float(v[0] * vn), float(v[1] * vn), float(v[2] * vn), float(v[3] * vn));
+      a.vr[base + i] = float(atan(sqrt(fmax(L.vt[i][3], 0.0))));
+    }
+  }
   if (lane == 0) {
     const double theta = atan(sqrt(fmax(cell.r2, 0.0)));
     a.theta[src] = float(theta);
@@ -1939,7 +1973,8 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
 
 void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                   const NearBuffers* grid, const NearCut& cut, float c_start, float c_max, float* theta,
-                  float4* witness, uint32_t* flags, float* faces) {
+                  float4* witness, uint32_t* flags, float* faces, float4* vq, float* vr, uint32_t* vcount,
+                  uint32_t vcap) {
   CoverArgs a{};
   a.q = q;
   a.n = n;
@@ -1954,6 +1989,10 @@ void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const flo
   a.witness = witness;
   a.flags = flags;
   a.faces = faces;
+  a.vq = vq;
+  a.vr = vr;
+  a.vcount = vcount;
+  a.vcap = vcap;
   p.run("cover", [&] { hipLaunchKernelGGL(k_cover, dim3(n), dim3(64), 0, s, a); });
 }
 
