@@ -236,6 +236,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--serving-requests", type=int, default=40, help="0 skips the serving run")
     ap.add_argument("--serving-rate", type=float, default=4.0)
+    ap.add_argument("--no-widened", action="store_true",
+                    help="skip the generation, covering and exact-count measurements")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (the HBM counter calibration kernel)")
     args = ap.parse_args()
@@ -319,7 +321,75 @@ def main():
         out["batched"] = batched_section(args.batched_jobs, args.batched_steps)
     if args.serving_requests > 0:
         out["serving"] = serving_section(args.serving_requests, args.serving_rate)
+    if not args.no_widened:
+        out["widened"] = widened_section()
     print(json.dumps(out), flush=True)
+
+
+def _timed(grid, call):
+    """Wall time of call() and the per-kernel device times (HIP events) it recorded."""
+    grid.profile(True)
+    before = grid.profile_report()
+    t0 = time.perf_counter()
+    call()
+    dt = time.perf_counter() - t0
+    after = grid.profile_report()
+    grid.profile(False)
+    kernels = {}
+    for name, (count, ms) in after.items():
+        c0, m0 = before.get(name, (0, 0.0))
+        if count > c0:
+            kernels[name] = round(ms - m0, 4)
+    return dt * 1e3, kernels
+
+
+def widened_section():
+    """The rows next to the hot path, one workload each, timed after a warm-up call:
+    inverse Hopf generation (O, 1e7 points: host FP64 construction, upload, GPU fold), the
+    FCC transport (T, h = 64: 1.43e6 points, FP64 map per point on the GPU), the covering
+    radius (O, 5e5 points = 1.2e7 images: per-point Voronoi cells in FP64 over the site
+    grid) and an exact count (O, 1e5 points: base grid plus empty-cap insertion).  The fold
+    is HBM-bound (32 B per point); k_transport and k_cover are FP64-VALU-bound."""
+    import zxtalori_amd as zo
+    out = {}
+    print("[bench] widened: generation, transport, covering, exact", file=sys.stderr, flush=True)
+    with zo.haar("O", 8, 1) as grid:
+        lib, handle = grid._lib, grid._handle
+
+        def generate(group_id, h, target, kind):
+            grid._check(lib.zo_generate(handle, group_id, h, target, 0, kind))
+
+        generate(9, 0, 10_000_000, 0)
+        ms, kernels = _timed(grid, lambda: generate(9, 0, 10_000_000, 0))
+        n = len(grid)
+        fold_ms = kernels.get("fold", float("nan"))
+        out["hopf"] = {"workload": "zo.generate('O', count=1e7): h = 281, inverse Hopf grid", "n": n, "ms": ms,
+                       "points_per_s": n / (ms * 1e-3), "fold_ms": fold_ms,
+                       "fold_roofline": {"bound": "hbm", "achieved": 32 * n / (fold_ms * 1e-3) / 1e9, "peak": 8000.0,
+                                         "unit": "GB/s", "frac": 32 * n / (fold_ms * 1e-3) / 8e12}}
+        generate(8, 64, 2048, 2)
+        ms, kernels = _timed(grid, lambda: generate(8, 64, 2048, 2))
+        n = len(grid)
+        out["transport"] = {"workload": "zo.generate('T', h=64, kind='transport')", "n": n, "ms": ms,
+                            "kernel_ms": kernels.get("transport", float("nan")),
+                            "points_per_s_device": n / (kernels.get("transport", float("nan")) * 1e-3),
+                            "bound": "fp64 valu"}
+    with zo.generate("O", count=500_000) as grid:
+        grid.covering()
+        grid.set(grid.quaternions, "O")  # a new sample: the next call measures again
+        ms, kernels = _timed(grid, grid.covering)
+        stats = grid.stats
+        out["covering"] = {"workload": "zo.generate('O', count=5e5).covering(), after a warm-up on the same points",
+                           "n": len(grid),
+                           "images": 2 * 24 * len(grid), "ms": ms, "kernels_ms": kernels,
+                           "cells_per_s_device": len(grid) / (kernels.get("cover", float("nan")) * 1e-3),
+                           "covering_radius_deg": stats["covering_radius_deg"], "bound": "fp64 valu"}
+    t0 = time.perf_counter()
+    with zo.generate("O", count=100_000, exact=True) as grid:
+        dt = time.perf_counter() - t0
+        out["exact"] = {"workload": "zo.generate('O', count=1e5, exact=True)", "n": len(grid), "ms": dt * 1e3,
+                        "base_count": grid.stats["base_count"], "inserted_count": grid.stats["inserted_count"]}
+    return out
 
 
 def spacing_section(n):
