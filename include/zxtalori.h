@@ -75,11 +75,16 @@ void zo_cancel(zxt_engine* e);
  * 2 (Voronoi faces); reused while the sample is unchanged.  Errors: "Covering radius:
  * TooFewOrientations" (fewer than 4), "...: CoveringDomainNotEnclosed". */
 int32_t zo_cover(zxt_engine* e);
-/* Out of this build's path (covering optimisation, Delaunay mesh), present so the
- * reference's loader resolves every name (src/exports.zig:88-120).  They return -1 (or
- * 0/NULL) with a "NotImplemented" message. */
+/* Out of this build's path (covering optimisation), present so the reference's loader
+ * resolves every name (src/exports.zig:104-120).  They return -1 with a "NotImplemented"
+ * message. */
 int32_t zo_cover_optimize(zxt_engine* e, uint32_t steps, uint32_t mode, uint32_t finish, float floor_deg);
 int32_t zo_cover_kicks(zxt_engine* e, uint32_t patience, float progress, float size, float trust);
+/* src/exports.zig:88-103 (engine.zig:352-358): a covering measurement that also keeps the
+ * Delaunay tetrahedra around the sample's orbit points: vertices as stereographic
+ * (x, y, z, 0) of the sample (first, in order) and of the other orbit images used, tetrahedra
+ * as 4 vertex indices; valid until the next mutating call.  Errors as zo_cover, prefixed
+ * "DT display". */
 int32_t zo_mesh(zxt_engine* e);
 const float* zo_mesh_vertices(zxt_engine* e);
 const uint32_t* zo_mesh_tets(zxt_engine* e);

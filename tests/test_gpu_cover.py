@@ -130,3 +130,32 @@ def test_too_few_orientations():
     with zo.from_quaternions(np.eye(4, dtype=np.float32)[:3], "O") as grid:
         with pytest.raises(RuntimeError, match="TooFewOrientations"):
             grid.covering()
+
+
+def test_mesh_export():
+    """tests/test_explorer.py:17-22, and every tetrahedron's circumcap is empty with the
+    largest at the covering radius."""
+    with zo.generate("O", h=2, kind="transport") as grid:
+        mesh = grid.triangulation()
+        p = mesh["stereographic"].astype(np.float64)
+        tets = mesh["tetrahedra"]
+        assert len(p) >= len(grid) and len(tets) > 0 and tets.max() < len(p)
+        assert grid.stats["covering_current"]
+        radius = grid.stats["covering_radius_deg"]
+        # Back to S3: q = ((1 - r^2), 2p) / (1 + r^2).
+        r2 = np.sum(p * p, axis=1, keepdims=True)
+        q = np.concatenate([(1 - r2), 2 * p], axis=1) / (1 + r2)
+        np.testing.assert_allclose(q[:len(grid)], grid.quaternions, atol=2e-6)
+        a = q[tets]                                   # (t, 4, 4)
+        # The circumcap centre c: c . a_i equal for the four points, |c| = 1.
+        e = a[:, 1:] - a[:, :1]                       # (t, 3, 4)
+        c = np.stack([(-1) ** k * np.linalg.det(np.delete(e, k, axis=2)) for k in range(4)], axis=1)
+        c /= np.linalg.norm(c, axis=1, keepdims=True)
+        h = np.einsum("tk,tk->t", c, a[:, 0])
+        c *= np.sign(h)[:, None]
+        caps = 2 * np.degrees(np.arccos(np.clip(np.abs(h), -1, 1)))
+        assert abs(caps.max() - radius) < 1e-3, (caps.max(), radius)
+        cloud = np.concatenate([q, -q])
+        # Empty: no vertex of the mesh strictly inside any cap.
+        inside = (c @ cloud.T) > np.abs(h)[:, None] + 1e-6
+        assert not inside.any()

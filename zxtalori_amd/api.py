@@ -3,8 +3,8 @@
 Mirrors the reference's Python adapter (python/zxtalori/api.py): the same names,
 argument meanings, error behaviour (RuntimeError carrying zo_error's text) and JSON stats.
 The QVM relaxation and spacing path, inverse Hopf and FCC transport generation (also with
-exact counts) and the covering radius run here; covering optimisation and the Delaunay mesh
-raise RuntimeError("...: NotImplemented").
+exact counts), the covering radius and its Delaunay mesh run here; covering optimisation
+raises RuntimeError("...: NotImplemented").
 """
 from __future__ import annotations
 
@@ -269,9 +269,16 @@ class Grid:
         return self.stats
 
     def triangulation(self):
+        """The Delaunay tetrahedra around the sample's orbit points: stereographic xyz of the
+        sample (first, in order) and of the other orbit images that close its cells, and
+        tetrahedron indices.  Measures the covering radius too; the arrays are copies."""
         self._open()
         self._check(self._lib.zo_mesh(self._handle))
-        return None
+        n = self._lib.zo_mesh_vertex_count(self._handle)
+        t = self._lib.zo_mesh_tet_count(self._handle)
+        vertices = np.ctypeslib.as_array(self._lib.zo_mesh_vertices(self._handle), shape=(n, 4))[:, :3].copy()
+        tetrahedra = np.ctypeslib.as_array(self._lib.zo_mesh_tets(self._handle), shape=(t, 4)).copy()
+        return {"stereographic": vertices, "tetrahedra": tetrahedra, "stats": self.stats}
 
     def relax(self, steps=50, *, callback=None, neighbourhood=None):
         """Quey geodesic force relaxation with BB steps; updates this grid.

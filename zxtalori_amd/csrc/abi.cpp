@@ -178,11 +178,13 @@ int32_t zo_cover_optimize(zxt_engine* e, uint32_t, uint32_t, uint32_t, float) {
   return not_implemented(e, "Covering optimization");
 }
 int32_t zo_cover_kicks(zxt_engine* e, uint32_t, float, float, float) { return not_implemented(e, "Covering kicks"); }
-int32_t zo_mesh(zxt_engine* e) { return not_implemented(e, "DT display"); }
-const float* zo_mesh_vertices(zxt_engine*) { return nullptr; }
-const uint32_t* zo_mesh_tets(zxt_engine*) { return nullptr; }
-uint32_t zo_mesh_vertex_count(zxt_engine*) { return 0; }
-uint32_t zo_mesh_tet_count(zxt_engine*) { return 0; }
+int32_t zo_mesh(zxt_engine* e) {
+  return guarded(e, "DT display", [&] { e->engine.build_mesh(); });
+}
+const float* zo_mesh_vertices(zxt_engine* e) { return e->engine.mesh_vertices(); }
+const uint32_t* zo_mesh_tets(zxt_engine* e) { return e->engine.mesh_tets(); }
+uint32_t zo_mesh_vertex_count(zxt_engine* e) { return e->engine.mesh_vertex_count(); }
+uint32_t zo_mesh_tet_count(zxt_engine* e) { return e->engine.mesh_tet_count(); }
 int32_t zo_generate_exact(zxt_engine* e, uint32_t group, uint32_t target, uint32_t flags) {
   if (!valid_group(group)) {
     e->engine.err = "Unknown group";

@@ -17,6 +17,7 @@
 
 #include "groups_table.h"
 #include "hopf.h"
+#include "qvm_device.h"
 #include "transport.h"
 
 namespace zxt {
@@ -886,10 +887,10 @@ void Engine::cover() {
     cover_.cached = true;
     return;
   }
-  cover_measure(false);
+  cover_measure(false, false);
 }
 
-void Engine::cover_measure(bool vertices) {
+void Engine::cover_measure(bool vertices, bool tets) {
   if (n_ < 4) fail("TooFewOrientations");
   const uint32_t ord = order();
   const double density = double(n_) * ord;
@@ -903,11 +904,13 @@ void Engine::cover_measure(bool vertices) {
   cover_flags_.ensure(size_t(n_) * 4);
   cover_faces_.ensure(size_t(n_) * 4);
   const uint32_t vcap = vertices ? uint32_t(std::min<uint64_t>(uint64_t(n_) * 64, 1ull << 26)) : 0;
+  const uint32_t tcap = tets ? uint32_t(std::min<uint64_t>(uint64_t(n_) * 64, 1ull << 25)) : 0;
   if (vertices) {
     cover_vq_.ensure(size_t(vcap) * 16);
     cover_vr_.ensure(size_t(vcap) * 4);
-    cover_vcount_.ensure(4);
   }
+  if (tets) cover_tets_.ensure(size_t(tcap) * 16);
+  if (vertices || tets) cover_vcount_.ensure(8);
   float4* q = q_.as<float4>();
   uint32_t rounds = 0, rebuilt = 0;
   while (rounds < 8) {
@@ -927,11 +930,12 @@ void Engine::cover_measure(bool vertices) {
       grid = &view;
       cut = cover_near_.cut;
     }
-    if (vertices) launch_fill_u32(prof, stream, cover_vcount_.as<uint32_t>(), 0u, 1);
+    if (vertices || tets) launch_fill_u32(prof, stream, cover_vcount_.as<uint32_t>(), 0u, 2);
     launch_cover(prof, stream, q, n_, ops_device(), ord, grid, cut, c_start, float(c_max), cover_theta_.as<float>(),
                  cover_witness_.as<float4>(), cover_flags_.as<uint32_t>(), cover_faces_.as<float>(),
                  vertices ? cover_vq_.as<float4>() : nullptr, vertices ? cover_vr_.as<float>() : nullptr,
-                 vertices ? cover_vcount_.as<uint32_t>() : nullptr, vcap);
+                 vertices ? cover_vcount_.as<uint32_t>() : nullptr, vcap, tets ? cover_tets_.as<uint4>() : nullptr,
+                 tets ? cover_vcount_.as<uint32_t>() + 1 : nullptr, tcap);
     download(point_covering_, cover_theta_, n_);
     download(point_faces_, cover_faces_, n_);
     download(cover_flags_host_, cover_flags_, n_);
@@ -966,12 +970,15 @@ void Engine::cover_measure(bool vertices) {
     for (uint32_t i = 0; i < n_; ++i) point_covering_[i] = float(double(point_covering_[i]) * 360.0 / kPi);
     cover_ = r;
     point_covering_iteration_ = long(iteration_);
-    if (vertices) {
-      uint32_t count = 0;
-      check(hipMemcpy(&count, cover_vcount_.p, 4, hipMemcpyDeviceToHost), "download");
-      count = std::min(count, vcap);
-      download(cover_vq_host_, cover_vq_, count);
-      download(cover_vr_host_, cover_vr_, count);
+    if (vertices || tets) {
+      uint32_t counts[2] = {0, 0};
+      check(hipMemcpy(counts, cover_vcount_.p, 8, hipMemcpyDeviceToHost), "download");
+      if (vertices) {
+        const uint32_t count = std::min(counts[0], vcap);
+        download(cover_vq_host_, cover_vq_, count);
+        download(cover_vr_host_, cover_vr_, count);
+      }
+      if (tets) download(cover_tets_host_, cover_tets_, std::min(counts[1], tcap));
       sync();
     }
     return;
@@ -1060,7 +1067,7 @@ class SphereHash {
 // taken when its cap holds no orbit image of the sample or of a centre already taken (3e-7
 // of slack on the cap's height), and an identical centre only once.
 std::vector<float> Engine::cap_centres(uint32_t wanted) {
-  cover_measure(true);
+  cover_measure(true, false);
   const uint32_t ord = order();
   std::vector<Q4> ops(ord);
   for (uint32_t k = 0; k < ord; ++k) {
@@ -1189,6 +1196,63 @@ void Engine::generate_exact(uint32_t group, uint32_t target, bool add_identity, 
   h_ = h;
   base_count_ = base;
   inserted_count_ = target - base;
+}
+
+// The Delaunay tetrahedra around the sample's orbit points (engine.zig:352-358): a covering
+// measurement that also lists, for every certified cell, the source and the three
+// neighbours whose bisectors meet at each vertex.  Vertices are stereographic (x, y, z)/(1 + w)
+// of the sample's points (first, in order) and of the other orbit images that close its
+// cells; each tetrahedron once.
+void Engine::build_mesh() {
+  if (n_ < 4) fail("TooFewOrientations");
+  cover_measure(false, true);
+  const uint32_t ord = order(), two = 2 * ord;
+  std::vector<std::array<uint32_t, 4>> tets;
+  tets.reserve(cover_tets_host_.size());
+  for (size_t i = 0; i < cover_tets_host_.size(); ++i) {
+    const uint4 t = cover_tets_host_[i];
+    if (t.y == ~0u || t.z == ~0u || t.w == ~0u) continue;
+    std::array<uint32_t, 4> k{t.x, t.y, t.z, t.w};
+    std::sort(k.begin(), k.end());
+    tets.push_back(k);
+  }
+  std::sort(tets.begin(), tets.end());
+  tets.erase(std::unique(tets.begin(), tets.end()), tets.end());
+  std::unordered_map<uint32_t, uint32_t> index;
+  mesh_vertices_.clear();
+  auto vertex = [&](uint32_t code) -> uint32_t {
+    const uint32_t j = code / two;
+    if (code % two == 0) return j;
+    auto it = index.find(code);
+    if (it != index.end()) return it->second;
+    const uint32_t at = n_ + uint32_t(index.size());
+    index.emplace(code, at);
+    return at;
+  };
+  std::vector<uint32_t> flat;
+  flat.reserve(tets.size() * 4);
+  for (const auto& t : tets)
+    for (uint32_t c : t) flat.push_back(vertex(c));
+  std::vector<uint32_t> codes(index.size());
+  for (const auto& kv : index) codes[kv.second - n_] = kv.first;
+  mesh_vertices_.resize(size_t(n_ + codes.size()) * 4);
+  auto put = [&](uint32_t at, const float4 q) {
+    const double w = 1.0 + double(q.x);
+    mesh_vertices_[4 * at + 0] = float(q.y / w);
+    mesh_vertices_[4 * at + 1] = float(q.z / w);
+    mesh_vertices_[4 * at + 2] = float(q.w / w);
+    mesh_vertices_[4 * at + 3] = 0.0f;
+  };
+  const float4* ops = reinterpret_cast<const float4*>(operations());
+  for (uint32_t j = 0; j < n_; ++j) put(j, make_float4(host_[4 * j], host_[4 * j + 1], host_[4 * j + 2], host_[4 * j + 3]));
+  for (size_t i = 0; i < codes.size(); ++i) {
+    const uint32_t c = codes[i], j = c / two, k = (c % two) / 2;
+    const float4 src = make_float4(host_[4 * j], host_[4 * j + 1], host_[4 * j + 2], host_[4 * j + 3]);
+    float4 y = image(src, ops[k], k);
+    if (c & 1u) y = neg4(y);
+    put(n_ + uint32_t(i), y);
+  }
+  mesh_tets_.swap(flat);
 }
 
 const uint32_t* Engine::point_indices() const {

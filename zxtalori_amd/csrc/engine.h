@@ -177,6 +177,15 @@ class Engine {
   // Exactly `target` orientations (engine.zig:199-232): the largest base grid that fits,
   // then centres of the largest empty caps until the count is reached.
   void generate_exact(uint32_t group, uint32_t target, bool add_identity, uint32_t kind);
+  // The Delaunay mesh around the sample (covering measured too): stereographic vertices
+  // (4 floats each, the sample first) and tetrahedra (4 indices each).
+  void build_mesh();
+  const float* mesh_vertices() const { return mesh_vertices_.data(); }
+  const uint32_t* mesh_tets() const { return mesh_tets_.data(); }
+  uint32_t mesh_vertex_count() const { return uint32_t(mesh_vertices_.size() / 4); }
+  uint32_t mesh_tet_count() const { return uint32_t(mesh_tets_.size() / 4); }
+  std::vector<float> mesh_vertices_;
+  std::vector<uint32_t> mesh_tets_;
   void measure();
   void spacing();
   void relax(uint32_t steps);
@@ -293,8 +302,11 @@ class Engine {
   PinnedVec<float4> cover_vq_host_;
   PinnedVec<float> cover_vr_host_;
   uint32_t inserted_count_ = 0;
-  // One covering measurement (with every certified cell's vertices when asked).
-  void cover_measure(bool vertices);
+  DevBuf cover_tets_;
+  PinnedVec<uint4> cover_tets_host_;
+  // One covering measurement (with every certified cell's vertices, or its Delaunay
+  // tetrahedra, when asked).
+  void cover_measure(bool vertices, bool tets);
   std::vector<float> cap_centres(uint32_t wanted);
   uint32_t orbit_compact(std::vector<float>& q, uint32_t unique) const;
   PinnedVec<uint32_t> point_index_;

@@ -231,11 +231,14 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
 // a flag (0 certified, 1 needs a cutoff beyond c_max, 2 too many neighbours) and the cell's
 // face count; grid: the near-field buffers built at cutoff c_max, or null to take every
 // signed image.  With vq, every vertex of every certified cell is appended (direction and
-// S3 radius) up to vcap, *vcount counting them all (zero it first).
+// S3 radius) up to vcap, *vcount counting them all (zero it first); with tets, the
+// Delaunay tetrahedra at those vertices as four orbit codes j*2|K| + 2k + sign (~0 for a
+// box face) up to tcap, *tcount counting them.
 void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                   const NearBuffers* grid, const NearCut& cut, float c_start, float c_max, float* theta,
                   float4* witness, uint32_t* flags, float* faces, float4* vq = nullptr, float* vr = nullptr,
-                  uint32_t* vcount = nullptr, uint32_t vcap = 0);
+                  uint32_t* vcount = nullptr, uint32_t vcap = 0, uint4* tets = nullptr, uint32_t* tcount = nullptr,
+                  uint32_t tcap = 0);
 void launch_degrees(Prof& p, hipStream_t s, const float* chord, uint32_t n, float* out, uint32_t* least);
 
 // BB step (relax.wgsl) of every job in the set, resident: stats, sums, median, move.

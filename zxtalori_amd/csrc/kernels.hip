@@ -1588,7 +1588,7 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
 // gives the cell, and only its faces are kept (the cell only shrinks, so a plane that
 // misses it now misses it for good).  The cell is certified when 2 theta is below the
 // distance of every unseen neighbour; the windows grow up to c_max.
-constexpr uint32_t kCoverGather = 512;  // candidates held while a window is chosen
+constexpr uint32_t kCoverGather = 256;  // candidates held while a window is chosen
 constexpr uint32_t kCoverWindow = 64;   // new planes per window
 constexpr uint32_t kCoverPlanes = 320;  // faces kept plus one window
 constexpr uint32_t kCoverFirst = 32;
@@ -1599,11 +1599,14 @@ constexpr uint32_t kCoverVertices = 192;  // vertices of the last cell kept for 
 struct CoverLds {
   float4 y[kCoverGather];
   float d2[kCoverGather];
+  uint32_t code[kCoverGather];  // the candidate's orbit code j*2|K| + 2k + sign
   double pl[kCoverPlanes][4];  // n.x, n.y, n.z, d: current faces, then the window nearest first
   float pd2[kCoverPlanes];     // chord^2 of the plane's neighbour
+  uint32_t pcode[kCoverPlanes];  // its orbit code
   uint8_t face[kCoverPlanes];  // the plane bounds the cell (some vertex lies on it)
   float cut;
-  double vt[kCoverVertices][4];  // the cell's vertices (t, |t|^2), with `faces`
+  double vt[kCoverVertices][4];    // the cell's vertices (t, |t|^2), with `faces`
+  uint32_t vcode[kCoverVertices][3];  // the codes of the three planes through each (~0: box)
   uint32_t nv;
 };
 
@@ -1686,6 +1689,9 @@ __device__ CoverVertex cover_cell(CoverLds& L, uint32_t m, bool faces) {
           L.vt[at][1] = t[1];
           L.vt[at][2] = t[2];
           L.vt[at][3] = r2;
+          L.vcode[at][0] = a < m ? L.pcode[a] : ~0u;
+          L.vcode[at][1] = b < m ? L.pcode[b] : ~0u;
+          L.vcode[at][2] = c < m ? L.pcode[c] : ~0u;
         }
       }
     }
@@ -1726,6 +1732,11 @@ struct CoverArgs {
   float* vr;
   uint32_t* vcount;
   uint32_t vcap;
+  // Optional: the Delaunay tetrahedra of every certified cell (the source and the three
+  // neighbours whose planes meet at a vertex), as orbit codes, up to tcap; *tcount counts.
+  uint4* tets;
+  uint32_t* tcount;
+  uint32_t tcap;
 };
 
 __device__ inline float chord2_of_angle(float a) {
@@ -1779,12 +1790,13 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
     float hi2 = chord2_of_angle(hi);
     // Gather the window (lo2, hi2].
     uint32_t count = 0;
-    auto take = [&](bool ok, const float4 y, float d2) {
+    auto take = [&](bool ok, const float4 y, float d2, uint32_t code) {
       const uint64_t mask = __ballot(ok);
       const uint32_t pos = count + mbcnt(mask);
       if (ok && pos < kCoverGather) {
         L.y[pos] = y;
         L.d2[pos] = d2;
+        L.code[pos] = code;
       }
       count += uint32_t(__popcll(mask));
     };
@@ -1818,12 +1830,14 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
             bool ok = false;
             float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
             float d2 = 0.f;
+            uint32_t code = 0;
             if (c < n_row) {
               y = b.sq[s_row + c];
               d2 = chord2(x, y);
-              ok = d2 > lo2 && d2 <= hi2 && b.srank[s_row + c] != self_rank;
+              code = b.srank[s_row + c];
+              ok = d2 > lo2 && d2 <= hi2 && code != self_rank;
             }
-            take(ok, y, d2);
+            take(ok, y, d2, code);
           }
         }
       }
@@ -1834,14 +1848,20 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
         bool ok = false;
         float4 y = make_float4(0.f, 0.f, 0.f, 0.f);
         float d2 = 0.f;
+        uint32_t code = 0;
         if (idx < images) {
           const uint32_t j = idx / a.order, k = idx % a.order;
           y = image(a.q[j], a.ops[k], k);
-          if (dot4(x, y) < 0.0f) y = neg4(y);
+          uint32_t sign = 0;
+          if (dot4(x, y) < 0.0f) {
+            y = neg4(y);
+            sign = 1;
+          }
           d2 = chord2(x, y);
+          code = j * 2 * a.order + 2 * k + sign;
           ok = d2 > lo2 && d2 <= hi2 && !(j == src && k == 0);
         }
-        take(ok, y, d2);
+        take(ok, y, d2, code);
       }
     }
     if (count > kCoverGather) {
@@ -1894,6 +1914,7 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
         for (int k = 0; k < 3; ++k) L.pl[at][k] = y[0] * e[k][0] + y[1] * e[k][1] + y[2] * e[k][2] + y[3] * e[k][3];
         L.pl[at][3] = 1.0 - (y[0] * p[0] + y[1] * p[1] + y[2] * p[2] + y[3] * p[3]);
         L.pd2[at] = L.d2[i];
+        L.pcode[at] = L.code[i];
       }
     }
     __syncthreads();
@@ -1919,9 +1940,11 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
       const bool f = i < keep && L.face[i] != 0;
       double v[4] = {0, 0, 0, 0};
       float d = 0.f;
+      uint32_t code = 0;
       if (f) {
         for (int k = 0; k < 4; ++k) v[k] = L.pl[i][k];
         d = L.pd2[i];
+        code = L.pcode[i];
       }
       const uint64_t mask = __ballot(f);
       const uint32_t pos = out + mbcnt(mask);
@@ -1929,6 +1952,7 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
       if (f) {
         for (int k = 0; k < 4; ++k) L.pl[pos][k] = v[k];
         L.pd2[pos] = d;
+        L.pcode[pos] = code;
       }
       out += uint32_t(__popcll(mask));
       __syncthreads();
@@ -1959,6 +1983,17 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
       a.vr[base + i] = float(atan(sqrt(fmax(L.vt[i][3], 0.0))));
     }
   }
+  if (a.tets && flag == 0) {
+    const uint32_t nv = min(L.nv, kCoverVertices);
+    const uint32_t self_code = src * 2 * a.order;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(a.tcount, nv);
+    base = uint32_t(__shfl(int(base), 0));
+    for (uint32_t i = lane; i < nv; i += 64) {
+      if (base + i >= a.tcap) break;
+      a.tets[base + i] = make_uint4(self_code, L.vcode[i][0], L.vcode[i][1], L.vcode[i][2]);
+    }
+  }
   if (lane == 0) {
     const double theta = atan(sqrt(fmax(cell.r2, 0.0)));
     a.theta[src] = float(theta);
@@ -1974,7 +2009,7 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
 void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                   const NearBuffers* grid, const NearCut& cut, float c_start, float c_max, float* theta,
                   float4* witness, uint32_t* flags, float* faces, float4* vq, float* vr, uint32_t* vcount,
-                  uint32_t vcap) {
+                  uint32_t vcap, uint4* tets, uint32_t* tcount, uint32_t tcap) {
   CoverArgs a{};
   a.q = q;
   a.n = n;
@@ -1993,6 +2028,9 @@ void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const flo
   a.vr = vr;
   a.vcount = vcount;
   a.vcap = vcap;
+  a.tets = tets;
+  a.tcount = tcount;
+  a.tcap = tcap;
   p.run("cover", [&] { hipLaunchKernelGGL(k_cover, dim3(n), dim3(64), 0, s, a); });
 }
 
