@@ -75,9 +75,13 @@ void zo_cancel(zxt_engine* e);
  * 2 (Voronoi faces); reused while the sample is unchanged.  Errors: "Covering radius:
  * TooFewOrientations" (fewer than 4), "...: CoveringDomainNotEnclosed". */
 int32_t zo_cover(zxt_engine* e);
-/* Out of this build's path (covering optimisation), present so the reference's loader
- * resolves every name (src/exports.zig:104-120).  They return -1 with a "NotImplemented"
- * message. */
+/* src/exports.zig:104-120 (engine.zig:399-420): `steps` covering optimisation steps, each
+ * pulling the points of the largest empty cap toward its centre and kept when the radius
+ * drops (mode & 3: 0 "tet", 1 "face" = half steps; other bits as the reference's variants);
+ * `finish` leaves the best sample seen; moves never bring a pair nearer than floor_deg
+ * (SO(3) degrees).  zo_cover_kicks: plateau patience, relative gain, kick size (times the best
+ * radius) and restart trust.  Errors: "Covering optimization: InvalidCoveringMode",
+ * "...: InvalidSeparationFloor", "Covering kicks: InvalidKickPatience" and the like. */
 int32_t zo_cover_optimize(zxt_engine* e, uint32_t steps, uint32_t mode, uint32_t finish, float floor_deg);
 int32_t zo_cover_kicks(zxt_engine* e, uint32_t patience, float progress, float size, float trust);
 /* src/exports.zig:88-103 (engine.zig:352-358): a covering measurement that also keeps the

@@ -330,8 +330,8 @@ def test_error_paths():
         with pytest.raises(RuntimeError, match="TooFewOrientations"):
             grid.relax(1)
     with zo.haar("O", 200, 3) as grid:
-        with pytest.raises(RuntimeError, match="NotImplemented"):
-            grid.optimize_covering(10)
+        with pytest.raises(ValueError):
+            grid.optimize_covering(1, mode="everything")
 
 
 def test_fixed_identity_stays_exact(operator_fixture):

@@ -3,8 +3,7 @@
 Mirrors the reference's Python adapter (python/zxtalori/api.py): the same names,
 argument meanings, error behaviour (RuntimeError carrying zo_error's text) and JSON stats.
 The QVM relaxation and spacing path, inverse Hopf and FCC transport generation (also with
-exact counts), the covering radius and its Delaunay mesh run here; covering optimisation
-raises RuntimeError("...: NotImplemented").
+exact counts), the covering radius, its Delaunay mesh and its optimisation run here.
 """
 from __future__ import annotations
 
@@ -305,9 +304,48 @@ class Grid:
                 break
         return self
 
-    def optimize_covering(self, steps=2000, **_):
+    def optimize_covering(self, steps=2000, *, mode="face", batch=100, callback=None, local=True, check=False,
+                          local_rounds=None, min_separation_deg=None, live=True, patience=150, plateau_gain=5e-4,
+                          kick=0.04, kick_trust=0.03):
+        """Lower the covering radius; keeps the best sample (python/zxtalori/api.py:269-336).
+
+        Each step pulls the points spanning the largest empty cap toward its centre ("tet";
+        "face" takes half steps) and keeps the move when the measured covering radius
+        drops.  After `patience` steps without a relative gain of `plateau_gain`, the best
+        sample is restored and every point turned by an angle of `kick` times the best
+        radius; steps restart at `kick_trust`.  min_separation_deg rejects moves that bring
+        any pair nearer than that SO(3) angle.  callback(grid, stats) runs after each batch
+        of steps; return False to stop.  local, check, live and local_rounds select the
+        reference's local re-tessellation variants and are accepted for compatibility (every
+        step here measures the whole sample).  Energy and separation are not measured:
+        stats["measures_current"] is False until measure().
+        """
         self._open()
-        self._check(self._lib.zo_cover_optimize(self._handle, int(steps), 0, 1, 0.0))
+        modes = {"tet": 0, "face": 1}
+        floor = float(min_separation_deg or 0)
+        if not 0 <= floor < 180:
+            raise ValueError("min_separation_deg must be in [0, 180)")
+        if local_rounds is not None and not 1 <= int(local_rounds) <= 255:
+            raise ValueError("local_rounds must be 1-255")
+        variant = (0 if local else 4) | (8 if check and local else 0) | (0 if live else 16) | (int(local_rounds or 0) << 16)
+        if mode not in modes:
+            raise ValueError(f"mode must be one of {', '.join(modes)}")
+        if int(steps) < 0 or int(batch) < 1:
+            raise ValueError("steps must be nonnegative and batch positive")
+        if not (1 <= int(patience) <= 1000000 and 0 <= plateau_gain < 1 and 0 <= kick <= 1 and 0 < kick_trust <= 1):
+            raise ValueError("patience must be 1-1000000, plateau_gain in [0, 1), kick in [0, 1] and kick_trust in (0, 1]")
+        self._check(self._lib.zo_cover_kicks(self._handle, int(patience), float(plateau_gain), float(kick), float(kick_trust)))
+        done = 0
+        while done < int(steps):
+            count = min(int(batch), int(steps) - done)
+            done += count
+            last = done >= int(steps)
+            self._check(self._lib.zo_cover_optimize(self._handle, count, modes[mode] | variant, int(last), floor))
+            stats = self.stats
+            self.history.append(stats)
+            if not last and callback is not None and callback(self, stats) is False:
+                self._check(self._lib.zo_cover_optimize(self._handle, 0, modes[mode] | variant, 1, floor))
+                break
         return self
 
     def save(self, path):

@@ -174,10 +174,13 @@ void zo_cancel(zxt_engine* e) { e->engine.cancelled.store(true); }
 int32_t zo_cover(zxt_engine* e) {
   return guarded(e, "Covering radius", [&] { e->engine.cover(); });
 }
-int32_t zo_cover_optimize(zxt_engine* e, uint32_t, uint32_t, uint32_t, float) {
-  return not_implemented(e, "Covering optimization");
+int32_t zo_cover_optimize(zxt_engine* e, uint32_t steps, uint32_t mode, uint32_t finish, float floor_deg) {
+  return guarded(e, "Covering optimization",
+                 [&] { e->engine.optimize_covering(steps, mode, finish != 0, floor_deg); });
 }
-int32_t zo_cover_kicks(zxt_engine* e, uint32_t, float, float, float) { return not_implemented(e, "Covering kicks"); }
+int32_t zo_cover_kicks(zxt_engine* e, uint32_t patience, float progress, float size, float trust) {
+  return guarded(e, "Covering kicks", [&] { e->engine.set_kicks(patience, progress, size, trust); });
+}
 int32_t zo_mesh(zxt_engine* e) {
   return guarded(e, "DT display", [&] { e->engine.build_mesh(); });
 }

@@ -393,6 +393,7 @@ void Engine::reset_run() {
   point_index_iteration_ = -1;
   point_covering_iteration_ = -1;
   cover_ = CoverResult{};
+  opt_ = Optimizer{};
   relax_state_.release();
   spacing_state_.release();
   measure_state_.release();
@@ -709,6 +710,7 @@ void Engine::spacing() {
 void Engine::begin_relax() {
   if (n_ < 2) fail("TooFewOrientations");
   cancelled.store(false);
+  opt_.active = false;  // a relaxed sample leaves the optimiser's best behind
   sync_neighbourhood();
   relax_radius_ = mean_radius(uint64_t(n_) * order());
 }
@@ -890,7 +892,7 @@ void Engine::cover() {
   cover_measure(false, false);
 }
 
-void Engine::cover_measure(bool vertices, bool tets) {
+void Engine::cover_measure(bool vertices, bool tets, bool codes) {
   if (n_ < 4) fail("TooFewOrientations");
   const uint32_t ord = order();
   const double density = double(n_) * ord;
@@ -910,6 +912,7 @@ void Engine::cover_measure(bool vertices, bool tets) {
     cover_vr_.ensure(size_t(vcap) * 4);
   }
   if (tets) cover_tets_.ensure(size_t(tcap) * 16);
+  if (codes) cover_wcode_.ensure(size_t(n_) * 16);
   if (vertices || tets) cover_vcount_.ensure(8);
   float4* q = q_.as<float4>();
   uint32_t rounds = 0, rebuilt = 0;
@@ -935,7 +938,13 @@ void Engine::cover_measure(bool vertices, bool tets) {
                  cover_witness_.as<float4>(), cover_flags_.as<uint32_t>(), cover_faces_.as<float>(),
                  vertices ? cover_vq_.as<float4>() : nullptr, vertices ? cover_vr_.as<float>() : nullptr,
                  vertices ? cover_vcount_.as<uint32_t>() : nullptr, vcap, tets ? cover_tets_.as<uint4>() : nullptr,
-                 tets ? cover_vcount_.as<uint32_t>() + 1 : nullptr, tcap);
+                 tets ? cover_vcount_.as<uint32_t>() + 1 : nullptr, tcap, codes ? cover_wcode_.as<uint4>() : nullptr);
+    if (codes) {
+      download(cover_wcode_host_, cover_wcode_, n_);
+      cover_witnesses_.resize(n_);
+      check(hipMemcpyAsync(cover_witnesses_.data(), cover_witness_.p, size_t(n_) * 16, hipMemcpyDeviceToHost, stream),
+            "download");
+    }
     download(point_covering_, cover_theta_, n_);
     download(point_faces_, cover_faces_, n_);
     download(cover_flags_host_, cover_flags_, n_);
@@ -1255,6 +1264,195 @@ void Engine::build_mesh() {
   mesh_tets_.swap(flat);
 }
 
+// ---- covering optimisation -----------------------------------------------------------
+void Engine::set_kicks(uint32_t patience, float progress, float size, float trust) {
+  if (patience < 1 || patience > 1000000) fail("InvalidKickPatience");
+  if (!(progress >= 0.0f && progress < 1.0f)) fail("InvalidKickProgress");
+  if (!(size >= 0.0f && size <= 1.0f)) fail("InvalidKickSize");
+  if (!(trust > 0.0f && trust <= 1.0f)) fail("InvalidKickTrust");
+  kicks_ = Kicks{patience, progress, size, trust};
+}
+
+// Rows `index` of the sample set to `rows` (4 floats each), folded into the zone first
+// when asked (on the GPU, those rows only), device and host copies both.
+void Engine::write_rows(const std::vector<uint32_t>& index, const std::vector<float>& rows, bool fold) {
+  const size_t m = index.size();
+  if (!m) return;
+  opt_rows_.ensure(m * 16);
+  check(hipMemcpyAsync(opt_rows_.p, rows.data(), m * 16, hipMemcpyHostToDevice, stream), "upload");
+  if (fold) launch_fold(prof, stream, opt_rows_.as<float4>(), uint32_t(m), ops_device(), order(), group_);
+  for (size_t i = 0; i < m; ++i)
+    check(hipMemcpyAsync(q_.as<float4>() + index[i], opt_rows_.as<float4>() + i, 16, hipMemcpyDeviceToDevice, stream),
+          "rows");
+  std::vector<float> folded(m * 4);
+  check(hipMemcpyAsync(folded.data(), opt_rows_.p, m * 16, hipMemcpyDeviceToHost, stream), "download");
+  sync();
+  for (size_t i = 0; i < m; ++i) std::memcpy(&host_[size_t(index[i]) * 4], &folded[4 * i], 16);
+}
+
+// The whole sample replaced by values (already in the zone), keeping the run's records.
+void Engine::write_sample(const std::vector<float>& values) {
+  check(hipMemcpyAsync(q_.p, values.data(), values.size() * 4, hipMemcpyHostToDevice, stream), "upload");
+  sync();
+  std::memcpy(host_.data(), values.data(), values.size() * 4);
+}
+
+void Engine::optimize_covering(uint32_t steps, uint32_t mode, bool finish, float floor_deg) {
+  if (n_ < 4) fail("TooFewOrientations");
+  if ((mode & 3u) > 1 || (mode & ~0xff001fu) != 0) fail("InvalidCoveringMode");
+  if (!(floor_deg >= 0.0f && floor_deg < 180.0f)) fail("InvalidSeparationFloor");
+  if (fix_identity_ && fixed_index_ < 0) fail("IdentityMissingEnableAddIdentity");
+  cancelled.store(false);
+  const uint32_t ord = order(), two = 2 * ord;
+  const float4* ops = reinterpret_cast<const float4*>(operations());
+  // Moves pull a cap's points this fraction of the way to its centre; "face" (mode 1)
+  // moves half as far per step.
+  const double pull = (mode & 3u) == 1 ? 0.5 : 1.0;
+  auto measure = [&]() { cover_measure(false, false, true); };
+  auto current_ok = [&]() {
+    return cover_.valid && cover_.iteration == long(iteration_) && cover_wcode_host_.size() == n_;
+  };
+  if (!opt_.active) {
+    if (!current_ok()) measure();
+    opt_ = Optimizer{};
+    opt_.active = true;
+    opt_.best = opt_.reference = cover_.radius_deg;
+    opt_.best_q.assign(host_.begin(), host_.end());
+  } else if (!current_ok()) {
+    measure();
+  }
+  // The separation floor is checked with a spacing pass (SO(3) degrees).
+  auto separation_ok = [&]() {
+    if (floor_deg <= 0.0f) return true;
+    const double delta = kSpacingDelta;
+    spacing_state_.set_near(group_, n_ > kSpacingAllPairs ? &delta : nullptr);
+    return nearest_pass(spacing_state_) >= double(floor_deg);
+  };
+  uint64_t rng = 0x9E3779B97F4A7C15ull ^ (uint64_t(opt_.kicks + 1) << 17);
+  auto uniform = [&]() {
+    rng ^= rng << 13;
+    rng ^= rng >> 7;
+    rng ^= rng << 17;
+    return double(rng >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (uint32_t s = 0; s < steps; ++s) {
+    if (cancelled.load()) break;
+    uint32_t worst = 0;
+    for (uint32_t i = 1; i < n_; ++i)
+      if (point_covering_[i] > point_covering_[worst]) worst = i;
+    const uint4 codes = cover_wcode_host_[worst];
+    const float4 wv = cover_witnesses_[worst];
+    const double v[4] = {wv.x, wv.y, wv.z, wv.w};
+    std::vector<uint32_t> index;
+    std::vector<float> rows, old_rows;
+    for (uint32_t code : {codes.x, codes.y, codes.z, codes.w}) {
+      if (code == ~0u) continue;
+      const uint32_t j = code / two, k = (code % two) / 2;
+      const bool neg = (code & 1u) != 0;
+      if (fix_identity_ && long(j) == fixed_index_) continue;
+      if (std::find(index.begin(), index.end(), j) != index.end()) continue;
+      const float4 qj = make_float4(host_[4 * j], host_[4 * j + 1], host_[4 * j + 2], host_[4 * j + 3]);
+      float4 y = image(qj, ops[k], k);
+      if (neg) y = neg4(y);
+      double yd[4] = {y.x, y.y, y.z, y.w};
+      const double side = yd[0] * v[0] + yd[1] * v[1] + yd[2] * v[2] + yd[3] * v[3] < 0 ? -1.0 : 1.0;
+      double nyv[4], norm = 0.0;
+      for (int c = 0; c < 4; ++c) {
+        nyv[c] = yd[c] + opt_.alpha * pull * (side * v[c] - yd[c]);
+        norm += nyv[c] * nyv[c];
+      }
+      norm = 1.0 / std::sqrt(norm);
+      // q_j' = (+-y') g_k^-1
+      const float4 yn = make_float4(float(nyv[0] * norm), float(nyv[1] * norm), float(nyv[2] * norm),
+                                    float(nyv[3] * norm));
+      const float4 g = ops[k];
+      float4 qn = qmul(neg ? neg4(yn) : yn, make_float4(g.x, -g.y, -g.z, -g.w));
+      index.push_back(j);
+      rows.insert(rows.end(), {qn.x, qn.y, qn.z, qn.w});
+      old_rows.insert(old_rows.end(), {qj.x, qj.y, qj.z, qj.w});
+    }
+    ++opt_.steps;
+    if (index.empty()) {
+      ++opt_.rejected;
+      continue;
+    }
+    const CoverResult before = cover_;
+    std::vector<float> before_radius(point_covering_.begin(), point_covering_.end());
+    std::vector<uint4> before_codes(cover_wcode_host_.begin(), cover_wcode_host_.end());
+    std::vector<float4> before_witness = cover_witnesses_;
+    write_rows(index, rows, true);
+    ++iteration_;
+    measure();
+    const bool better = cover_.radius_deg < before.radius_deg - 1e-12 && separation_ok();
+    if (better) {
+      ++opt_.improved;
+      opt_.alpha = std::min(0.6, opt_.alpha * 1.25);
+      if (cover_.radius_deg < opt_.best - 1e-12) {
+        opt_.best = cover_.radius_deg;
+        opt_.best_q.assign(host_.begin(), host_.end());
+      }
+      if ((opt_.reference - cover_.radius_deg) >= double(kicks_.progress) * opt_.reference) {
+        opt_.reference = cover_.radius_deg;
+        opt_.since_gain = 0;
+      } else {
+        ++opt_.since_gain;
+      }
+    } else {
+      ++opt_.rejected;
+      ++opt_.since_gain;
+      write_rows(index, old_rows, false);
+      --iteration_;
+      cover_ = before;
+      std::copy(before_radius.begin(), before_radius.end(), point_covering_.data());
+      std::copy(before_codes.begin(), before_codes.end(), cover_wcode_host_.data());
+      cover_witnesses_ = before_witness;
+      point_covering_iteration_ = long(iteration_);
+      opt_.alpha = opt_.alpha * 0.5 < 1e-3 ? 0.3 : opt_.alpha * 0.5;
+    }
+    if (opt_.since_gain >= kicks_.patience) {
+      // Plateau: from the best sample, every free point turned by a random rotation of RMS
+      // angle `size` times the best radius; steps restart at `trust`.
+      std::vector<float> values = opt_.best_q;
+      const double angle = double(kicks_.size) * opt_.best * kPi / 360.0;
+      for (uint32_t i = 0; i < n_ && angle > 0.0; ++i) {
+        if (fix_identity_ && long(i) == fixed_index_) continue;
+        double a[3], len = 0.0;
+        for (double& x : a) {
+          x = 2.0 * uniform() - 1.0;
+          len += x * x;
+        }
+        len = std::sqrt(std::max(len, 1e-30));
+        const double t = angle * std::sqrt(3.0) * uniform();
+        const float4 r = make_float4(float(std::cos(0.5 * t)), float(std::sin(0.5 * t) * a[0] / len),
+                                     float(std::sin(0.5 * t) * a[1] / len), float(std::sin(0.5 * t) * a[2] / len));
+        const float4 q = make_float4(values[4 * i], values[4 * i + 1], values[4 * i + 2], values[4 * i + 3]);
+        const float4 out = normalize4(qmul(q, r));
+        values[4 * i] = out.x;
+        values[4 * i + 1] = out.y;
+        values[4 * i + 2] = out.z;
+        values[4 * i + 3] = out.w;
+      }
+      write_sample(values);
+      fold(n_);
+      fetch();
+      ++iteration_;
+      measure();
+      ++opt_.kicks;
+      opt_.since_gain = 0;
+      opt_.reference = cover_.radius_deg;
+      opt_.alpha = std::max(double(kicks_.trust) * 10.0, 1e-3);
+    }
+  }
+  // The sample kept is the best one seen (kicked samples included only once a move from
+  // them was accepted, so a floor the moves respect holds for it).
+  if (finish && !std::equal(opt_.best_q.begin(), opt_.best_q.end(), host_.begin())) {
+    write_sample(opt_.best_q);
+    ++iteration_;
+    measure();
+  }
+  measured_ = measured_ && measure_iteration_ == iteration_;
+}
+
 const uint32_t* Engine::point_indices() const {
   if (point_index_iteration_ < 0 || point_index_.size() != n_) return nullptr;
   return point_index_.data();
@@ -1293,6 +1491,10 @@ struct Json {
   void boolean(const char* k, bool v) {
     key(k);
     s += v ? "true" : "false";
+  }
+  void raw(const char* k, const char* text) {
+    key(k);
+    s += text;
   }
   void floats(const char* k, const float* v, int count) {
     key(k);
@@ -1388,6 +1590,20 @@ std::string Engine::stats() {
                           "covering_tetrahedra", "covering_certificate_probes", "covering_exact_predicates",
                           "covering_rounds"})
       j.null(k);
+  }
+  j.integer("covering_optimizer_steps", opt_.steps);
+  j.integer("covering_optimizer_kicks", opt_.kicks);
+  if (opt_.active)
+    j.num("covering_optimizer_best_deg", opt_.best);
+  else
+    j.null("covering_optimizer_best_deg");
+  j.integer("covering_optimizer_improved", opt_.improved);
+  j.integer("covering_optimizer_rejected", opt_.rejected);
+  {
+    char b[256];
+    snprintf(b, sizeof(b), "{\"patience\":%u,\"progress\":%.9g,\"size\":%.9g,\"trust\":%.9g}", kicks_.patience,
+             double(kicks_.progress), double(kicks_.size), double(kicks_.trust));
+    j.raw("covering_kick_settings", b);
   }
   j.boolean("measured", m);
   j.boolean("measures_current", current);

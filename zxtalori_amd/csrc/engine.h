@@ -177,6 +177,11 @@ class Engine {
   // Exactly `target` orientations (engine.zig:199-232): the largest base grid that fits,
   // then centres of the largest empty caps until the count is reached.
   void generate_exact(uint32_t group, uint32_t target, bool add_identity, uint32_t kind);
+  // Covering optimisation: `steps` greedy cap moves (mode 0 "tet", 1 "face" = half steps);
+  // `finish` leaves the best sample seen; the floor (SO(3) degrees) rejects moves that bring
+  // any pair nearer than it.  set_kicks: plateau patience, relative gain, kick size, trust.
+  void set_kicks(uint32_t patience, float progress, float size, float trust);
+  void optimize_covering(uint32_t steps, uint32_t mode, bool finish, float floor_deg);
   // The Delaunay mesh around the sample (covering measured too): stereographic vertices
   // (4 floats each, the sample first) and tetrahedra (4 indices each).
   void build_mesh();
@@ -302,11 +307,29 @@ class Engine {
   PinnedVec<float4> cover_vq_host_;
   PinnedVec<float> cover_vr_host_;
   uint32_t inserted_count_ = 0;
-  DevBuf cover_tets_;
-  PinnedVec<uint4> cover_tets_host_;
+  DevBuf cover_tets_, cover_wcode_, opt_rows_;
+  PinnedVec<uint4> cover_tets_host_, cover_wcode_host_;
+  std::vector<float4> cover_witnesses_;
   // One covering measurement (with every certified cell's vertices, or its Delaunay
   // tetrahedra, when asked).
-  void cover_measure(bool vertices, bool tets);
+  void cover_measure(bool vertices, bool tets, bool codes = false);
+  // Covering optimisation (engine.zig:399-420, covering.zig Live): greedy moves of the
+  // points spanning the largest empty cap toward its centre, kept when the measured radius
+  // drops; plateau kicks from the best sample.
+  struct Kicks {
+    uint32_t patience = 150;
+    float progress = 5e-4f, size = 0.04f, trust = 0.03f;
+  };
+  struct Optimizer {
+    bool active = false;
+    uint32_t steps = 0, kicks = 0, improved = 0, rejected = 0, since_gain = 0;
+    double best = 0.0, reference = 0.0, alpha = 0.3;
+    std::vector<float> best_q;
+  };
+  Kicks kicks_;
+  Optimizer opt_;
+  void write_rows(const std::vector<uint32_t>& index, const std::vector<float>& rows, bool fold);
+  void write_sample(const std::vector<float>& values);
   std::vector<float> cap_centres(uint32_t wanted);
   uint32_t orbit_compact(std::vector<float>& q, uint32_t unique) const;
   PinnedVec<uint32_t> point_index_;

@@ -233,12 +233,13 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
 // signed image.  With vq, every vertex of every certified cell is appended (direction and
 // S3 radius) up to vcap, *vcount counting them all (zero it first); with tets, the
 // Delaunay tetrahedra at those vertices as four orbit codes j*2|K| + 2k + sign (~0 for a
-// box face) up to tcap, *tcount counting them.
+// box face) up to tcap, *tcount counting them; with wcode, per source the codes of the four
+// points spanning its cell's farthest vertex.
 void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                   const NearBuffers* grid, const NearCut& cut, float c_start, float c_max, float* theta,
                   float4* witness, uint32_t* flags, float* faces, float4* vq = nullptr, float* vr = nullptr,
                   uint32_t* vcount = nullptr, uint32_t vcap = 0, uint4* tets = nullptr, uint32_t* tcount = nullptr,
-                  uint32_t tcap = 0);
+                  uint32_t tcap = 0, uint4* wcode = nullptr);
 void launch_degrees(Prof& p, hipStream_t s, const float* chord, uint32_t n, float* out, uint32_t* least);
 
 // BB step (relax.wgsl) of every job in the set, resident: stats, sums, median, move.

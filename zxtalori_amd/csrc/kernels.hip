@@ -1613,6 +1613,7 @@ struct CoverLds {
 struct CoverVertex {
   double r2;
   double t[3];
+  uint32_t plane[3];  // the three planes through it (indices of the pass; >= m: box)
 };
 
 // Plane i of a pass over m neighbour planes: i < m a neighbour, else one of the six faces of
@@ -1636,7 +1637,7 @@ __device__ inline void cover_plane(const CoverLds& L, uint32_t m, uint32_t i, do
 __device__ CoverVertex cover_cell(CoverLds& L, uint32_t m, bool faces) {
   const uint32_t lane = lane_id();
   const uint32_t total = m + 6;
-  CoverVertex best{-1.0, {0.0, 0.0, 0.0}};
+  CoverVertex best{-1.0, {0.0, 0.0, 0.0}, {~0u, ~0u, ~0u}};
   for (uint32_t a = 0; a + 2 < total; ++a) {
     double na[3], da;
     cover_plane(L, m, a, na, da);
@@ -1678,7 +1679,7 @@ __device__ CoverVertex cover_cell(CoverLds& L, uint32_t m, bool faces) {
       const double slack = kCoverBox * (1.0 + 1e-12);
       inside = inside && fabs(t[0]) <= slack && fabs(t[1]) <= slack && fabs(t[2]) <= slack;
       if (!inside) continue;
-      if (r2 > best.r2) best = {r2, {t[0], t[1], t[2]}};
+      if (r2 > best.r2) best = {r2, {t[0], t[1], t[2]}, {a, b, c}};
       if (faces) {
         if (a < m) L.face[a] = 1;
         if (b < m) L.face[b] = 1;
@@ -1709,7 +1710,10 @@ __device__ CoverVertex cover_cell(CoverLds& L, uint32_t m, bool faces) {
   }
   CoverVertex out;
   out.r2 = r2;
-  for (int k = 0; k < 3; ++k) out.t[k] = __shfl(best.t[k], int(who));
+  for (int k = 0; k < 3; ++k) {
+    out.t[k] = __shfl(best.t[k], int(who));
+    out.plane[k] = uint32_t(__shfl(int(best.plane[k]), int(who)));
+  }
   return out;
 }
 
@@ -1737,6 +1741,9 @@ struct CoverArgs {
   uint4* tets;
   uint32_t* tcount;
   uint32_t tcap;
+  // Optional: per source, the orbit codes of the four points whose cap is its cell's
+  // farthest vertex (the source first; ~0 for a box face).
+  uint4* wcode;
 };
 
 __device__ inline float chord2_of_angle(float a) {
@@ -1785,7 +1792,8 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
   }
   uint32_t faces = 0, flag = 1, nfaces = 0;
   float lo2 = -1.0f, hi = fminf(a.c_start, a.c_max);
-  CoverVertex cell{0.0, {0.0, 0.0, 0.0}};
+  CoverVertex cell{0.0, {0.0, 0.0, 0.0}, {~0u, ~0u, ~0u}};
+  uint32_t wc[3] = {~0u, ~0u, ~0u};
   for (int round = 0; round < 256; ++round) {
     float hi2 = chord2_of_angle(hi);
     // Gather the window (lo2, hi2].
@@ -1931,6 +1939,7 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
     __syncthreads();
     cell = cover_cell(L, keep, true);
     __syncthreads();
+    for (int k = 0; k < 3; ++k) wc[k] = cell.plane[k] < keep ? L.pcode[cell.plane[k]] : ~0u;
     const double theta = atan(sqrt(fmax(cell.r2, 0.0)));
     const double reach = angle_of_chord2(hi2);
     // Keep only the faces (stable compaction; a chunk is read before it is written).
@@ -2003,13 +2012,14 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
     a.witness[src] = make_float4(float(v[0] * vn), float(v[1] * vn), float(v[2] * vn), float(v[3] * vn));
     a.flags[src] = flag;
     a.faces[src] = float(nfaces);
+    if (a.wcode) a.wcode[src] = make_uint4(src * 2 * a.order, wc[0], wc[1], wc[2]);
   }
 }
 
 void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                   const NearBuffers* grid, const NearCut& cut, float c_start, float c_max, float* theta,
                   float4* witness, uint32_t* flags, float* faces, float4* vq, float* vr, uint32_t* vcount,
-                  uint32_t vcap, uint4* tets, uint32_t* tcount, uint32_t tcap) {
+                  uint32_t vcap, uint4* tets, uint32_t* tcount, uint32_t tcap, uint4* wcode) {
   CoverArgs a{};
   a.q = q;
   a.n = n;
@@ -2031,6 +2041,7 @@ void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const flo
   a.tets = tets;
   a.tcount = tcount;
   a.tcap = tcap;
+  a.wcode = wcode;
   p.run("cover", [&] { hipLaunchKernelGGL(k_cover, dim3(n), dim3(64), 0, s, a); });
 }
 
