@@ -60,6 +60,16 @@ def test_exact_count_thin_shell_hull(group, count):
         assert len(grid) == count
 
 
+@pytest.mark.parametrize("group", ["D2", "T", "O", "I"])
+def test_exact_ten_thousand_relaxes(group):
+    """tests/browser-parity.html:61-66: 10,000 with the identity, fixed, two steps."""
+    with zo.generate(group, count=10_000, exact=True, identity=True) as grid:
+        grid.identity(fixed=True)
+        grid.relax(2)
+        assert len(grid) == 10_000
+        assert grid.stats["separation_deg"] > 1
+
+
 def test_exact_range():
     with pytest.raises(RuntimeError, match="ExactCountRange32To1000000"):
         zo.generate("O", count=31, exact=True)
