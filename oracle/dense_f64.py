@@ -165,6 +165,33 @@ def measures(q, ops):
     }
 
 
+def log_ratio(q, ops):
+    """measure()'s log_ratio alone, in FP64 through one Gram matrix per operator: for unit x,
+    y with c = x.y, |x-y|^2 |x+y|^2 = 4(1 - c^2) (tests/test_log_scale.py:19-35 uses the same
+    identity).  Agrees with measures()['log_ratio'] to about 1e-11 at a fraction of its cost."""
+    q = np.asarray(q, dtype=np.float64)
+    q = q / np.linalg.norm(q, axis=-1, keepdims=True)
+    ops = np.asarray(ops, dtype=np.float64)
+    ops = ops / np.linalg.norm(ops, axis=-1, keepdims=True)
+    n = len(q)
+    total = 0.0
+    for k, op in enumerate(ops):
+        y = multiply(q, op)
+        y /= np.linalg.norm(y, axis=-1, keepdims=True)
+        c = q @ y.T
+        product = 4 * (1 - c * c)
+        if k == 0:
+            np.fill_diagonal(product, 1.0)
+        terms = -.5 * np.log(product) + .5
+        if k == 0:
+            np.fill_diagonal(terms, 0.0)
+        total += terms.sum()
+    total += n * (-math.log(2) + .25)
+    count = 2 * len(ops) * n
+    delta = total / n + .25 + math.log(count) / 3
+    return delta / (-5 / 12)
+
+
 def nearest_orbit(q, ops):
     """FP64 nearest orbit point of every source over the full signed orbit (the source's
     own identity image excluded): (rank j*2*order + 2*k + sign, chord^2)."""

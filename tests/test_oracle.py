@@ -301,3 +301,11 @@ def test_relax_descends():
     q3, rows, _ = oracle.relax(q, ops, GROUP_ID[group], 3)
     assert rows[:, 4].sum() <= e0
     np.testing.assert_allclose(np.linalg.norm(q3, axis=1), 1, atol=3e-7)
+
+
+@pytest.mark.parametrize("group", ["C2", "D4", "T", "I"])
+def test_gram_log_ratio_matches_explicit_differences(operator_fixture, group):
+    """dense.log_ratio (4(1 - c^2) per pair) against measures() (|x-y|^2 |x+y|^2 explicitly)."""
+    q = random_unit(120, 936).astype(np.float64)
+    ops = operator_fixture[group]
+    assert abs(dense.log_ratio(q, ops) - dense.measures(q, ops)["log_ratio"]) < 1e-9
