@@ -104,6 +104,29 @@ def test_near_rows(operator_fixture, group, count, delta):
     check_rows(force, energy, nearest, ref)
 
 
+@pytest.mark.parametrize("copies", [1, 2])
+def test_near_rows_with_exact_duplicates(operator_fixture, copies):
+    """Duplicates in the near field (I, N=3000, neighbourhood 20: no own image in reach):
+    the source's identity image is skipped, every other site at chord 0 is a pair
+    (pairs.wgsl:404-408), so the duplicates' nearest chord is 0 and their rows hold the
+    clamped pair terms; a relaxation step then fails like the reference's."""
+    note("near duplicates", copies)
+    q = random_unit(3000, 29)
+    q[1:1 + copies] = q[0]
+    with zo.from_quaternions(q, "I") as grid:
+        grid.relax(0, neighbourhood=20)
+        force, energy, nearest = grid.force_rows()
+        q = grid.quaternions
+        info = grid.near_info()
+        assert info["radius"] < 0.62  # below I's own-orbit angle, 0.628 rad
+        with pytest.raises(RuntimeError, match="DuplicateOrientations"):
+            grid.relax(1)
+    assert (nearest[:1 + copies] == 0).all() and (nearest[1 + copies:] > 0).all()
+    ops = ops32(operator_fixture, "I")
+    cut, _ = dense.near_cut(3000, len(ops), 20)
+    check_rows(force, energy, nearest, oracle.near_brute(q, ops, cut))
+
+
 def test_near_rows_repeat_bitwise_in_crowded_cells(operator_fixture):
     """A wide cutoff over a small sample (C2, N=5073, neighbourhood 20: r = 1.23 rad) crowds
     thousands of sites into the chart's central cells; their in-cell order must still be the

@@ -124,7 +124,6 @@ NearBuffers NearField::view() const {
   b.scan_tmp = scan_tmp.as<uint32_t>();
   b.counters = counters.as<unsigned long long>();
   b.counts = counts.as<uint32_t>();
-  b.self_pos = self_pos.as<uint32_t>();
   b.crowded = crowded.as<uint32_t>();
   b.crowded_capacity = crowded.p ? crowded_capacity(capacity) : 0u;
   b.huge = huge.as<uint32_t>();
@@ -135,7 +134,7 @@ NearBuffers NearField::view() const {
 static void release_near(NearField& f) {
   for (DevBuf* b : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
                     &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor, &f.order_buf,
-                    &f.scan_tmp, &f.counters, &f.counts, &f.self_pos, &f.crowded, &f.huge})
+                    &f.scan_tmp, &f.counters, &f.counts, &f.crowded, &f.huge})
     b->release();
   f.ready = false;
 }
@@ -228,7 +227,6 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, kMortonCells)) * 4);
   counters.ensure(4 * sizeof(unsigned long long));
   counts.ensure(size_t(n) * 8);
-  self_pos.ensure(size_t(n) * 4);
   crowded.ensure(size_t(crowded_capacity(capacity)) * 4);
   huge.ensure(size_t(huge_capacity(capacity)) * 4);
   ready = true;
@@ -811,7 +809,7 @@ size_t Engine::device_bytes() const {
     const NearField& f = rs->near;
     for (const DevBuf* d : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
                             &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor,
-                            &f.order_buf, &f.scan_tmp, &f.counters, &f.counts, &f.self_pos, &f.crowded, &f.huge})
+                            &f.order_buf, &f.scan_tmp, &f.counters, &f.counts, &f.crowded, &f.huge})
       b += d->bytes;
   }
   return b;

@@ -197,7 +197,6 @@ struct NearBuffers {
   uint32_t* scan_tmp = nullptr;  // scan_tmp_words(max cells)
   unsigned long long* counters = nullptr;  // [0] accepted pairs, [1] sites tested (force pass)
   uint32_t* counts = nullptr;    // 2n: per source, pairs and sites tested of the last force pass
-  uint32_t* self_pos = nullptr;  // n: sorted position of each source's identity image
   uint32_t* crowded = nullptr;   // crowded_capacity: cells with more sites than one thread sorts
   uint32_t crowded_capacity = 0;
   uint32_t* huge = nullptr;      // huge_capacity: cells with more sites than one wave sorts

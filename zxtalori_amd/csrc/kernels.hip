@@ -1183,7 +1183,6 @@ __global__ void k_site_gather(NearBuffers b) {
     b.sq[k] = b.uq[s];
     b.scode[k] = code;
     b.srank[k] = b.urank[s];
-    if (code & 0x80000000u) b.self_pos[code & 0x7fffffffu] = k;  // the source's own identity image
   }
 }
 
@@ -1266,8 +1265,7 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 
 // ---- near-field rows (pairs.wgsl:363-427) -------------------------------------------
 struct WaveLds {
-  float4 qe[128];  // x - y of each queued site
-  float qd[128];   // |x - y|^2
+  float4 qe[128];  // x - y of each queued site; |x - y|^2 is recomputed from it bit for bit
 };
 
 __device__ inline float row_gap(float c, int idx, uint32_t dim) {
@@ -1290,15 +1288,16 @@ __device__ inline float row_gap(float c, int idx, uint32_t dim) {
 //
 // OWN: whether the source's own orbit can lie within the cutoff.  x.g is at S3 angle
 // acos|g.w| from x for every x, so below the smallest such angle (0.63 rad for I) no own
-// image is ever a candidate: then the candidate stage needs no site codes, the only site
-// to skip is the source's own identity image (its sorted position), and the nearest chord
-// is taken over the compacted pairs.  TERMS: the arctangent series (atan_small).
+// image is ever a candidate: then the candidate stage needs no site codes and no test at
+// all beyond the cutoff, the source's own identity image (chord exactly 0) being dropped
+// when its pair is evaluated, and the nearest chord is taken over the compacted pairs.
+// TERMS: the arctangent series (atan_small).
 template <bool OWN, int TERMS>
 __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint32_t n,
                                               const uint32_t* __restrict__ order, const float4* __restrict__ sq,
                                               const uint32_t* __restrict__ scode,
                                               const uint32_t* __restrict__ offsets, const NearInfo* __restrict__ info,
-                                              const uint32_t* __restrict__ self_pos, uint32_t sdim, NearCut cut,
+                                              uint32_t sdim, NearCut cut,
                                               uint32_t real_blocks, Rows rows, uint32_t* __restrict__ counts) {
   __shared__ WaveLds lds[kBlock / 64];
   const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
@@ -1328,20 +1327,33 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   const uint32_t ny = uint32_t(yhi - ylo + 1), nrows = ny * uint32_t(zhi - zlo + 1);
   const float rc2 = rc * rc;
   const uint32_t self_code = src | 0x80000000u;
-  const uint32_t self_site = OWN ? 0u : self_pos[src];
 
   NearSum ps{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   // Chords are sums of squares (>= 0), so their bits order as unsigned integers: one
   // integer min per update, no float canonicalisation.
   uint32_t nearest_bits = __float_as_uint(4.0f);
+  // Without own images in reach (!OWN), the source's identity image is accepted like any
+  // site and dropped when its pair is evaluated: it is the site bit-identical to x (sites
+  // are rebuilt from q before every pass), so its chord is exactly 0.  A zero chord enters
+  // the pair arithmetic as chord^2 2 (S3 angle pi/2, beyond every window, so its terms
+  // vanish exactly), stays out of the nearest chord, and is counted (uniform); a second
+  // one (an exact duplicate of x) is restored at the end.
+  uint32_t zeros = 0;
   uint32_t qhead = 0, qcount = 0, rounds = 0;
   uint32_t candidates = 0;  // sites tested (uniform); accepted pairs are 64 rounds + the tail
 
   auto process = [&](uint32_t e) {
-    const float dm2 = L.qd[e];
-    if (!OWN) nearest_bits = min(nearest_bits, __float_as_uint(dm2));
-    near_pair_terms<TERMS>(L.qe[e], dm2, cut.taper_start, cut.inv_width, ps);
+    const float4 d = L.qe[e];
+    float dm2 = dot4(d, d);  // the candidate test's operations: the same bits
+    if (!OWN) {
+      const uint32_t bits = __float_as_uint(dm2);
+      const bool zero = bits == 0u;
+      zeros += uint32_t(__popcll(__builtin_amdgcn_ballot_w64(zero)));
+      nearest_bits = min(nearest_bits, zero ? 0xffffffffu : bits);
+      dm2 = zero ? 2.0f : dm2;
+    }
+    near_pair_terms<TERMS>(d, dm2, cut.taper_start, cut.inv_width, ps);
   };
   // Every 64 rounds the lanes' FP32 sums (64 pairs each) go into the wave's FP64 sums.
   auto flush = [&]() {
@@ -1374,35 +1386,38 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
       const uint32_t n_row = uint32_t(__builtin_amdgcn_readlane(int(len), int(rr)));
       if (n_row == 0) continue;
       const uint32_t s_row = uint32_t(__builtin_amdgcn_readlane(int(start), int(rr)));
+      // The row's sites and codes from a uniform base: lane offsets stay 32-bit.
+      const float4* __restrict__ row_sites = sq + s_row;
+      const uint32_t* __restrict__ row_codes = scode + s_row;
       for (uint32_t c = 0; c < n_row; c += 64) {
-        const bool valid = c + lane < n_row;
-        const uint32_t site = s_row + min(c + lane, n_row - 1);  // in range for every lane
-        const float4 y = sq[site];
-        const float4 d = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
-        const float dm2 = dot4(d, d);  // chord2(x, y), the same operations
-        const bool inside = valid && dm2 <= cut.chord2;
-        bool accept;
-        if (OWN) {
-          // Every orbit point but the source itself bounds the nearest chord; the
-          // source's own images exert no force (pairs.wgsl:404-408).
-          const uint32_t code = scode[site];
-          if (inside && code != self_code) nearest_bits = min(nearest_bits, __float_as_uint(dm2));
-          accept = inside && (code & 0x7fffffffu) != src;
-        } else {
-          accept = inside && site != self_site;
+        const uint32_t k = c + lane;
+        float4 d;
+        float dm2 = 8.0f;  // beyond every cutoff: the lanes past the row's end
+        bool own = false;  // the source's own images exert no force (pairs.wgsl:404-408)
+        if (k < n_row) {   // the lanes past the row's end load nothing
+          const float4 y = row_sites[k];
+          d = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
+          dm2 = dot4(d, d);  // chord2(x, y), the same operations
+          if (OWN) {
+            // Every orbit point but the source itself bounds the nearest chord.
+            const uint32_t code = row_codes[k];
+            if (dm2 <= cut.chord2 && code != self_code) nearest_bits = min(nearest_bits, __float_as_uint(dm2));
+            own = (code & 0x7fffffffu) == src;
+          }
         }
-        const uint64_t mask = __ballot(accept);
+        const uint64_t mask = __builtin_amdgcn_ballot_w64(dm2 <= cut.chord2 && !own);
+        const bool take = __builtin_amdgcn_inverse_ballot_w64(mask);
         if (mask) {
-          if (accept) {
+          if (take) {
             const uint32_t e = (qhead + qcount + mbcnt(mask)) & 127u;
             L.qe[e] = d;
-            L.qd[e] = dm2;
           }
           qcount = __builtin_amdgcn_readfirstlane(qcount + uint32_t(__popcll(mask)));
           wave_sync();
           if (qcount >= 64) {
-            process((qhead + lane) & 127u);
-            qhead = __builtin_amdgcn_readfirstlane((qhead + 64) & 127u);
+            // qhead is 0 or 64, so qhead + lane indexes the ring without a wrap.
+            process(qhead + lane);
+            qhead = __builtin_amdgcn_readfirstlane(qhead ^ 64u);
             qcount -= 64;
             rounds = __builtin_amdgcn_readfirstlane(rounds + 1);
             if ((rounds & 63u) == 0) flush();
@@ -1412,12 +1427,25 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
       }
     }
   }
-  if (lane < qcount) process((qhead + lane) & 127u);
+  if (lane < qcount) process(qhead + lane);
   flush();
+  uint32_t pairs = 64 * rounds + qcount;
+  uint32_t nearest_key = wave_min_u32(nearest_bits);
+  if (!OWN) {
+    if (zeros > 1) {
+      // Each duplicate of x is a pair at chord 0 (pairs.wgsl:404-408 keeps it): its
+      // terms, clamped by the pair's regularisers, with no force (x - y = 0).
+      NearSum z{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+      near_pair_terms<TERMS>(make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, cut.taper_start, cut.inv_width, z);
+      acc[4] += double(zeros - 1) * double(z.curvature);
+      acc[5] += double(zeros - 1) * double(z.energy);
+      nearest_key = 0u;
+    }
+    pairs -= min(zeros, 1u);  // the identity image is no pair
+  }
   const double g[4] = {acc[0], acc[1], acc[2], acc[3]};
   const double curvature = acc[4], e = acc[5];
-  const float nearest = __uint_as_float(wave_min_u32(nearest_bits));
-  const uint32_t pairs = 64 * rounds + qcount;
+  const float nearest = __uint_as_float(nearest_key);
   if (lane == 0) {
     double f[3];
     near_force_of(x, g, f);
@@ -1451,7 +1479,7 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
   p.run("near_force", [&] {
     auto go = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.offsets,
-                         b.info, b.self_pos, b.sdim, cut, real, out, b.counts);
+                         b.info, b.sdim, cut, real, out, b.counts);
     };
     const int terms = atan_terms(cut.chord2);
     if (cut.own) {
