@@ -1340,6 +1340,7 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   // vanish exactly), stays out of the nearest chord, and is counted (uniform); a second
   // one (an exact duplicate of x) is restored at the end.
   uint32_t zeros = 0;
+  const float u_offset = -cut.taper_start * cut.inv_width;
   uint32_t qhead = 0, qcount = 0, rounds = 0;
   uint32_t candidates = 0;  // sites tested (uniform); accepted pairs are 64 rounds + the tail
 
@@ -1347,13 +1348,13 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
     const float4 d = L.qe[e];
     float dm2 = dot4(d, d);  // the candidate test's operations: the same bits
     if (!OWN) {
-      const uint32_t bits = __float_as_uint(dm2);
-      const bool zero = bits == 0u;
+      const bool zero = dm2 == 0.0f;
       zeros += uint32_t(__popcll(__builtin_amdgcn_ballot_w64(zero)));
-      nearest_bits = min(nearest_bits, zero ? 0xffffffffu : bits);
       dm2 = zero ? 2.0f : dm2;
+      // 2 exceeds every accepted chord (< 2), so it stands for "no pair" in the minimum.
+      nearest_bits = min(nearest_bits, __float_as_uint(dm2));
     }
-    near_pair_terms<TERMS>(d, dm2, cut.taper_start, cut.inv_width, ps);
+    near_pair_terms<TERMS>(d, dm2, cut.inv_width, u_offset, ps);
   };
   // Every 64 rounds the lanes' FP32 sums (64 pairs each) go into the wave's FP64 sums.
   auto flush = [&]() {
@@ -1436,11 +1437,12 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
       // Each duplicate of x is a pair at chord 0 (pairs.wgsl:404-408 keeps it): its
       // terms, clamped by the pair's regularisers, with no force (x - y = 0).
       NearSum z{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-      near_pair_terms<TERMS>(make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, cut.taper_start, cut.inv_width, z);
+      near_pair_terms<TERMS>(make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, cut.inv_width, u_offset, z);
       acc[4] += double(zeros - 1) * double(z.curvature);
       acc[5] += double(zeros - 1) * double(z.energy);
       nearest_key = 0u;
     }
+    if (nearest_key == __float_as_uint(2.0f)) nearest_key = __float_as_uint(4.0f);  // no pair at all
     pairs -= min(zeros, 1u);  // the identity image is no pair
   }
   const double g[4] = {acc[0], acc[1], acc[2], acc[3]};

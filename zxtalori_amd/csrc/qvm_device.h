@@ -152,9 +152,12 @@ struct NearSum {
 // The near-field pair for a site y within the cutoff, from e = x - y and dm2 = |e|^2:
 // dm2 <= chord2 < 2, so y is the nearer of +-y (sign +) and dp2 = 4 - dm2 for unit
 // quaternions; then near + far = 4 and 1/sin a = 2 rsq(near far).  Otherwise the
-// arithmetic of pair_terms<true> (pairs.wgsl:246-262, :349-361).
+// arithmetic of pair_terms<true> (pairs.wgsl:246-262, :349-361), with two changes of
+// form: the window coordinate (a - taper_start) / width is one fused multiply-add,
+// a * inv_width + u_offset with u_offset = -taper_start * inv_width (one rounding instead
+// of two), and the factor 2 of the slope term rides on its constant (exact).
 template <int TERMS>
-__device__ __forceinline__ void near_pair_terms(const float4 e, float dm2, float taper_start, float inv_width,
+__device__ __forceinline__ void near_pair_terms(const float4 e, float dm2, float inv_width, float u_offset,
                                                 NearSum& s) {
   const float nr = __builtin_amdgcn_fmed3f(dm2, 1e-30f, 4.0f);
   const float fr = 4.0f - dm2;
@@ -173,10 +176,10 @@ __device__ __forceinline__ void near_pair_terms(const float4 e, float dm2, float
   const float D = __builtin_amdgcn_rcpf(P12 * P3);
   const float iP1 = D * (P2 * P3), iP2 = D * (P1 * P3), iP3 = D * P12;
   const float energy = kPiF * iP3;
-  const float u = fminf(fmaxf((a - taper_start) * inv_width, 0.0f), 1.0f);
+  const float u = fminf(fmaxf(fmaf(a, inv_width, u_offset), 0.0f), 1.0f);
   const float w = fmaf(u * u, fmaf(2.0f, u, -3.0f), 1.0f);
-  const float slope = -6.0f * inv_width * (u * (1.0f - u));
-  const float coef = fmaf(w, (a2 - b2) * iP1, energy * slope * (2.0f * R));
+  const float slope2 = -12.0f * inv_width * (u * (1.0f - u));  // twice the window's slope
+  const float coef = fmaf(w, (a2 - b2) * iP1, energy * slope2 * R);
   s.gx = fmaf(coef, e.x, s.gx);
   s.gy = fmaf(coef, e.y, s.gy);
   s.gz = fmaf(coef, e.z, s.gz);
