@@ -3,7 +3,7 @@
 Workload (the configuration the metric is quoted on): group I, the first 1,000,000 of
 zo.haar("I", 10_000_000, seed 193), neighbourhood 20 (about 8,000 orbit points per
 source).  A step is one near-field energy/force pass over the whole sample: shell of
-images near the zone, cell sort of the sites, Morton sort of the sources, the pair
+images near the zone, cell sort of the sites, source order from the sorted sites, the pair
 kernel.  Positions are resident in HBM before the timed region.
 
 value = accepted pair evaluations per second over the timed passes (wall clock around

@@ -117,9 +117,6 @@ NearBuffers NearField::view() const {
   b.srank = srank.as<uint32_t>();
   b.offsets = offsets.as<uint32_t>();
   b.cursor = cursor.as<uint32_t>();
-  b.src_cell = src_cell.as<uint32_t>();
-  b.src_offsets = src_offsets.as<uint32_t>();
-  b.src_cursor = src_cursor.as<uint32_t>();
   b.order = order_buf.as<uint32_t>();
   b.scan_tmp = scan_tmp.as<uint32_t>();
   b.counters = counters.as<unsigned long long>();
@@ -133,7 +130,7 @@ NearBuffers NearField::view() const {
 
 static void release_near(NearField& f) {
   for (DevBuf* b : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
-                    &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor, &f.order_buf,
+                    &f.srank, &f.offsets, &f.cursor, &f.order_buf,
                     &f.scan_tmp, &f.counters, &f.counts, &f.crowded, &f.huge})
     b->release();
   f.ready = false;
@@ -222,18 +219,15 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   uq.ensure(size_t(capacity) * 16);
   ucode.ensure(size_t(capacity) * 4);
   urank.ensure(size_t(capacity) * 4);
-  ucell.ensure(size_t(capacity) * 4);
+  ucell.ensure((size_t(capacity) + 1) * 4);
   perm.ensure(size_t(capacity) * 4);
   sq.ensure(size_t(capacity) * 16);
   scode.ensure(size_t(capacity) * 4);
   srank.ensure(size_t(capacity) * 4);
   offsets.ensure((cells + 1) * 4);
   cursor.ensure((cells + 1) * 4);
-  src_cell.ensure(size_t(n) * 4);
-  src_offsets.ensure((size_t(kMortonCells) + 1) * 4);
-  src_cursor.ensure((size_t(kMortonCells) + 1) * 4);
   order_buf.ensure(size_t(n) * 4);
-  scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, kMortonCells)) * 4);
+  scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, capacity)) * 4);
   counters.ensure(4 * sizeof(unsigned long long));
   counts.ensure(size_t(n) * 8);
   crowded.ensure(size_t(crowded_capacity(capacity)) * 4);
@@ -817,7 +811,7 @@ size_t Engine::device_bytes() const {
       b += d->bytes;
     const NearField& f = rs->near;
     for (const DevBuf* d : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
-                            &f.srank, &f.offsets, &f.cursor, &f.src_cell, &f.src_offsets, &f.src_cursor,
+                            &f.srank, &f.offsets, &f.cursor,
                             &f.order_buf, &f.scan_tmp, &f.counters, &f.counts, &f.crowded, &f.huge})
       b += d->bytes;
   }

@@ -122,8 +122,8 @@ struct NearField {
   float band = 0;
   // The last pass's sites, shell and overflow flag (valid after a wait).
   NearInfo status{};
-  DevBuf planes, info, rim, uq, ucode, urank, ucell, perm, sq, scode, srank, offsets, cursor, src_cell, src_offsets,
-      src_cursor, order_buf, scan_tmp, counters, counts, crowded, huge;
+  DevBuf planes, info, rim, uq, ucode, urank, ucell, perm, sq, scode, srank, offsets, cursor, order_buf,
+      scan_tmp, counters, counts, crowded, huge;
   NearBuffers view() const;
   void setup(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius);
   // Records sites and sorts for the positions q (setting up first when needed).

@@ -183,18 +183,15 @@ struct NearBuffers {
   float4* uq = nullptr;          // capacity, unsorted sites
   uint32_t* ucode = nullptr;     // capacity
   uint32_t* urank = nullptr;     // capacity
-  uint32_t* ucell = nullptr;     // capacity
+  uint32_t* ucell = nullptr;     // capacity + 1: each site's cell; then the core flags' scan
   uint32_t* perm = nullptr;      // capacity
   float4* sq = nullptr;          // capacity, sorted by cell
   uint32_t* scode = nullptr;     // capacity
   uint32_t* srank = nullptr;     // capacity
   uint32_t* offsets = nullptr;   // sdim^3 + 1
   uint32_t* cursor = nullptr;    // sdim^3 + 1
-  uint32_t* src_cell = nullptr;  // n
-  uint32_t* src_offsets = nullptr;  // kMortonCells + 1
-  uint32_t* src_cursor = nullptr;   // kMortonCells + 1
-  uint32_t* order = nullptr;     // n, sources in Morton order
-  uint32_t* scan_tmp = nullptr;  // scan_tmp_words(max cells)
+  uint32_t* order = nullptr;     // n, sources in the cell order of their identity images
+  uint32_t* scan_tmp = nullptr;  // scan_tmp_words(max(cells, capacity))
   unsigned long long* counters = nullptr;  // [0] accepted pairs, [1] sites tested (force pass)
   uint32_t* counts = nullptr;    // 2n: per source, pairs and sites tested of the last force pass
   uint32_t* crowded = nullptr;   // crowded_capacity: cells with more sites than one thread sorts
@@ -208,15 +205,13 @@ inline uint32_t crowded_capacity(uint32_t capacity) { return capacity / 17 + 1; 
 inline uint32_t huge_capacity(uint32_t capacity) { return capacity / 1025 + 1; }
 constexpr uint32_t kCrowdedBlocks = 512;
 constexpr uint32_t kHugeBlocks = 64;
-constexpr uint32_t kMortonBits = 6;
-constexpr uint32_t kMortonCells = 1u << (3 * kMortonBits);
 size_t scan_tmp_words(size_t count);
 
 // Writes the sites (core + shell) and their chart box only (sizing pass).
 void launch_near_sites(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                        const NearBuffers& b);
-// Builds the sites (core + shell), sorts them into the cell grid and the sources into
-// Morton order.  Reads nothing back.
+// Builds the sites (core + shell), sorts them into the cell grid and orders the sources
+// by the cell order of their identity images.  Reads nothing back.
 void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                        const NearBuffers& b);
 // Force rows (mode 0) or nearest orbit point with rank (mode 1) over the built sites.

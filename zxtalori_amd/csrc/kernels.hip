@@ -19,7 +19,8 @@
 //    64 candidates at a time.  Candidates within the cutoff are compacted through an LDS
 //    ring (ballot + mbcnt), so the ~100-op pair kernel always runs on full waves; the
 //    accepted order is deterministic, and so are the sums.  Workgroups are remapped so each
-//    XCD takes a contiguous Morton range of sources and its L2 holds their sites.
+//    XCD takes a contiguous range of sources (in the cell order of their identity
+//    images) and its L2 holds their sites.
 //  * Sorts are counting sorts (atomic counts, a three-kernel scan, atomic scatter, then an
 //    in-cell sort by a unique key), so every pass is deterministic.
 #include <hip/hip_runtime.h>
@@ -1175,41 +1176,33 @@ __global__ __launch_bounds__(256) void k_cell_sort_block(const uint32_t* __restr
   }
 }
 
+// Sites in cell order; ucell (free once the scatter has run) takes each slot's core flag,
+// zeros past the last site, for the scan that orders the sources.
 __global__ void k_site_gather(NearBuffers b) {
   const uint32_t S = min(b.info->sites, b.capacity);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < b.capacity; k += gridDim.x * blockDim.x) {
+    uint32_t core = 0;
+    if (k < S) {
+      const uint32_t s = b.perm[k];
+      const uint32_t code = b.ucode[s];
+      b.sq[k] = b.uq[s];
+      b.scode[k] = code;
+      b.srank[k] = b.urank[s];
+      core = code >> 31;
+    }
+    b.ucell[k] = core;
+  }
+}
+
+// The sources in the cell order of their identity images (the core sites): waves of
+// neighbouring sources read overlapping site runs, and each XCD takes a contiguous range
+// of them (xcd_remap).  ucell holds the exclusive scan of the core flags.
+__global__ void k_core_order(NearBuffers b) {
+  const uint32_t S = min(b.info->sites, b.capacity);
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x) {
-    const uint32_t s = b.perm[k];
-    const uint32_t code = b.ucode[s];
-    b.sq[k] = b.uq[s];
-    b.scode[k] = code;
-    b.srank[k] = b.urank[s];
+    const uint32_t code = b.scode[k];
+    if (code >> 31) b.order[b.ucell[k]] = code & 0x7fffffffu;
   }
-}
-
-__device__ inline uint32_t spread6(uint32_t v) {
-  uint32_t x = v & 0x3fu;
-  x = (x | (x << 8)) & 0x0000f00fu;
-  x = (x | (x << 4)) & 0x000c30c3u;
-  x = (x | (x << 2)) & 0x00249249u;
-  return x;
-}
-
-__global__ void k_src_count(const float4* __restrict__ q, uint32_t n, NearBuffers b) {
-  const float lx = b.info->lower[0], ly = b.info->lower[1], lz = b.info->lower[2];
-  const float cpu = b.info->cpu * float(1u << kMortonBits) / float(b.sdim);
-  const uint32_t d = 1u << kMortonBits;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float3 p = chart(q[i]);
-    const uint32_t c = spread6(cell_axis(p.x, lx, cpu, d)) | (spread6(cell_axis(p.y, ly, cpu, d)) << 1) |
-                       (spread6(cell_axis(p.z, lz, cpu, d)) << 2);
-    b.src_cell[i] = c;
-    atomicAdd(&b.src_offsets[c], 1u);
-  }
-}
-
-__global__ void k_src_scatter(uint32_t n, NearBuffers b) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    b.order[atomicAdd(&b.src_cursor[b.src_cell[i]], 1u)] = i;
 }
 
 static uint32_t stride_grid(size_t work) { return uint32_t(std::min<size_t>(8192, (work + kBlock - 1) / kBlock + 1)); }
@@ -1231,7 +1224,6 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
                        const NearBuffers& b) {
   const size_t cells = size_t(b.sdim) * b.sdim * b.sdim;
   launch_fill_u32(p, s, b.offsets, 0u, cells + 1);
-  launch_fill_u32(p, s, b.src_offsets, 0u, kMortonCells + 1);
   launch_near_sites(p, s, q, n, ops, order, b);
   const uint32_t g = stride_grid(b.capacity);
   p.run("site_count", [&] { hipLaunchKernelGGL(k_site_count, dim3(g), dim3(kBlock), 0, s, b); });
@@ -1251,16 +1243,9 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
     }
   });
   p.run("site_gather", [&] { hipLaunchKernelGGL(k_site_gather, dim3(g), dim3(kBlock), 0, s, b); });
-  const uint32_t gn = stride_grid(n);
-  p.run("src_count", [&] { hipLaunchKernelGGL(k_src_count, dim3(gn), dim3(kBlock), 0, s, q, n, b); });
-  launch_scan(p, s, b.src_offsets, kMortonCells, b.scan_tmp);
-  p.run("copy", [&] {
-    hipLaunchKernelGGL(k_copy_u32, dim3(stride_grid(kMortonCells)), dim3(kBlock), 0, s, b.src_offsets, b.src_cursor,
-                       size_t(kMortonCells));
-  });
-  p.run("src_scatter", [&] { hipLaunchKernelGGL(k_src_scatter, dim3(gn), dim3(kBlock), 0, s, n, b); });
-  // Sources stay in the scatter's order within a Morton cell: the order only schedules the
-  // waves, each source's row being its own.
+  launch_scan(p, s, b.ucell, b.capacity, b.scan_tmp);
+  p.run("core_order", [&] { hipLaunchKernelGGL(k_core_order, dim3(g), dim3(kBlock), 0, s, b); });
+  // The order only schedules the waves: each source's row is its own.
 }
 
 // ---- near-field rows (pairs.wgsl:363-427) -------------------------------------------
