@@ -126,8 +126,7 @@ uint32_t zo_hopf_points(uint32_t group, uint32_t f, float* out, uint32_t capacit
   const uint32_t n = uint32_t(zxt::hopf::raw_count(group, f));
   if (!out || capacity < n) return n;
   try {
-    const std::vector<float> rows = zxt::hopf::points(group, f);
-    std::memcpy(out, rows.data(), rows.size() * sizeof(float));
+    zxt::hopf::points(group, f, out);
     return n;
   } catch (...) {
     return 0;

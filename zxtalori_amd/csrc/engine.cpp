@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <unordered_map>
 
 #include "groups_table.h"
@@ -481,13 +482,15 @@ void Engine::generate(uint32_t group, uint32_t h, uint32_t target, uint32_t seed
     return;
   }
   const uint32_t f = h == 0 ? hopf::nearest_frequency(group, target == 0 ? 2048 : target) : h;
-  const std::vector<float> rows = hopf::points(group, f);
-  const uint32_t n = uint32_t(rows.size() / 4);
+  const uint32_t n = hopf::count(group, f);
+  // Written by the generator's threads, so no zero fill first.
+  const std::unique_ptr<float[]> rows(new float[size_t(n) * 4]);
+  hopf::points(group, f, rows.get());
   const uint32_t old = group_;
   set_group(group);
   try {
     q_.ensure(size_t(n) * 16);
-    check(hipMemcpyAsync(q_.p, rows.data(), size_t(n) * 16, hipMemcpyHostToDevice, stream), "upload");
+    check(hipMemcpyAsync(q_.p, rows.get(), size_t(n) * 16, hipMemcpyHostToDevice, stream), "upload");
     // Grid points are orbit images: the fold picks each one's zone representative.
     fold(n);
     adopt(n);

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 
@@ -36,6 +37,25 @@ enum Id : uint32_t { C2, C3, C4, C6, D2, D3, D4, D6, T, O, I, C1 };
 constexpr uint32_t kOrder[] = {2, 3, 4, 6, 4, 6, 8, 12, 12, 24, 60, 1};
 
 [[noreturn]] void fail(const char* what) { throw EngineError{what}; }
+
+// Runs body(begin, end) over [0, n) split into contiguous ranges, one per host thread (at
+// most 16); every index is computed by the same code whichever thread takes it, so the
+// result does not depend on the split.  Bodies must not throw.
+template <class Body>
+void parallel_ranges(size_t n, const Body& body) {
+  const unsigned threads = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (threads == 1 || n < 8192) {
+    body(size_t(0), n);
+    return;
+  }
+  const size_t chunk = (n + threads - 1) / threads;
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < threads; ++t) {
+    const size_t a = size_t(t) * chunk, b = std::min(n, a + chunk);
+    if (a < b) pool.emplace_back([&body, a, b] { body(a, b); });
+  }
+  for (std::thread& t : pool) t.join();
+}
 
 Quat operator*(const Quat& a, const Quat& b) {
   return {a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3],
@@ -278,7 +298,11 @@ Mesh polyhedral_mesh(uint32_t root, uint32_t f) {
   std::unordered_set<uint64_t> seen;
   seen.reserve(2 * size_t(total));
   const double scale = 1.0 / double(f);
-  for (size_t fi = 0; fi < nfaces; ++fi) {
+  // First the nodes in order of first occurrence (face, i, j), then their positions on the
+  // sphere (the area map, the costly part) on all host threads.
+  std::vector<std::array<uint32_t, 3>> site;
+  site.reserve(total);
+  for (uint32_t fi = 0; fi < nfaces; ++fi) {
     const uint8_t* face = ico ? kIcoFaces[fi] : kOctaFaces[fi];
     for (uint32_t i = 0; i <= f; ++i) {
       for (uint32_t j = 0; j + i <= f; ++j) {
@@ -292,18 +316,27 @@ Mesh polyhedral_mesh(uint32_t root, uint32_t f) {
         uint64_t key = uint64_t(np);
         for (int k = 0; k < np; ++k) key = key << 14 | pairs[k];
         if (!seen.insert(key).second) continue;
-        if (m.nodes.size() == total) fail("BaseMeshCount");
-        Vec x{0, 0, 0};
-        for (int k = 0; k < 3; ++k) {
-          const double b = double(w[k]) * scale;
-          for (int e = 0; e < 3; ++e) x[e] += b * v[face[k]][e];
-        }
-        m.nodes.push_back(area(unit(x)));
+        if (site.size() == total) fail("BaseMeshCount");
+        site.push_back({fi, i, j});
         m.layers.push_back(f - std::max(w[0], std::max(w[1], w[2])));
       }
     }
   }
-  if (m.nodes.size() != total) fail("BaseMeshCount");
+  if (site.size() != total) fail("BaseMeshCount");
+  m.nodes.resize(total);
+  parallel_ranges(total, [&](size_t a, size_t b) {
+    for (size_t k = a; k < b; ++k) {
+      const uint8_t* face = ico ? kIcoFaces[site[k][0]] : kOctaFaces[site[k][0]];
+      const uint32_t i = site[k][1], j = site[k][2];
+      const uint32_t w[3] = {i, j, f - i - j};
+      Vec x{0, 0, 0};
+      for (int c = 0; c < 3; ++c) {
+        const double bw = double(w[c]) * scale;
+        for (int e = 0; e < 3; ++e) x[e] += bw * v[face[c]][e];
+      }
+      m.nodes[k] = area(unit(x));
+    }
+  });
   return m;
 }
 
@@ -504,7 +537,7 @@ const char* description(uint32_t group) {
   }
 }
 
-std::vector<float> points(uint32_t group, uint32_t f) {
+void points(uint32_t group, uint32_t f, float* out) {
   if (group > C1) fail("UnknownGroup");
   const uint32_t total = count(group, f);
   const uint32_t root = root_of(group);
@@ -544,9 +577,15 @@ std::vector<float> points(uint32_t group, uint32_t f) {
   const NodeIndex index(mesh.nodes);
   const Sector sector(root);
   std::vector<bool> seen(mesh.nodes.size(), false);
-  std::vector<float> out;
-  out.reserve(size_t(total) * 4);
-  std::vector<Quat> heads(cosets.size());
+  // One record per orbit in node order: its section, spin phase and turns; the rows are
+  // written afterwards on all host threads, each orbit at its offset.
+  struct Orbit {
+    Quat s;
+    uint32_t phase, turns;
+    size_t first;
+  };
+  std::vector<Orbit> orbits;
+  size_t rows = 0;
   for (size_t i = 0; i < mesh.nodes.size(); ++i) {
     if (seen[i]) continue;
     const Vec& v = mesh.nodes[i];
@@ -570,19 +609,30 @@ std::vector<float> points(uint32_t group, uint32_t f) {
     }
     if (!have) fail("OrbitOutsideSector");
     if (n % stabiliser != 0) fail("FibreNotDivisible");
-    const Quat s = section(rep);
-    for (size_t k = 0; k < cosets.size(); ++k) heads[k] = s * tails[k];
-    const Quat* row = &spins[size_t(mesh.layers[i] % 2) * n];
     const uint32_t turns = n / stabiliser;
-    if (out.size() / 4 + size_t(turns) * cosets.size() > total) fail("ProductCount");
-    for (uint32_t j = 0; j < turns; ++j)
-      for (const Quat& h : heads) {
-        const Quat q = row[j] * h;
-        out.insert(out.end(), {float(q[0]), float(q[1]), float(q[2]), float(q[3])});
-      }
+    if (rows + size_t(turns) * cosets.size() > total) fail("ProductCount");
+    orbits.push_back({section(rep), mesh.layers[i] % 2, turns, rows});
+    rows += size_t(turns) * cosets.size();
   }
-  if (out.size() / 4 != total) fail("ProductCount");
-  return out;
+  if (rows != total) fail("ProductCount");
+  parallel_ranges(orbits.size(), [&](size_t a, size_t b) {
+    std::vector<Quat> heads(cosets.size());
+    for (size_t o = a; o < b; ++o) {
+      const Orbit& orb = orbits[o];
+      for (size_t k = 0; k < cosets.size(); ++k) heads[k] = orb.s * tails[k];
+      const Quat* row = &spins[size_t(orb.phase) * n];
+      float* dst = out + orb.first * 4;
+      for (uint32_t j = 0; j < orb.turns; ++j)
+        for (const Quat& h : heads) {
+          const Quat q = row[j] * h;
+          dst[0] = float(q[0]);
+          dst[1] = float(q[1]);
+          dst[2] = float(q[2]);
+          dst[3] = float(q[3]);
+          dst += 4;
+        }
+    }
+  });
 }
 
 }  // namespace zxt::hopf

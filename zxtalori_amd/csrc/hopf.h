@@ -33,9 +33,10 @@ const char* description(uint32_t group);
 // decreasing scalar part, stably (hopf.zig:161-213).
 std::vector<std::array<double, 4>> elements(uint32_t group);
 
-// The frequency-f grid as FP32 rows, scalar first, 4 floats per point; each point is one
-// image of its orbit (the engine's fold picks the zone representative).  Throws
-// EngineError on invalid input.
-std::vector<float> points(uint32_t group, uint32_t f);
+// Writes the frequency-f grid to out as count(group, f) FP32 rows, scalar first, 4 floats
+// per point (out holds that many); each point is one image of its orbit (the engine's fold
+// picks the zone representative).  Computed on all host threads, the rows independent of
+// the split.  Throws EngineError on invalid input.
+void points(uint32_t group, uint32_t f, float* out);
 
 }  // namespace zxt::hopf
