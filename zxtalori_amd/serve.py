@@ -132,24 +132,27 @@ class _Lane:
 class Server:
     """Admits requests by HBM budget into batches and steps them until all are served.
 
-    Requests go to a lane by size: jobs whose steps cost more than `large_pairs` pair
-    evaluations (a near-field N of about 270,000 at neighbourhood 20) share the large lane,
-    the rest the small lane.  Each lane is a Batch stepped by its own host thread and kept
-    at most `ahead` steps ahead of the device, so the lanes' kernels overlap on the GPU
-    and a small job does not wait for a large job's steps.  Jobs within a lane advance
-    together, one BB step per lane step.  A large job fills the device by itself, so the
+    Requests go to a lane by the pair evaluations one of their steps costs: up to
+    `bounds[0]` the small lane, up to `bounds[1]` the medium lane (a near-field N of about
+    270,000 at neighbourhood 20), beyond it the large lane.  Each lane is a Batch stepped
+    by its own host thread and kept at most `ahead` steps ahead of the device, so the
+    lanes' kernels overlap on the GPU and a small job does not wait for a larger job's
+    steps.  Jobs within a lane advance together, one BB step per lane step, so a lane's
+    step lasts as long as its jobs' steps together: lanes of similar costs keep a job from
+    being slowed by much costlier ones.  A large job fills the device by itself, so the
     large lane runs one at a time, in arrival order: sharing steps among several would
     only delay the first to finish.
     """
 
-    def __init__(self, hbm_budget=64 << 30, *, graphs=True, keep_results=False, large_pairs=2e9, ahead=1):
+    def __init__(self, hbm_budget=64 << 30, *, graphs=True, keep_results=False, bounds=(2e8, 2e9), ahead=1):
         self.hbm_budget = int(hbm_budget)
         self.keep_results = keep_results
-        self.large_pairs = float(large_pairs)
+        self.bounds = tuple(float(b) for b in bounds)
         self.ahead = int(ahead)
-        # The large lane's passes fill the device alone; the small lane spreads its jobs'
+        # The large lane's passes fill the device alone; the others spread their jobs'
         # near-field passes over two extra streams.
-        self.lanes = [_Lane("small", graphs, 2), _Lane("large", graphs, 0, limit=1)]
+        self.lanes = [_Lane(f"lane{k}", graphs, 2) for k in range(len(self.bounds))]
+        self.lanes.append(_Lane("large", graphs, 0, limit=1))
         self.peak_bytes = 0
         self.max_live = 0
         self._lock = threading.Lock()
@@ -182,7 +185,10 @@ class Server:
             per_step = req.count * 7500.0 if req.neighbourhood > 0 else req.count ** 2 * ORDERS[req.group.upper()]
         else:
             per_step = req.count * 7500.0
-        return self.lanes[1] if per_step > self.large_pairs else self.lanes[0]
+        for lane, bound in zip(self.lanes, self.bounds):
+            if per_step <= bound:
+                return lane
+        return self.lanes[-1]
 
     def run(self, requests, *, realtime=True, clock=time.perf_counter):
         """Serves `requests`; with realtime=False every request counts as arrived at the
