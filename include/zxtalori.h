@@ -67,6 +67,8 @@ int32_t zo_identity(zxt_engine* e, uint32_t flags);
 /* src/exports.zig:73-81 */
 const char* zo_stats(zxt_engine* e);
 const char* zo_error(zxt_engine* e);
+/* Any thread: a running zo_relax or zo_cover_optimize stops between steps (the relax waits
+ * for the device every ~2e10 pair evaluations, as engine.zig:896-915 does). */
 void zo_cancel(zxt_engine* e);
 
 /* src/exports.zig:45-48 (engine.zig:378-398): the covering radius of the sample, the

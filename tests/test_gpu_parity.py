@@ -406,3 +406,26 @@ def test_near_field_large_sample(operator_fixture):
     assert gpu_err <= 2e-6 and gpu_err <= 2 * cpu_err
     assert (n1 > 0).all()
     assert 7000 < stats["accepted"] / 2000 < 9000
+
+
+def test_cancel_stops_a_long_relaxation():
+    """zo_cancel (exports.zig:79-81) from another thread ends relax between steps: the engine
+    waits every ~2e10 pair evaluations, so the flag is seen mid-call (engine.zig:896-915)."""
+    import threading
+    import time
+    note("cancel: I N=1e6, 1000 steps requested")
+    with zo.haar("I", 1_000_000, 193) as grid:
+        grid.relax(1, neighbourhood=20)
+        timer = threading.Timer(0.5, lambda: grid._lib.zo_cancel(grid._handle))
+        t0 = time.perf_counter()
+        timer.start()
+        grid.relax(1000)
+        elapsed = time.perf_counter() - t0
+        timer.join()
+        done = grid.stats["iteration"] - 1
+        assert 0 < done < 1000, done
+        assert elapsed < 5.0, elapsed
+        # The stopped sample is a normal one: measured, and the next call runs in full.
+        assert grid.stats["separation_deg"] > 0
+        grid.relax(2)
+        assert grid.stats["iteration"] == done + 3

@@ -752,6 +752,13 @@ void Engine::relax(uint32_t steps) {
   begin_relax();
   bool recorded = false;
   const uint32_t count = std::min<uint32_t>(steps, 1000);
+  // Steps are recorded without reading anything back, but every sync_every steps (about
+  // 2e10 pair evaluations, some tens of ms) the host waits for the device, so zo_cancel
+  // from another thread stops a long call between steps (engine.zig:896-915).
+  const double all = double(n_) * double(order());
+  const double neighbours =
+      neighbourhood_ > 0 ? std::min(all, 0.94 * neighbourhood_ * neighbourhood_ * neighbourhood_ + 64.0) : all;
+  const uint32_t sync_every = uint32_t(std::max(1.0, std::floor(2e10 / (double(n_) * neighbours))));
   for (uint32_t index = 0; index < count; ++index) {
     if (cancelled.load()) break;
     JobDesc ap;
@@ -762,6 +769,7 @@ void Engine::relax(uint32_t steps) {
     launch_fold(prof, stream, one);
     stepped();
     recorded = true;
+    if ((index + 1) % sync_every == 0 && index + 1 < count) sync();
   }
   finish_relax(recorded);
 }
