@@ -109,6 +109,9 @@ bool zo_profile(zxt_engine* e, bool enabled);
 size_t zo_profile_report(zxt_engine* e, char* output, size_t capacity);
 /* Scales the all-pairs work split (tests shrink it: only rounding may change). */
 void zo_pair_budget(zxt_engine* e, float scale);
+/* src/native_api.zig:37-39: picks among the reference's Delaunay schedulers.  This build
+ * has one covering method (per-point Voronoi cells), so the mode is accepted and ignored. */
+void zo_delaunay_mode(zxt_engine* e, uint32_t mode);
 /* No counterpart in the reference: the autotuner's hook (zxtalori_amd/tune.py).  Sets, for
  * every engine of the process, the all-pairs force pass's tile plan: about `target`
  * workgroups, `ops_per_split` operators per workgroup (0: automatic); target 0 returns to the

@@ -176,3 +176,20 @@ def test_mesh_stars_are_closed():
         assert (counts == 2).all(), np.bincount(counts)
         # Every sample point has a star.
         assert set(np.unique(tets[(tets < n).any(axis=1)])) >= set(range(n))
+
+
+@pytest.mark.parametrize("group", GROUPS)
+def test_scheduler_mode_is_accepted(group):
+    """tests/test_gao_covering.py:10-21: with the reference's scheduler mode 32 set, the
+    covering still matches the full-orbit hull before and after three steps."""
+    import ctypes
+    kind = "transport" if group in ("T", "O", "I") else None
+    with zo.generate(group, h=2, count=100, kind=kind) as grid:
+        grid._lib.zo_delaunay_mode.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        grid._lib.zo_delaunay_mode(grid._handle, 32)
+        for step in (0, 3):
+            if step:
+                grid.relax(step)
+            measured = grid.covering()
+            assert abs(measured["covering_radius_deg"] - full_orbit_covering(grid.quaternions, grid.operations)) < 2e-4
+            assert measured["covering_current"] and np.isfinite(measured["covering_witness"]).all()

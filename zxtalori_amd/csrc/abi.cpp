@@ -214,6 +214,7 @@ size_t zo_profile_report(zxt_engine* e, char* output, size_t capacity) {
   return n;
 }
 void zo_pair_budget(zxt_engine* e, float scale) { e->engine.pair_budget_scale = double(scale); }
+void zo_delaunay_mode(zxt_engine*, uint32_t) {}
 void zo_tune_allpairs(uint32_t target, uint32_t ops_per_split) { zxt::set_allpairs_override(target, ops_per_split); }
 
 const uint32_t* zo_point_indices(zxt_engine* e) { return e->engine.point_indices(); }
