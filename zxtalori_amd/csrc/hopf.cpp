@@ -50,9 +50,15 @@ void parallel_ranges(size_t n, const Body& body) {
   }
   const size_t chunk = (n + threads - 1) / threads;
   std::vector<std::thread> pool;
+  pool.reserve(threads);
   for (unsigned t = 0; t < threads; ++t) {
     const size_t a = size_t(t) * chunk, b = std::min(n, a + chunk);
-    if (a < b) pool.emplace_back([&body, a, b] { body(a, b); });
+    if (a >= b) continue;
+    try {
+      pool.emplace_back([&body, a, b] { body(a, b); });
+    } catch (...) {
+      body(a, b);  // no thread to be had: this range on the calling thread
+    }
   }
   for (std::thread& t : pool) t.join();
 }
