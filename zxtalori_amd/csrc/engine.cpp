@@ -1095,14 +1095,31 @@ std::vector<float> Engine::cap_centres(uint32_t wanted) {
     const float* o = kOperators[kGroupOffset[group_] + k];
     ops[k] = unit4({o[0], o[1], o[2], o[3]});
   }
-  struct Candidate {
-    Q4 q;
-    double radius;
+  // Candidates largest radius first, ties in vertex order (a stable sort's order), produced
+  // in growing sorted prefixes: only the first few thousand of the millions of vertices are
+  // ever visited, and only those are carried into the zone.
+  const size_t total = cover_vq_host_.size();
+  if (total == 0) fail("NoEmptyDelaunayCentres");
+  std::vector<uint32_t> rank(total);
+  for (size_t i = 0; i < total; ++i) rank[i] = uint32_t(i);
+  const float* vr = cover_vr_host_.data();
+  const auto before = [vr](uint32_t a, uint32_t b) { return vr[a] > vr[b] || (vr[a] == vr[b] && a < b); };
+  size_t sorted = 0;
+  const auto extend = [&](size_t upto) {
+    upto = std::min(total, upto);
+    if (upto <= sorted) return;
+    if (upto < total) std::nth_element(rank.begin() + long(sorted), rank.begin() + long(upto), rank.end(), before);
+    std::sort(rank.begin() + long(sorted), rank.begin() + long(upto), before);
+    sorted = upto;
   };
-  std::vector<Candidate> cands;
-  cands.reserve(cover_vq_host_.size());
-  double largest = 0.0;
-  for (size_t i = 0; i < cover_vq_host_.size(); ++i) {
+  extend(size_t(wanted) * 16 + 4096);
+  const double largest = double(vr[rank[0]]);
+  SphereHash occupied(2.0 * std::sin(0.5 * largest));
+  for (uint32_t i = 0; i < n_; ++i) occupied.add(unit4({host_[4 * i], host_[4 * i + 1], host_[4 * i + 2], host_[4 * i + 3]}));
+  std::vector<float> out;
+  for (size_t at = 0; at < total && out.size() / 4 < wanted; ++at) {
+    if (at == sorted) extend(2 * sorted);
+    const uint32_t i = rank[at];
     const float4 v = cover_vq_host_[i];
     const Q4 c = unit4({v.x, v.y, v.z, v.w});
     // The zone's image: c g^-1 with the largest |scalar part| = |c . g|.
@@ -1119,27 +1136,19 @@ std::vector<float> Engine::cap_centres(uint32_t wanted) {
     Q4 r = mul4(c, {g[0], -g[1], -g[2], -g[3]});
     if (r[0] < 0)
       for (double& x : r) x = -x;
-    cands.push_back({unit4(r), double(cover_vr_host_[i])});
-    largest = std::max(largest, double(cover_vr_host_[i]));
-  }
-  std::stable_sort(cands.begin(), cands.end(), [](const Candidate& a, const Candidate& b) { return a.radius > b.radius; });
-  SphereHash occupied(2.0 * std::sin(0.5 * largest));
-  for (uint32_t i = 0; i < n_; ++i) occupied.add(unit4({host_[4 * i], host_[4 * i + 1], host_[4 * i + 2], host_[4 * i + 3]}));
-  std::vector<float> out;
-  for (const Candidate& c : cands) {
-    if (out.size() / 4 >= wanted) break;
-    if (occupied.near(c.q, 1.0 - 1e-12)) continue;  // the same centre from another cell
-    const double height = std::cos(c.radius);
+    const Q4 q = unit4(r);
+    if (occupied.near(q, 1.0 - 1e-12)) continue;  // the same centre from another cell
+    const double height = std::cos(double(vr[i]));
     bool empty = true;
-    for (const Q4& g : ops) {
-      if (occupied.near(mul4(c.q, {g[0], -g[1], -g[2], -g[3]}), height + 3e-7)) {
+    for (const Q4& h : ops) {
+      if (occupied.near(mul4(q, {h[0], -h[1], -h[2], -h[3]}), height + 3e-7)) {
         empty = false;
         break;
       }
     }
     if (!empty) continue;
-    occupied.add(c.q);
-    for (double x : c.q) out.push_back(float(x));
+    occupied.add(q);
+    for (double x : q) out.push_back(float(x));
   }
   if (out.empty()) fail("NoEmptyDelaunayCentres");
   return out;
