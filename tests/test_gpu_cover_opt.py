@@ -49,6 +49,10 @@ def test_optimizer_holds_fixed_identity_and_stops_on_best():
         assert len(calls) == 3 and stats["covering_optimizer_steps"] == 150
         np.testing.assert_array_equal(grid.quaternions[index], [1, 0, 0, 0])
         assert stats["covering_radius_deg"] == pytest.approx(stats["covering_optimizer_best_deg"], abs=1e-5)
+        # The reference's local re-tessellation counters have no counterpart: present, null.
+        for key in ("covering_optimizer_jobs", "covering_optimizer_mismatches", "covering_optimizer_local_rounds",
+                    "covering_optimizer_full_steps", "covering_optimizer_phase_ms"):
+            assert key in stats and stats[key] is None, key
         with pytest.raises(ValueError):
             grid.optimize_covering(1, mode="everything")
 

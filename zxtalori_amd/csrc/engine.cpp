@@ -1620,6 +1620,15 @@ std::string Engine::stats() {
     j.null("covering_optimizer_best_deg");
   j.integer("covering_optimizer_improved", opt_.improved);
   j.integer("covering_optimizer_rejected", opt_.rejected);
+  // The reference's counters of its local re-tessellation and LP jobs (engine.zig:1000-1011)
+  // have no counterpart in this optimiser, which measures the whole covering after every
+  // move: present, so that readers of the record find every key, and null.
+  for (const char* k : {"covering_optimizer_jobs", "covering_optimizer_skipped", "covering_optimizer_local_rounds",
+                        "covering_optimizer_mismatches", "covering_optimizer_local_error",
+                        "covering_optimizer_skip_reasons", "covering_optimizer_overflows",
+                        "covering_optimizer_full_steps", "covering_optimizer_live_builds",
+                        "covering_optimizer_live_failures", "covering_optimizer_phase_ms"})
+    j.null(k);
   {
     char b[256];
     snprintf(b, sizeof(b), "{\"patience\":%u,\"progress\":%.9g,\"size\":%.9g,\"trust\":%.9g}", kicks_.patience,
