@@ -203,7 +203,7 @@ struct NearBuffers {
 // capacity / 1025 + 1 those holding more than 1024.
 inline uint32_t crowded_capacity(uint32_t capacity) { return capacity / 17 + 1; }
 inline uint32_t huge_capacity(uint32_t capacity) { return capacity / 1025 + 1; }
-constexpr uint32_t kCrowdedBlocks = 512;
+constexpr uint32_t kCrowdedBlocks = 1024;
 constexpr uint32_t kHugeBlocks = 64;
 size_t scan_tmp_words(size_t count);
 

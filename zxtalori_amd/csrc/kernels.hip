@@ -1045,9 +1045,10 @@ __global__ void k_cell_sort(const uint32_t* __restrict__ offsets, uint32_t cells
   }
 }
 
-// Crowded cells (more than kThreadCell items; a wide cutoff over a small sample crowds the
-// chart's centre), one wave each: the cell sorted in the wave's LDS by a bitonic network
-// padded to a power of two (keys are unique).  Cells of more than kWaveChunk items are
+// Crowded cells (more than kThreadCell items: coarse cells, or a wide cutoff over a small
+// sample crowding the chart's centre), one wave each: up to 64 items placed by rank (the
+// count of smaller keys; keys are unique), larger cells sorted in the wave's LDS by a
+// bitonic network padded to a power of two.  Cells of more than kWaveChunk items are
 // listed for k_cell_sort_block.
 constexpr uint32_t kWaveChunk = 1024;
 struct SortLds {
@@ -1069,6 +1070,19 @@ __global__ __launch_bounds__(256) void k_cell_sort_waves(const uint32_t* __restr
     const uint32_t begin = offsets[c], m = offsets[c + 1] - begin;
     if (m > kWaveChunk) {
       if (lane == 0) huge_list[atomicAdd(huge, 1u)] = c;
+      continue;
+    }
+    if (m <= 64) {
+      // Up to one item per lane: each lane's rank is the number of smaller keys (unique),
+      // read back from LDS as broadcasts; no sorting network.
+      const uint32_t v = lane < m ? perm[begin + lane] : 0u;
+      const uint32_t k = lane < m ? (key ? key[v] : v) : 0xffffffffu;
+      L.k[lane] = k;
+      wave_sync();
+      uint32_t rank = 0;
+      for (uint32_t j = 0; j < m; ++j) rank += L.k[j] < k ? 1u : 0u;
+      if (lane < m) perm[begin + rank] = v;
+      wave_sync();
       continue;
     }
     uint32_t width = 64;
