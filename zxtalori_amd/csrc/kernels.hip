@@ -949,8 +949,19 @@ __global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uin
     const float4 x = live ? q[src] : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
     uint32_t mine = 0;
     float4 y;
+    // The passing images, as bits (per <= 119 < 128): the writing pass recomputes only
+    // their positions, in the same order, without the tests.
+    uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
     if (live)
-      for (uint32_t img = 1; img <= per; ++img) mine += shell_image(x, img, ops, planes, nplanes, band, w_floor, y);
+      for (uint32_t img = 1; img <= per; ++img)
+        if (shell_image(x, img, ops, planes, nplanes, band, w_floor, y)) {
+          ++mine;
+          const uint32_t bit = 1u << (img & 31u), word = img >> 5;
+          m0 |= word == 0 ? bit : 0u;
+          m1 |= word == 1 ? bit : 0u;
+          m2 |= word == 2 ? bit : 0u;
+          m3 |= word == 3 ? bit : 0u;
+        }
     const uint32_t incl = wave_incl_scan(mine);
     const uint32_t total = __shfl(incl, 63);
     uint32_t base = 0;
@@ -958,17 +969,23 @@ __global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uin
     base = __shfl(base, 63) + incl - mine;
     if (live && mine) {
       uint32_t slot = base;
-      for (uint32_t img = 1; img <= per; ++img) {
-        if (!shell_image(x, img, ops, planes, nplanes, band, w_floor, y)) continue;
-        if (slot < b.capacity) {
-          b.uq[slot] = y;
-          b.ucode[slot] = src;
-          b.urank[slot] = src * (2 * order) + img;
-          box_add(lo, hi, y);
-        } else {
-          b.info->overflow = 1u;
+      for (uint32_t word = 0; word < 4; ++word) {
+        uint32_t bits = word == 0 ? m0 : word == 1 ? m1 : word == 2 ? m2 : m3;
+        for (; bits; bits &= bits - 1) {
+          const uint32_t img = word * 32 + uint32_t(__builtin_ctz(bits));
+          const uint32_t k = img >> 1;
+          y = image(x, ops[k], k);
+          if (img & 1u) y = neg4(y);
+          if (slot < b.capacity) {
+            b.uq[slot] = y;
+            b.ucode[slot] = src;
+            b.urank[slot] = src * (2 * order) + img;
+            box_add(lo, hi, y);
+          } else {
+            b.info->overflow = 1u;
+          }
+          ++slot;
         }
-        ++slot;
       }
     }
   }
