@@ -140,8 +140,9 @@ class Server:
     steps.  Jobs within a lane advance together, one BB step per lane step, so a lane's
     step lasts as long as its jobs' steps together: lanes of similar costs keep a job from
     being slowed by much costlier ones.  A large job fills the device by itself, so the
-    large lane runs one at a time, in arrival order: sharing steps among several would
-    only delay the first to finish.
+    large lane runs one relax job at a time, in arrival order: sharing steps among several
+    would only delay the first to finish.  Its single-pass requests (measure, spacing) join
+    beside the running relax job instead of waiting for all of its steps.
     """
 
     def __init__(self, hbm_budget=64 << 30, *, graphs=True, keep_results=False, bounds=(2e8, 2e9), ahead=1):
@@ -217,30 +218,46 @@ class Server:
         self.max_live = max(lane.max_live for lane in self.lanes)
         return outcomes, summary(outcomes, clock() - start)
 
+    def _admit(self, lane, requests, index, now):
+        """Adds request `index` to the lane's batch if its bytes fit; False if they do not."""
+        req = requests[index]
+        need = req.estimate_bytes()
+        with self._lock:
+            if lane.live and self._held + need > self.hbm_budget:
+                return False
+            self._held += need
+            self.peak_bytes = max(self.peak_bytes, self._held)
+        job = lane.batch.add(req.group, count=req.count, seed=req.seed, kind=req.kind, steps=req.steps,
+                             neighbourhood=req.neighbourhood)
+        lane.live[job] = (index, now, need)
+        return True
+
     def _serve_lane(self, lane, requests, outcomes, realtime, clock, start):
-        head = 0
+        pending = list(lane.queue)  # request indices in arrival order, not yet admitted
         batch = lane.batch
-        while head < len(lane.queue) or lane.live:
+        while pending or lane.live:
             now = clock() - start
-            while head < len(lane.queue):
-                req = requests[lane.queue[head]]
+            while pending:
+                req = requests[pending[0]]
                 if realtime and req.arrival > now:
                     break
                 if lane.limit and len(lane.live) >= lane.limit:
                     break
-                need = req.estimate_bytes()
-                with self._lock:
-                    if lane.live and self._held + need > self.hbm_budget:
+                if not self._admit(lane, requests, pending[0], now):
+                    break
+                pending.pop(0)
+            if lane.limit and len(lane.live) >= lane.limit:
+                # A lane limited to one job at a time still takes the single-pass requests
+                # (measure, spacing) that have arrived, in arrival order, beside the running
+                # relax job: they would otherwise wait for all of its steps.
+                for index in [i for i in pending if requests[i].kind != "relax"
+                              and not (realtime and requests[i].arrival > now)]:
+                    if not self._admit(lane, requests, index, now):
                         break
-                    self._held += need
-                    self.peak_bytes = max(self.peak_bytes, self._held)
-                job = batch.add(req.group, count=req.count, seed=req.seed, kind=req.kind, steps=req.steps,
-                                neighbourhood=req.neighbourhood)
-                lane.live[job] = (lane.queue[head], now, need)
-                head += 1
+                    pending.remove(index)
             if not lane.live:
                 # Idle until the next arrival (or until memory frees up).
-                wait = requests[lane.queue[head]].arrival - (clock() - start) if realtime else 0.0
+                wait = requests[pending[0]].arrival - (clock() - start) if realtime else 0.0
                 time.sleep(min(max(wait, 0.0005), 0.02))
                 continue
             lane.max_live = max(lane.max_live, len(lane.live))
