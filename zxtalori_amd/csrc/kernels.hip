@@ -1635,7 +1635,10 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
 // gives the cell, and only its faces are kept (the cell only shrinks, so a plane that
 // misses it now misses it for good).  The cell is certified when 2 theta is below the
 // distance of every unseen neighbour; the windows grow up to c_max.
-constexpr uint32_t kCoverGather = 256;  // candidates held while a window is chosen
+// Candidates held while a window is chosen (a multiple of 64).  With 192 the wave's LDS
+// (26 KB) lets 6 waves share a CU instead of 5 with 256: 199 -> 167 ms for O at 5e5
+// points; 128 narrows the windows so far that some samples' cells are not enclosed.
+constexpr uint32_t kCoverGather = 192;
 constexpr uint32_t kCoverWindow = 64;   // new planes per window
 constexpr uint32_t kCoverPlanes = 320;  // faces kept plus one window
 constexpr uint32_t kCoverFirst = 32;
