@@ -1636,8 +1636,8 @@ void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, ui
 // misses it now misses it for good).  The cell is certified when 2 theta is below the
 // distance of every unseen neighbour; the windows grow up to c_max.
 // Candidates held while a window is chosen (a multiple of 64).  With 192 the wave's LDS
-// (26 KB) lets 6 waves share a CU instead of 5 with 256: 199 -> 167 ms for O at 5e5
-// points; 128 narrows the windows so far that some samples' cells are not enclosed.
+// (26 KB with vertices, 18 KB without) lets 6 (8) waves share a CU instead of 5 with
+// 256; 128 narrows the windows so far that some samples' cells are not enclosed.
 constexpr uint32_t kCoverGather = 192;
 constexpr uint32_t kCoverWindow = 64;   // new planes per window
 constexpr uint32_t kCoverPlanes = 320;  // faces kept plus one window
@@ -1646,6 +1646,9 @@ constexpr double kCoverBox = 8.0;       // |t_k| <= 8: cells within ~82 degrees 
 
 constexpr uint32_t kCoverVertices = 192;  // vertices of the last cell kept for output
 
+// VERTS: the cell's vertices are kept for output (vertex and tetrahedron lists); a measuring
+// pass leaves them out, and the 8 KB it saves let 8 waves share a CU instead of 6.
+template <bool VERTS>
 struct CoverLds {
   float4 y[kCoverGather];
   float d2[kCoverGather];
@@ -1655,8 +1658,8 @@ struct CoverLds {
   uint32_t pcode[kCoverPlanes];  // its orbit code
   uint8_t face[kCoverPlanes];  // the plane bounds the cell (some vertex lies on it)
   float cut;
-  double vt[kCoverVertices][4];    // the cell's vertices (t, |t|^2), with `faces`
-  uint32_t vcode[kCoverVertices][3];  // the codes of the three planes through each (~0: box)
+  double vt[VERTS ? kCoverVertices : 1][4];    // the cell's vertices (t, |t|^2), with `faces`
+  uint32_t vcode[VERTS ? kCoverVertices : 1][3];  // the codes of the three planes through each (~0: box)
   uint32_t nv;
 };
 
@@ -1668,7 +1671,8 @@ struct CoverVertex {
 
 // Plane i of a pass over m neighbour planes: i < m a neighbour, else one of the six faces of
 // the box.
-__device__ inline void cover_plane(const CoverLds& L, uint32_t m, uint32_t i, double* n, double& d) {
+template <bool VERTS>
+__device__ inline void cover_plane(const CoverLds<VERTS>& L, uint32_t m, uint32_t i, double* n, double& d) {
   if (i < m) {
     n[0] = L.pl[i][0];
     n[1] = L.pl[i][1];
@@ -1684,7 +1688,8 @@ __device__ inline void cover_plane(const CoverLds& L, uint32_t m, uint32_t i, do
 
 // The farthest vertex of the cell of m neighbour planes and the box, over the wave; with
 // `faces`, every vertex is checked and the planes through one are marked in L.face.
-__device__ CoverVertex cover_cell(CoverLds& L, uint32_t m, bool faces) {
+template <bool VERTS>
+__device__ CoverVertex cover_cell(CoverLds<VERTS>& L, uint32_t m, bool faces) {
   const uint32_t lane = lane_id();
   const uint32_t total = m + 6;
   CoverVertex best{-1.0, {0.0, 0.0, 0.0}, {~0u, ~0u, ~0u}};
@@ -1734,8 +1739,8 @@ __device__ CoverVertex cover_cell(CoverLds& L, uint32_t m, bool faces) {
         if (a < m) L.face[a] = 1;
         if (b < m) L.face[b] = 1;
         if (c < m) L.face[c] = 1;
-        const uint32_t at = atomicAdd(&L.nv, 1u);
-        if (at < kCoverVertices) {
+        const uint32_t at = VERTS ? atomicAdd(&L.nv, 1u) : kCoverVertices;
+        if (VERTS && at < kCoverVertices) {
           L.vt[at][0] = t[0];
           L.vt[at][1] = t[1];
           L.vt[at][2] = t[2];
@@ -1802,8 +1807,9 @@ __device__ inline float chord2_of_angle(float a) {
 }
 __device__ inline double angle_of_chord2(float d2) { return 2.0 * asin(fmin(1.0, 0.5 * sqrt(double(d2)))); }
 
+template <bool VERTS>
 __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
-  __shared__ CoverLds L;
+  __shared__ CoverLds<VERTS> L;
   const uint32_t slot = blockIdx.x;
   if (slot >= a.n) return;
   const uint32_t lane = lane_id();
@@ -2026,7 +2032,7 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
     lo2 = hi2;
     hi = fminf(a.c_max, fmaxf(1.5f * float(reach), float(2.1 * theta)));
   }
-  if (a.vq && flag == 0) {
+  if (VERTS && a.vq && flag == 0) {
     // The certified cell's vertices (its last pass listed them; a cell with more than
     // kCoverVertices lists the first found).
     const uint32_t nv = min(L.nv, kCoverVertices);
@@ -2042,7 +2048,7 @@ __global__ __launch_bounds__(64) void k_cover(CoverArgs a) {
       a.vr[base + i] = float(atan(sqrt(fmax(L.vt[i][3], 0.0))));
     }
   }
-  if (a.tets && flag == 0) {
+  if (VERTS && a.tets && flag == 0) {
     const uint32_t nv = min(L.nv, kCoverVertices);
     const uint32_t self_code = src * 2 * a.order;
     uint32_t base = 0;
@@ -2092,7 +2098,12 @@ void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const flo
   a.tcount = tcount;
   a.tcap = tcap;
   a.wcode = wcode;
-  p.run("cover", [&] { hipLaunchKernelGGL(k_cover, dim3(n), dim3(64), 0, s, a); });
+  p.run("cover", [&] {
+    if (a.vq || a.tets)
+      hipLaunchKernelGGL(k_cover<true>, dim3(n), dim3(64), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_cover<false>, dim3(n), dim3(64), 0, s, a);
+  });
 }
 
 // Nearest-neighbour angles in SO(3) degrees, 4 asin(min(1, sqrt(c)/2)) 180/pi in FP32
