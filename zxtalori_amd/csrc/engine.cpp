@@ -209,12 +209,12 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   // Sites per cell of the box.  A ball of `reach` mean radii holds about reach^3 sites:
   // thousands at reach 20, where coarse cells (8 sites) win by giving fewer, fuller chunks
   // of 64 sites per row at the cost of more candidates; tens at reach 4 (the spacing
-  // pass), where fine cells (2 sites) win.  Measured on MI355X at I, N = 1e6: the force
-  // pass at reach 20 takes 15.5 ms with 8 sites per cell against 17.3 ms with 2, at
-  // reach 8 2.2 ms with 4 against 2.6 ms with 2; the reach-4 nearest pass at N = 1e7 4.1 ms
-  // with 2 against 5.3 ms with 8.
+  // pass), where fine cells win.  Measured on MI355X at I, N = 1e6: the force pass at
+  // reach 20 takes 15.5 ms with 8 sites per cell against 17.3 ms with 2, at reach 8 2.2 ms
+  // with 4 against 2.6 ms with 2; the reach-4 spacing pass at N = 1e7 (sort and nearest)
+  // 8.3 ms with 1.6 against 8.5 ms with 2 and 9.3 ms with 3.
   const double reach = radius / mean_radius_;
-  const double per_cell = std::min(8.0, std::max(2.0, reach / 2.5));
+  const double per_cell = std::min(8.0, std::max(1.5, reach / 2.5));
   sdim = uint32_t(std::min(320.0, std::max(1.0, std::round(std::cbrt(double(count) / per_cell)))));
   const size_t cells = size_t(sdim) * sdim * sdim;
   uq.ensure(size_t(capacity) * 16);
