@@ -304,6 +304,9 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profiles(),
                      "kernel": "near_force", "kernel_ms": kernel_s * 1e3, "pairs_per_launch": pairs,
                      "alg_valu_per_pair": valu, "alg_trans_per_pair": trans,
+                     "basis": "the reference's per-pair arithmetic (VALU + 8 x transcendental slots) at the "
+                              "published FP32 peak; the kernel executes fewer instructions per pair, so a "
+                              "fraction near or above 1 is speed on that work, not issue utilisation",
                      "candidates_per_pair": info["candidates"] / max(pairs, 1)},
         "breakdown_ms": breakdown,
         "sites": info["sites"],
