@@ -58,11 +58,6 @@ int32_t guarded_batch(zxt_batch* b, const char* prefix, F&& f) {
   return -1;
 }
 
-int32_t not_implemented(zxt_engine* h, const char* prefix) {
-  h->engine.err = std::string(prefix) + ": NotImplemented";
-  return -1;
-}
-
 bool valid_group(uint32_t id) { return id < zxt::kGroupCount; }
 
 }  // namespace

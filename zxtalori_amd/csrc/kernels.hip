@@ -55,17 +55,6 @@ __device__ inline uint32_t mbcnt(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
-// Appends the lanes with pred to a global counter; returns each such lane's slot.
-__device__ inline uint32_t wave_append(uint32_t* counter, bool pred) {
-  const uint64_t mask = __ballot(pred);
-  if (mask == 0) return 0;
-  const int leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (int(lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-  base = __shfl(base, leader);
-  return base + mbcnt(mask);
-}
-
 // Cross-lane steps by DPP (immediate lane patterns: no address registers, which the
 // ds_bpermute shuffles would keep live across the loops that call these).  row_shr:n
 // within each 16-lane row, then row_bcast:15 and row_bcast:31 carry row totals upward;
@@ -118,24 +107,12 @@ __device__ inline double uniform_add(double acc, float v) {
   return acc + __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
 }
 
-__device__ inline float wave_min(float v) {
-  for (int d = 32; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d));
-  return v;
-}
 __device__ inline uint32_t wave_min_u32(uint32_t v) {
   for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
   return v;
 }
 __device__ inline uint32_t wave_max_u32(uint32_t v) {
   for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
-  return v;
-}
-__device__ inline uint64_t wave_min_u64(uint64_t v) {
-  for (int d = 32; d > 0; d >>= 1) {
-    uint32_t lo = __shfl_xor(uint32_t(v), d), hi = __shfl_xor(uint32_t(v >> 32), d);
-    uint64_t o = (uint64_t(hi) << 32) | lo;
-    v = o < v ? o : v;
-  }
   return v;
 }
 
