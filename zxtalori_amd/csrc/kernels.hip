@@ -1518,7 +1518,9 @@ __global__ __launch_bounds__(256) void k_spacing(const float4* __restrict__ q, u
                                                  uint32_t two_order, uint32_t real_blocks,
                                                  float* __restrict__ chord_out, uint32_t* __restrict__ rank_out) {
   constexpr uint32_t G = kSpacingLanes;
-  const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
+  // Workgroups in launch order: for these short per-source scans that measured faster (4.1
+  // against 4.2 ms at N = 1e7) than one contiguous source range per XCD.
+  const uint32_t vb = blockIdx.x;
   if (vb >= real_blocks) return;
   const uint32_t sub = threadIdx.x % G;
   const uint32_t slot = vb * (kBlock / G) + threadIdx.x / G;
