@@ -151,8 +151,9 @@ class Server:
         self.bounds = tuple(float(b) for b in bounds)
         self.ahead = int(ahead)
         # The large lane's passes fill the device alone; the others spread their jobs'
-        # near-field passes over two extra streams.
-        self.lanes = [_Lane(f"lane{k}", graphs, 2) for k in range(len(self.bounds))]
+        # near-field passes over three extra streams (the 256-job mix: 9.4 s against 9.6 s
+        # with two).
+        self.lanes = [_Lane(f"lane{k}", graphs, 3) for k in range(len(self.bounds))]
         self.lanes.append(_Lane("large", graphs, 0, limit=1))
         self.peak_bytes = 0
         self.max_live = 0
