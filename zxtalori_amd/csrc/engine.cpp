@@ -1027,6 +1027,11 @@ Q4 unit4(Q4 q) {
 class SphereHash {
  public:
   explicit SphereHash(double cell) : cell_(std::max(cell, 1e-9)) {}
+  // Room for `count` add() calls (each stores q and -q) without rehashing.
+  void reserve(size_t count) {
+    cells_.reserve(2 * count);
+    points_.reserve(2 * count);
+  }
   void add(const Q4& q) {
     for (double sign : {1.0, -1.0}) {
       const Q4 p{sign * q[0], sign * q[1], sign * q[2], sign * q[3]};
@@ -1115,6 +1120,7 @@ std::vector<float> Engine::cap_centres(uint32_t wanted) {
   extend(size_t(wanted) * 16 + 4096);
   const double largest = double(vr[rank[0]]);
   SphereHash occupied(2.0 * std::sin(0.5 * largest));
+  occupied.reserve(size_t(n_) + wanted);
   for (uint32_t i = 0; i < n_; ++i) occupied.add(unit4({host_[4 * i], host_[4 * i + 1], host_[4 * i + 2], host_[4 * i + 3]}));
   std::vector<float> out;
   for (size_t at = 0; at < total && out.size() / 4 < wanted; ++at) {
@@ -1167,6 +1173,7 @@ uint32_t Engine::orbit_compact(std::vector<float>& q, uint32_t unique) const {
   SphereHash kept(2.0 * tolerance);
   uint32_t count = 0;
   const uint32_t rows = uint32_t(q.size() / 4);
+  kept.reserve(rows);
   for (uint32_t i = 0; i < rows; ++i) {
     const Q4 src{q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
     bool duplicate = false;
