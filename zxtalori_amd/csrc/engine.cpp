@@ -220,7 +220,7 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   uq.ensure(size_t(capacity) * 16);
   ucode.ensure(size_t(capacity) * 4);
   urank.ensure(size_t(capacity) * 4);
-  ucell.ensure((size_t(capacity) + 1) * 4);
+  ucell.ensure((std::max<size_t>(capacity, tile_visits(sdim)) + 1) * 4);  // then the visits' scan
   perm.ensure(size_t(capacity) * 4);
   sq.ensure(size_t(capacity) * 16);
   scode.ensure(size_t(capacity) * 4);
@@ -228,7 +228,7 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   offsets.ensure((cells + 1) * 4);
   cursor.ensure((cells + 1) * 4);
   order_buf.ensure(size_t(n) * 4);
-  scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, capacity)) * 4);
+  scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, tile_visits(sdim))) * 4);
   counters.ensure(4 * sizeof(unsigned long long));
   counts.ensure(size_t(n) * 8);
   crowded.ensure(size_t(crowded_capacity(capacity)) * 4);

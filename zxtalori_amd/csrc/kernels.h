@@ -199,6 +199,13 @@ struct NearBuffers {
   uint32_t* huge = nullptr;      // huge_capacity: cells with more sites than one wave sorts
   uint32_t huge_capacity = 0;
 };
+// The sources are visited by tiles of kTile^3 cells; the padded grid of tiles has this many
+// visits (at least sdim^3).
+constexpr uint32_t kTile = 8;
+__host__ __device__ inline uint32_t tile_visits(uint32_t sdim) {
+  const uint32_t nt = (sdim + kTile - 1) / kTile;
+  return nt * nt * nt * kTile * kTile * kTile;
+}
 // capacity / 17 + 1 bounds the cells holding more than 16 of `capacity` sites, and
 // capacity / 1025 + 1 those holding more than 1024.
 inline uint32_t crowded_capacity(uint32_t capacity) { return capacity / 17 + 1; }

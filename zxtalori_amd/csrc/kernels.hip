@@ -1185,32 +1185,54 @@ __global__ __launch_bounds__(256) void k_cell_sort_block(const uint32_t* __restr
   }
 }
 
-// Sites in cell order; ucell (free once the scatter has run) takes each slot's core flag,
-// zeros past the last site, for the scan that orders the sources.
+// Sites in cell order.
 __global__ void k_site_gather(NearBuffers b) {
   const uint32_t S = min(b.info->sites, b.capacity);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < b.capacity; k += gridDim.x * blockDim.x) {
-    uint32_t core = 0;
-    if (k < S) {
-      const uint32_t s = b.perm[k];
-      const uint32_t code = b.ucode[s];
-      b.sq[k] = b.uq[s];
-      b.scode[k] = code;
-      b.srank[k] = b.urank[s];
-      core = code >> 31;
-    }
-    b.ucell[k] = core;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x) {
+    const uint32_t s = b.perm[k];
+    b.sq[k] = b.uq[s];
+    b.scode[k] = b.ucode[s];
+    b.srank[k] = b.urank[s];
   }
 }
 
-// The sources in the cell order of their identity images (the core sites): waves of
+// The sources are visited tile by tile (8x8x8 cells, tiles in row-major order, cells
+// row-major within a tile), each cell's identity images in their sorted order: waves of
 // neighbouring sources read overlapping site runs, and each XCD takes a contiguous range
-// of them (xcd_remap).  ucell holds the exclusive scan of the core flags.
+// of them (xcd_remap) whose neighbourhood its L2 holds.  Visit v of the padded tile grid
+// (tile_visits(sdim) entries) names the cell, or none past the grid's edge.
+__device__ inline bool visit_cell(uint32_t v, uint32_t sdim, uint32_t& cell) {
+  const uint32_t nt = (sdim + kTile - 1) / kTile;
+  const uint32_t w = v % (kTile * kTile * kTile), t = v / (kTile * kTile * kTile);
+  const uint32_t x = (t % nt) * kTile + w % kTile, y = (t / nt % nt) * kTile + w / kTile % kTile,
+                 z = (t / (nt * nt)) * kTile + w / (kTile * kTile);
+  if (x >= sdim || y >= sdim || z >= sdim) return false;
+  cell = x + sdim * (y + sdim * z);
+  return true;
+}
+
+// ucell (free once the scatter has run) takes each visit's count of identity images.
+__global__ void k_core_count(NearBuffers b) {
+  const uint32_t visits = tile_visits(b.sdim);
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < visits; v += gridDim.x * blockDim.x) {
+    uint32_t cell, count = 0;
+    if (visit_cell(v, b.sdim, cell))
+      for (uint32_t k = b.offsets[cell]; k < b.offsets[cell + 1]; ++k) count += b.scode[k] >> 31;
+    b.ucell[v] = count;
+  }
+}
+
+// With ucell scanned: each visited cell writes its sources from its offset on.
 __global__ void k_core_order(NearBuffers b) {
-  const uint32_t S = min(b.info->sites, b.capacity);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x) {
-    const uint32_t code = b.scode[k];
-    if (code >> 31) b.order[b.ucell[k]] = code & 0x7fffffffu;
+  const uint32_t visits = tile_visits(b.sdim);
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < visits; v += gridDim.x * blockDim.x) {
+    uint32_t cell;
+    if (!visit_cell(v, b.sdim, cell)) continue;
+    uint32_t at = b.ucell[v];
+    for (uint32_t k = b.offsets[cell]; k < b.offsets[cell + 1]; ++k) {
+      const uint32_t code = b.scode[k];
+      if (code >> 31) b.order[at++] = code & 0x7fffffffu;
+    }
   }
 }
 
@@ -1252,8 +1274,10 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
     }
   });
   p.run("site_gather", [&] { hipLaunchKernelGGL(k_site_gather, dim3(g), dim3(kBlock), 0, s, b); });
-  launch_scan(p, s, b.ucell, b.capacity, b.scan_tmp);
-  p.run("core_order", [&] { hipLaunchKernelGGL(k_core_order, dim3(g), dim3(kBlock), 0, s, b); });
+  const uint32_t visits = tile_visits(b.sdim), gv = stride_grid(visits);
+  p.run("core_order", [&] { hipLaunchKernelGGL(k_core_count, dim3(gv), dim3(kBlock), 0, s, b); });
+  launch_scan(p, s, b.ucell, visits, b.scan_tmp);
+  p.run("core_order", [&] { hipLaunchKernelGGL(k_core_order, dim3(gv), dim3(kBlock), 0, s, b); });
   // The order only schedules the waves: each source's row is its own.
 }
 
