@@ -190,7 +190,7 @@ struct NearBuffers {
   uint32_t* srank = nullptr;     // capacity
   uint32_t* offsets = nullptr;   // sdim^3 + 1
   uint32_t* cursor = nullptr;    // sdim^3 + 1
-  uint32_t* order = nullptr;     // n, sources in the cell order of their identity images
+  uint32_t* order = nullptr;     // n, sources in the tile-by-tile visit order of their identity images
   uint32_t* scan_tmp = nullptr;  // scan_tmp_words(max(cells, capacity))
   unsigned long long* counters = nullptr;  // [0] accepted pairs, [1] sites tested (force pass)
   uint32_t* counts = nullptr;    // 2n: per source, pairs and sites tested of the last force pass
@@ -218,7 +218,7 @@ size_t scan_tmp_words(size_t count);
 void launch_near_sites(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                        const NearBuffers& b);
 // Builds the sites (core + shell), sorts them into the cell grid and orders the sources
-// by the cell order of their identity images.  Reads nothing back.
+// by the tile-by-tile visit order of their identity images.  Reads nothing back.
 void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                        const NearBuffers& b);
 // Force rows (mode 0) or nearest orbit point with rank (mode 1) over the built sites.

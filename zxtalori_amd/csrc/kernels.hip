@@ -19,8 +19,8 @@
 //    64 candidates at a time.  Candidates within the cutoff are compacted through an LDS
 //    ring (ballot + mbcnt), so the ~100-op pair kernel always runs on full waves; the
 //    accepted order is deterministic, and so are the sums.  Workgroups are remapped so each
-//    XCD takes a contiguous range of sources (in the cell order of their identity
-//    images) and its L2 holds their sites.
+//    XCD takes a contiguous range of sources (visited tile by tile, 8^3 cells) and its
+//    L2 holds their sites.
 //  * Sorts are counting sorts (atomic counts, a three-kernel scan, atomic scatter, then an
 //    in-cell sort by a unique key), so every pass is deterministic.
 #include <hip/hip_runtime.h>
