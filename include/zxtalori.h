@@ -141,6 +141,14 @@ int32_t zo_near_info(zxt_engine* e, uint32_t* out3);
 uint32_t zo_hopf_points(uint32_t group, uint32_t f, float* out, uint32_t capacity);
 /* Host only: the frequency zo_generate takes for h = 0 and `target` (src/hopf.zig:82-97). */
 uint32_t zo_hopf_frequency(uint32_t group, uint32_t target);
+/* Development hook, the counterpart of the reference's capacity growth (src/near.zig:102-116,
+ * :191-195; src/pairs.zig:391-394): the next near-field sizing leaves room for exactly the
+ * site count it finds plus `sites` (may be negative; never fewer than the sources) instead
+ * of a quarter more, so that a pass overflows and is redone, as a shell that outgrows its
+ * buffers is.  Used once, then the usual headroom returns. */
+int32_t zo_near_squeeze(zxt_engine* e, int32_t sites);
+/* Near-field passes redone after their shell overflowed, over the engine's life. */
+uint32_t zo_near_resizes(zxt_engine* e);
 /* The last near-field pass's cutoff: chord^2, S3 radius r, and the number of Taylor terms
  * of the pair kernel's arctangent for it (0: the minimax polynomial). */
 int32_t zo_near_cut(zxt_engine* e, float* out3);

@@ -376,7 +376,8 @@ def test_fixed_identity_stays_exact(operator_fixture):
 
 def test_near_field_large_sample(operator_fixture):
     """Group I, N=1,000,000, neighbourhood 20 (the throughput workload): deterministic
-    passes, and the rows of a 2,000-source sample equal the oracle's grid restatement."""
+    passes, the rows of a 2,000-source sample equal the oracle's grid restatement, and
+    their energy sum is within 1e-6 relative of FP64 pair terms over the same pairs."""
     note("near I 1e6")
     ops = ops32(operator_fixture, "I")
     with zo.haar("I", 10_000_000, 193) as big:
@@ -399,6 +400,10 @@ def test_near_field_large_sample(operator_fixture):
     # where the reference's direction x0 y - y0 x + x x y cancels in FP32; so the FP32
     # oracle is no closer to FP64 than the kernel is.
     ref64 = oracle.near_grid64(q, ops, np.asarray(operator_fixture["I"], np.float64), cut, r, first, 2000)
+    e_err = abs(e.sum() - ref64[:, 4].sum()) / ref64[:, 4].sum()
+    note("energy sum rel err vs fp64", e_err, "worst row", np.abs(e / ref64[:, 4] - 1).max())
+    assert e_err <= 1e-6
+    np.testing.assert_allclose(e, ref64[:, 4], rtol=1e-5)
     scale = np.linalg.norm(ref64[:, 6])
     gpu_err = np.linalg.norm(f[:, :3] - ref64[:, :3]) / scale
     cpu_err = np.linalg.norm(ref[:, :3] - ref64[:, :3]) / scale
@@ -429,3 +434,43 @@ def test_cancel_stops_a_long_relaxation():
         assert grid.stats["separation_deg"] > 0
         grid.relax(2)
         assert grid.stats["iteration"] == done + 3
+
+
+def test_near_overflow_is_sized_again_and_redone(operator_fixture):
+    """The reference grows the near-field buffers when the shell outgrows them and redoes
+    the pass (src/near.zig:102-116, :191-195; src/pairs.zig:391-394).  Forced here with the
+    sizing hook: room for one site fewer than the sample has makes the first pass overflow,
+    and the rows that come back are those of a pass sized again (equal to an engine that
+    never overflowed); room for half the shell makes every pass of a relaxation overflow,
+    and the rows it ends with are redone at the final positions (equal to the oracle's)."""
+    group, count = "I", 5000
+    ops = ops32(operator_fixture, group)
+    cut, _ = dense.near_cut(count, len(ops), 20)
+    note("near overflow: one site short")
+    with zo.haar(group, count, 5) as plain:
+        plain.relax(0, neighbourhood=20)
+        expected = plain.force_rows()
+        shell = plain.near_info()["shell"]
+        assert plain._lib.zo_near_resizes(plain._handle) == 0
+    assert shell > 100
+    with zo.haar(group, count, 5) as grid:
+        grid.relax(0, neighbourhood=20)
+        grid._lib.zo_near_squeeze(grid._handle, -1)
+        rows = grid.force_rows()
+        assert grid._lib.zo_near_resizes(grid._handle) == 1
+        assert grid.near_info()["shell"] == shell
+    for a, b in zip(rows, expected):
+        np.testing.assert_array_equal(a, b)
+    note("near overflow: mid-relax, half the shell")
+    with zo.haar(group, count, 5) as grid:
+        grid._lib.zo_near_squeeze(grid._handle, -(shell // 2))
+        grid.relax(4, neighbourhood=20)
+        assert grid._lib.zo_near_resizes(grid._handle) == 1
+        assert grid.stats["iteration"] == 4
+        energy = grid.stats["energy"]
+        rows = grid.force_rows()
+        q = grid.quaternions
+        assert grid._lib.zo_near_resizes(grid._handle) == 1
+    ref = oracle.near_brute(q, ops, cut)
+    check_rows(*rows, ref)
+    assert energy == pytest.approx(ref[:, 4].sum() / (count * len(ops) * (count - 1)), rel=2e-6)

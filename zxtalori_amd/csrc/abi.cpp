@@ -231,6 +231,13 @@ int32_t zo_near_info(zxt_engine* e, uint32_t* out3) {
   return 0;
 }
 
+int32_t zo_near_squeeze(zxt_engine* e, int32_t sites) {
+  e->engine.near_squeeze = sites;
+  e->engine.near_squeeze_pending = true;
+  return 0;
+}
+uint32_t zo_near_resizes(zxt_engine* e) { return e->engine.near_resizes; }
+
 int32_t zo_near_cut(zxt_engine* e, float* out3) {
   out3[0] = e->engine.last_near_cut.chord2;
   out3[1] = float(e->engine.last_near_radius);

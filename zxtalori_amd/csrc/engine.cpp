@@ -204,8 +204,15 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   }
   if (!sized) fail("NearShellTooLarge");
   const uint64_t count = status.sites;
-  // Room for the shell to change as the sources move.
-  capacity = uint32_t(std::min<uint64_t>(std::max<uint64_t>(capacity, count + count / 4 + 1024), 0xfff00000ull));
+  if (e.near_squeeze_pending) {
+    // Core sites are written at their source's index, so the sources always fit.
+    e.near_squeeze_pending = false;
+    const int64_t want = std::max<int64_t>(int64_t(count) + e.near_squeeze, int64_t(n));
+    capacity = uint32_t(std::min<int64_t>(want, int64_t(capacity)));
+  } else {
+    // Room for the shell to change as the sources move.
+    capacity = uint32_t(std::min<uint64_t>(std::max<uint64_t>(capacity, count + count / 4 + 1024), 0xfff00000ull));
+  }
   // Sites per cell of the box.  A ball of `reach` mean radii holds about reach^3 sites:
   // thousands at reach 20, where coarse cells (8 sites) win by giving fewer, fuller chunks
   // of 64 sites per row at the cost of more candidates; tens at reach 4 (the spacing
@@ -254,6 +261,7 @@ bool NearField::overflowed(Engine& e) {
   e.last_near_radius = radius;
   if (!status.overflow) return false;
   ready = false;
+  ++e.near_resizes;
   return true;
 }
 

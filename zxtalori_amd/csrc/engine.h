@@ -221,6 +221,12 @@ class Engine {
   unsigned long long last_near_counts[2] = {0, 0};  // accepted pairs, sites tested
   NearCut last_near_cut{};
   double last_near_radius = 0;
+  // Development hook: the next near-field sizing leaves room for exactly its site count
+  // plus near_squeeze sites (never fewer than the sources), instead of a quarter more, so
+  // that the pass overflows when the shell grows; cleared once used.
+  bool near_squeeze_pending = false;
+  int32_t near_squeeze = 0;
+  uint32_t near_resizes = 0;  // near-field passes redone after their shell overflowed
 
   // Sends this engine's launches to s (returns the previous stream); a batch runs the
   // near-field passes of its jobs on several streams this way.

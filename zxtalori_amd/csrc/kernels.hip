@@ -1358,8 +1358,10 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   // site and dropped when its pair is evaluated: it is the site bit-identical to x (sites
   // are rebuilt from q before every pass), so its chord is exactly 0.  A zero chord enters
   // the pair arithmetic as chord^2 2 (S3 angle pi/2, beyond every window, so its terms
-  // vanish exactly), stays out of the nearest chord, and is counted (uniform); a second
-  // one (an exact duplicate of x) is restored at the end.
+  // vanish exactly), stays out of the nearest chord, and is counted; a second one (an
+  // exact duplicate of x) is restored at the end.  The count is a ballot over the active
+  // lanes, so every active lane holds it; the partial tail runs on lanes 0..qcount-1 only,
+  // and the count is read from lane 0 after it.
   uint32_t zeros = 0;
   const float u_offset = -cut.taper_start * cut.inv_width;
   uint32_t qhead = 0, qcount = 0, rounds = 0;
@@ -1450,6 +1452,7 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
     }
   }
   if (lane < qcount) process(qhead + lane);
+  zeros = __builtin_amdgcn_readfirstlane(zeros);
   flush();
   uint32_t pairs = 64 * rounds + qcount;
   uint32_t nearest_key = wave_min_u32(nearest_bits);
