@@ -153,6 +153,22 @@ uint32_t zo_near_resizes(zxt_engine* e);
  * of the pair kernel's arctangent for it (0: the minimax polynomial). */
 int32_t zo_near_cut(zxt_engine* e, float* out3);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops and harnesses that forbid hipMalloc): zo_spacing's
+ * all-pairs path on caller-owned device buffers, allocating nothing.  q: n device rows
+ * (w, x, y, z), 16-byte aligned, already in the fundamental zone (as zo_quaternions returns
+ * them; they are not folded here); outputs, all device memory: rank_out[n] the nearest orbit
+ * point's rank j*2|K| + 2k + sign, degrees_out[n] its SO(3) angle in degrees, least_out[1] the
+ * separation (the least angle; 0 for duplicate orientations, where zo_spacing fails).  Launches
+ * on `stream` (a hipStream_t, NULL for the null stream) and returns without waiting.  Limit:
+ * 2 <= n <= 20,000 (zo_spacing's all-pairs range); the scratch is one static 10 MiB device
+ * array per device, shared by every call on that device, so calls that overlap on one device
+ * race.  Returns 0, or -1 with the message in zo_create_error() ("Spacing:
+ * CountExceedsStaticScratch", "...: TooFewOrientations", "...: UnknownGroup", "...: NullPointer",
+ * "...: MisalignedBuffer", "...: GpuError: ..."). */
+int32_t zo_spacing_static(uint32_t group, const float* q, uint32_t n, uint32_t* rank_out, float* degrees_out,
+                          float* least_out, void* stream);
+
 /* ---- Batches: many samples relaxed together ------------------------------------------
  * No counterpart in the reference, which runs one sample per engine and one call at a time
  * (src/native_api.zig:9-19, python/zxtalori/api.py:95-122).  A batch holds jobs, each a

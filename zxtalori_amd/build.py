@@ -21,7 +21,7 @@ ARCH = os.environ.get("ZXTALORI_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result", "-ffp-contract=off"]
 DEVICE = [f"--offload-arch={ARCH}", "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["kernels.hip", "engine.cpp", "batch.cpp", "abi.cpp", "hopf.cpp", "transport.cpp"]
+SOURCES = ["kernels.hip", "spacing_static.hip", "engine.cpp", "batch.cpp", "abi.cpp", "hopf.cpp", "transport.cpp"]
 
 
 def hipcc():

@@ -62,6 +62,8 @@ bool valid_group(uint32_t id) { return id < zxt::kGroupCount; }
 
 }  // namespace
 
+void zxt::set_create_error(const char* text) { snprintf(g_create_error, sizeof(g_create_error), "%s", text); }
+
 extern "C" {
 
 const char* zo_version(void) { return "0.3.0-mi355x"; }

@@ -166,6 +166,14 @@ AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale, AllP
 void launch_allpairs(Prof& p, hipStream_t s, const JobSet& jobs);
 void launch_allpairs_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                              uint64_t* partial, float* chord_out, uint32_t* rank_out);
+// zo_spacing's all-pairs path with the group's operators read from the compiled-in table
+// and no profiling hooks (nothing allocated, host or device): nearest keys into partial
+// (order x n), chords into chord (n), then ranks, SO(3) degrees and the least degree's
+// bits (*least_bits).  Returns the launches' error state.
+hipError_t launch_spacing_table(hipStream_t s, const float4* q, uint32_t n, uint32_t group, uint64_t* partial,
+                                float* chord, uint32_t* rank_out, float* degrees_out, uint32_t* least_bits);
+// The text zo_create_error returns (abi.cpp), for failures of calls without an engine.
+void set_create_error(const char* text);
 // FP64 Riesz 1-3, log and Green row sums (out: n x 5), all pairs, self pair excluded.
 void launch_measure(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                     const AllPairsPlan& plan, double* partial, double* out);

@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdio>
 
+#include "groups_table.h"
 #include "kernels.h"
 #include "qvm_device.h"
 #include "transport_math.h"
@@ -522,15 +523,24 @@ void launch_allpairs(Prof& p, hipStream_t s, const JobSet& jobs) {
 
 // Nearest orbit point over all pairs with its rank j*2*order + 2*k + sign; the minimum
 // chord^2, ties to the lowest rank.  blockIdx.y is the operator.
+// TABLE: the operators come from the compiled-in table (ops holds the group's first row
+// index there) instead of a device copy; the same values, so the same bits.
+template <bool TABLE>
 __global__ __launch_bounds__(256) void k_allpairs_nearest(const float4* __restrict__ q, uint32_t n,
-                                                          const float4* __restrict__ ops, uint32_t order,
-                                                          uint64_t* __restrict__ partial) {
+                                                          const float4* __restrict__ ops, uint32_t table_row,
+                                                          uint32_t order, uint64_t* __restrict__ partial) {
   __shared__ float4 tile[kBlock];
   const uint32_t k = blockIdx.y;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const bool live = i < n;
   const float4 x = live ? q[i] : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-  const float4 op = ops[k];
+  float4 op;
+  if (TABLE) {
+    const float* row = kOperators[table_row + k];
+    op = make_float4(row[0], row[1], row[2], row[3]);
+  } else {
+    op = ops[k];
+  }
   uint64_t best = ~uint64_t(0);
   for (uint32_t start = 0; start < n; start += kBlock) {
     const uint32_t j = start + threadIdx.x;
@@ -570,14 +580,15 @@ __global__ void k_nearest_reduce(const uint64_t* __restrict__ partial, uint32_t 
 void launch_allpairs_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                              uint64_t* partial, float* chord_out, uint32_t* rank_out) {
   p.run("allpairs_nearest", [&] {
-    hipLaunchKernelGGL(k_allpairs_nearest, dim3(blocks_for(n, kBlock), order), dim3(kBlock), 0, s, q, n, ops, order,
-                       partial);
+    hipLaunchKernelGGL(k_allpairs_nearest<false>, dim3(blocks_for(n, kBlock), order), dim3(kBlock), 0, s, q, n, ops,
+                       0u, order, partial);
   });
   p.run("nearest_reduce", [&] {
     hipLaunchKernelGGL(k_nearest_reduce, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, partial, n, order,
                        chord_out, rank_out);
   });
 }
+
 
 // ---- full measure (pairs.wgsl:151-244) ----------------------------------------------
 // Riesz s=1,2,3, Green and log sums over every orbit point but the source itself, in
@@ -2136,6 +2147,19 @@ void launch_degrees(Prof& p, hipStream_t s, const float* chord, uint32_t n, floa
     hipLaunchKernelGGL(k_degrees, dim3(std::min<uint32_t>(2048, blocks_for(n, kBlock))), dim3(kBlock), 0, s, chord, n,
                        out, least);
   });
+}
+
+hipError_t launch_spacing_table(hipStream_t s, const float4* q, uint32_t n, uint32_t group, uint64_t* partial,
+                                float* chord, uint32_t* rank_out, float* degrees_out, uint32_t* least_bits) {
+  const uint32_t order = kGroupOrder[group];
+  hipLaunchKernelGGL(k_allpairs_nearest<true>, dim3(blocks_for(n, kBlock), order), dim3(kBlock), 0, s, q, n,
+                     nullptr, kGroupOffset[group], order, partial);
+  hipLaunchKernelGGL(k_nearest_reduce, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, partial, n, order, chord,
+                     rank_out);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, s, least_bits, 0xffffffffu, size_t(1));
+  hipLaunchKernelGGL(k_degrees, dim3(std::min<uint32_t>(2048, blocks_for(n, kBlock))), dim3(kBlock), 0, s, chord, n,
+                     degrees_out, least_bits);
+  return hipGetLastError();
 }
 
 // ---- BB step (relax.wgsl:29-165) -----------------------------------------------------
