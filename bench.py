@@ -221,6 +221,54 @@ def traffic_from_profiles():
     return None
 
 
+def counters_from_profile(name):
+    """Wave64 instruction counts per launch (SQ_INSTS_VALU including transcendentals,
+    SQ_INSTS_VALU_TRANS_F32, SQ_INSTS_SALU) of a kernel at its workload, from the committed
+    counter summary profiles/<name>_counters.md (the counts are fixed by the kernel and
+    its input, so they hold for every run of the same workload)."""
+    path = ROOT / "profiles" / f"{name}_counters.md"
+    out = {}
+    if path.is_file():
+        for line in path.read_text().splitlines():
+            for key in ("valu_per_launch", "trans_per_launch", "salu_per_launch"):
+                if line.startswith(key + ":"):
+                    out[key] = float(line.split(":", 1)[1])
+    return out if len(out) == 3 else None
+
+
+def issue_rates():
+    """The chip's measured VALU issue rates (zo_issue_probe): non-packed FP32 FMA and
+    transcendental wave64 instructions per second, each the best of three runs."""
+    import ctypes
+    from zxtalori_amd.api import _library
+    lib = _library()
+    rates = {}
+    for kind, name in ((0, "fma"), (1, "trans")):
+        best = 0.0
+        for _ in range(3):
+            value = ctypes.c_double(0.0)
+            if lib.zo_issue_probe(kind, 20000, ctypes.addressof(value)) == 0:
+                best = max(best, value.value)
+        rates[name] = best
+    return rates
+
+
+# gfx950 issue model: one non-packed FP32 VALU wave64 instruction per 2 cycles per SIMD (the
+# published 157.3 TFLOP/s), a transcendental per 8, 1024 SIMDs at 2.4 GHz.
+VALU_CYCLES, TRANS_CYCLES, SIMDS, CLOCK_HZ = 2.0, 8.0, 1024, 2.4e9
+
+
+def valu_busy(counts, kernel_s):
+    """Fraction of the published VALU issue peak the kernel's executed instructions fill:
+    the cycles its non-transcendental VALU and its transcendental wave64 instructions take at
+    the peak issue rates, over the kernel's time on all SIMDs at the peak clock."""
+    if not counts or not kernel_s:
+        return None
+    trans = counts["trans_per_launch"]
+    cycles = (counts["valu_per_launch"] - trans) * VALU_CYCLES + trans * TRANS_CYCLES
+    return cycles / (SIMDS * CLOCK_HZ * kernel_s)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,7 +288,12 @@ def main():
                     help="skip the generation, covering and exact-count measurements")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (the HBM counter calibration kernel)")
+    ap.add_argument("--calibrate-issue", action="store_true",
+                    help="run only the VALU issue-rate probe (FMA and transcendental chains)")
     args = ap.parse_args()
+    if args.calibrate_issue:
+        print(json.dumps({"issue_rates_wave64_per_s": issue_rates()}))
+        return
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 4
@@ -284,6 +337,9 @@ def main():
     flop_equiv = 2.0 * (valu + trans * PEAK_VALU / PEAK_TRANS)
     achieved = pairs * flop_equiv / kernel_s / 1e12
     breakdown = {k: round(ms / c, 4) for k, (c, ms) in sorted(report.items(), key=lambda kv: -kv[1][1])}
+    # Instruction counts per launch hold for this workload only (I, N = 1e6, neighbourhood 20).
+    default_workload = (args.group, args.n, args.delta) == ("I", 1_000_000, 20.0)
+    near_counts = counters_from_profile("near_force_i1e6") if default_workload else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -304,9 +360,17 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profiles(),
                      "kernel": "near_force", "kernel_ms": kernel_s * 1e3, "pairs_per_launch": pairs,
                      "alg_valu_per_pair": valu, "alg_trans_per_pair": trans,
-                     "basis": "the reference's per-pair arithmetic (VALU + 8 x transcendental slots) at the "
-                              "published FP32 peak; the kernel executes fewer instructions per pair, so a "
-                              "fraction near or above 1 is speed on that work, not issue utilisation",
+                     "basis": "frac: speed on the reference's per-pair arithmetic (its VALU and "
+                              "transcendental counts priced at the published peaks), not issue "
+                              "utilisation; valu_busy: the kernel's executed VALU and transcendental "
+                              "instructions (rocprof SQ counters, profiles/near_force_i1e6_counters.md) at "
+                              "2 and 8 cycles per wave64 instruction over its measured time on 1024 SIMDs "
+                              "at 2.4 GHz",
+                     "valu_busy": valu_busy(near_counts, kernel_s),
+                     "executed_per_64_pairs": None if not near_counts else {
+                         "valu": (near_counts["valu_per_launch"] - near_counts["trans_per_launch"]) * 64 / pairs,
+                         "trans": near_counts["trans_per_launch"] * 64 / pairs,
+                         "salu": near_counts["salu_per_launch"] * 64 / pairs},
                      "candidates_per_pair": info["candidates"] / max(pairs, 1)},
         "breakdown_ms": breakdown,
         "sites": info["sites"],

@@ -118,6 +118,12 @@ void zo_delaunay_mode(zxt_engine* e, uint32_t mode);
  * compiled-in tuned table. */
 void zo_tune_allpairs(uint32_t target, uint32_t ops_per_split);
 
+/* No counterpart in the reference: the chip's VALU issue rates, measured on device 0 with
+ * long chains of one instruction kind on 8 waves per SIMD (kind 0 non-packed FP32 FMA, kind
+ * 1 transcendentals v_rsq_f32/v_rcp_f32), in wave64 instructions per second; the peaks the
+ * pair kernels' counters are read against.  0, or -1 on failure. */
+int32_t zo_issue_probe(uint32_t kind, uint32_t iters, double* rate_out);
+
 /* Additions of this build (no reference counterpart). */
 /* Per point, the rank j*2*order + 2*k + sign of its nearest orbit point (source j,
  * operator k, sign 0 for +q_j*g_k), from the last zo_spacing; NULL before one. */

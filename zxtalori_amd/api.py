@@ -40,7 +40,7 @@ zo_stats e s;  zo_error e s;  zo_cancel e v
 zo_profile eb b;  zo_profile_report esz z;  zo_pair_budget ef v;  zo_tune_allpairs uu v
 zo_point_indices e U;  zo_force_passes eu i;  zo_near_info eU i;  zo_force_rows eppp i
 zo_near_pairs eQ i;  zo_near_cut eF i;  zo_near_squeeze ei i;  zo_near_resizes e u
-zo_hopf_points uupu u;  zo_hopf_frequency uu u
+zo_hopf_points uupu u;  zo_hopf_frequency uu u;  zo_issue_probe uup i
 zo_batch_create u e;  zo_batch_destroy e v;  zo_batch_error e s;  zo_batch_add eupuufuuQ i
 zo_batch_step eU i;  zo_batch_status eq i;  zo_batch_job_error eq s;  zo_batch_stats eq s
 zo_batch_quaternions eq F;  zo_batch_count eq u;  zo_batch_point_values equ F

@@ -214,6 +214,18 @@ void zo_pair_budget(zxt_engine* e, float scale) { e->engine.pair_budget_scale = 
 void zo_delaunay_mode(zxt_engine*, uint32_t) {}
 void zo_tune_allpairs(uint32_t target, uint32_t ops_per_split) { zxt::set_allpairs_override(target, ops_per_split); }
 
+int32_t zo_issue_probe(uint32_t kind, uint32_t iters, double* rate_out) {
+  if (kind > 1 || iters == 0 || !rate_out) return -1;
+  int sms = 0;
+  if (hipDeviceGetAttribute(&sms, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || sms < 1) return -1;
+  const uint32_t blocks = uint32_t(sms) * 8;  // 8 waves per SIMD
+  float* scratch = nullptr;
+  if (hipMalloc(&scratch, size_t(blocks) * 256 * sizeof(float)) != hipSuccess) return -1;
+  *rate_out = zxt::probe_issue_rate(nullptr, kind, iters, scratch, blocks);
+  (void)hipFree(scratch);
+  return *rate_out > 0 ? 0 : -1;
+}
+
 const uint32_t* zo_point_indices(zxt_engine* e) { return e->engine.point_indices(); }
 int32_t zo_force_passes(zxt_engine* e, uint32_t passes) {
   return guarded(e, "Force passes", [&] { e->engine.force_passes(passes); });

@@ -250,6 +250,10 @@ void launch_cover(Prof& p, hipStream_t s, const float4* q, uint32_t n, const flo
                   uint32_t* vcount = nullptr, uint32_t vcap = 0, uint4* tets = nullptr, uint32_t* tcount = nullptr,
                   uint32_t tcap = 0, uint4* wcode = nullptr);
 void launch_degrees(Prof& p, hipStream_t s, const float* chord, uint32_t n, float* out, uint32_t* least);
+// Issue-rate probe: wave64 instructions per second of non-packed FP32 FMA chains (kind 0)
+// or transcendental chains (kind 1) over `blocks` workgroups of 256 (scratch: blocks x 256
+// floats); -1 on failure.
+double probe_issue_rate(hipStream_t s, uint32_t kind, uint32_t iters, float* scratch, uint32_t blocks);
 
 // BB step (relax.wgsl) of every job in the set, resident: stats, sums, median, move.
 void launch_bb_step(Prof& p, hipStream_t s, const JobSet& jobs);

@@ -2377,4 +2377,51 @@ void launch_bb_step(Prof& p, hipStream_t s, const JobSet& jobs) {
   p.run("bb_move", [&] { hipLaunchKernelGGL(k_bb_move, dim3(jobs.point_total), dim3(kBlock), 0, s, jobs); });
 }
 
+// ---- issue-rate probe (zo_issue_probe) --------------------------------------------------
+// Long independent chains of one instruction kind per lane, on 8 waves per SIMD over the
+// whole chip, timed with events: the rate the VALU issues non-packed FP32 FMAs (8 chains)
+// or transcendentals (v_rsq_f32 and v_rcp_f32, 4 chains) in wave64 instructions per second.
+// The chains stay finite (an FMA contracting toward 1, reciprocals of values in [0.5, 2]).
+__global__ __launch_bounds__(256) void k_probe_fma(float* __restrict__ out, uint32_t iters) {
+  const float m = 0.999f, c = 0.001f;
+  float a[8];
+  for (int k = 0; k < 8; ++k) a[k] = 1.0f + float(threadIdx.x + k) * 1e-3f;
+  for (uint32_t i = 0; i < iters; ++i)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = fmaf(a[k], m, c);
+  out[blockIdx.x * blockDim.x + threadIdx.x] = a[0] + a[1] + a[2] + a[3] + a[4] + a[5] + a[6] + a[7];
+}
+__global__ __launch_bounds__(256) void k_probe_trans(float* __restrict__ out, uint32_t iters) {
+  float a0 = 1.0f + float(threadIdx.x) * 1e-3f, a1 = a0 + 0.25f, a2 = a0 + 0.5f, a3 = a0 + 0.75f;
+  for (uint32_t i = 0; i < iters; ++i) {
+    a0 = __builtin_amdgcn_rsqf(a0);
+    a1 = __builtin_amdgcn_rcpf(a1);
+    a2 = __builtin_amdgcn_rsqf(a2);
+    a3 = __builtin_amdgcn_rcpf(a3);
+  }
+  out[blockIdx.x * blockDim.x + threadIdx.x] = a0 + a1 + a2 + a3;
+}
+
+double probe_issue_rate(hipStream_t s, uint32_t kind, uint32_t iters, float* scratch, uint32_t blocks) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1.0;
+  auto launch = [&](uint32_t it) {
+    if (kind == 0)
+      hipLaunchKernelGGL(k_probe_fma, dim3(blocks), dim3(kBlock), 0, s, scratch, it);
+    else
+      hipLaunchKernelGGL(k_probe_trans, dim3(blocks), dim3(kBlock), 0, s, scratch, it);
+  };
+  launch(64);  // warm-up
+  (void)hipEventRecord(a, s);
+  launch(iters);
+  (void)hipEventRecord(b, s);
+  float ms = 0.0f;
+  const bool ok = hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (!ok || ms <= 0.0f) return -1.0;
+  const double waves = double(blocks) * kBlock / 64.0;
+  return waves * double(iters) * (kind == 0 ? 8.0 : 4.0) / (double(ms) * 1e-3);
+}
+
 }  // namespace zxt
