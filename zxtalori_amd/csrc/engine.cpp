@@ -29,6 +29,7 @@ constexpr uint32_t kMaxCount = 10000000;      // hopf.max_count (src/hopf.zig:23
 constexpr double kFullMeasurePairs = 5e10;    // engine.zig:1123
 constexpr uint32_t kSpacingAllPairs = 20000;  // engine.zig:1126
 constexpr double kSpacingDelta = 4;           // engine.zig:1127
+constexpr double kSpacingCellSites = 0.75;    // sites per cell of the spacing pass's grid
 const char* kVersion = "0.3.0-mi355x";
 
 [[noreturn]] void fail(const std::string& what) { throw EngineError{what}; }
@@ -117,8 +118,9 @@ NearBuffers NearField::view() const {
   b.scode = scode.as<uint32_t>();
   b.srank = srank.as<uint32_t>();
   b.offsets = offsets.as<uint32_t>();
-  b.cursor = cursor.as<uint32_t>();
+  b.uslot = uslot.as<uint32_t>();
   b.order = order_buf.as<uint32_t>();
+  b.opos = uq.as<float4>();  // the unsorted sites are spent once the build has sorted them
   b.scan_tmp = scan_tmp.as<uint32_t>();
   b.counters = counters.as<unsigned long long>();
   b.counts = counts.as<uint32_t>();
@@ -131,7 +133,7 @@ NearBuffers NearField::view() const {
 
 static void release_near(NearField& f) {
   for (DevBuf* b : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
-                    &f.srank, &f.offsets, &f.cursor, &f.order_buf,
+                    &f.srank, &f.offsets, &f.uslot, &f.order_buf,
                     &f.scan_tmp, &f.counters, &f.counts, &f.crowded, &f.huge})
     b->release();
   f.ready = false;
@@ -154,6 +156,11 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   const double c0 = 1.0 - margin / 2.0;
   cut.c0 = float(c0);
   cut.s0 = float(std::sqrt(std::max(0.0, 1.0 - c0 * c0)));
+  const double inner_chord = 2.0 * std::sin(0.5 * radius / 2.0);
+  cut.inner_chord2 = float(inner_chord * inner_chord);
+  const double inner_c0 = 1.0 - (double(cut.inner_chord2) * (1.0 + 1e-5) + 1e-6) / 2.0;
+  cut.inner_c0 = float(inner_c0);
+  cut.inner_s0 = float(std::sqrt(std::max(0.0, 1.0 - inner_c0 * inner_c0)));
   band = float(std::sin(radius + 1e-4));
   // x.g lies at S3 angle acos|g.w| from x (and -x.g at pi minus that) for every x, so the
   // source's own non-identity images enter the cutoff only when r reaches the smallest.
@@ -215,13 +222,14 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   }
   // Sites per cell of the box.  A ball of `reach` mean radii holds about reach^3 sites:
   // thousands at reach 20, where coarse cells (8 sites) win by giving fewer, fuller chunks
-  // of 64 sites per row at the cost of more candidates; tens at reach 4 (the spacing
-  // pass), where fine cells win.  Measured on MI355X at I, N = 1e6: the force pass at
-  // reach 20 takes 15.5 ms with 8 sites per cell against 17.3 ms with 2, at reach 8 2.2 ms
-  // with 4 against 2.6 ms with 2; the reach-4 spacing pass at N = 1e7 (sort and nearest)
-  // 8.3 ms with 1.6 against 8.5 ms with 2 and 9.3 ms with 3.
+  // of 64 sites per row at the cost of more candidates.  Measured on MI355X at I, N = 1e6:
+  // the force pass at reach 20 takes 15.5 ms with 8 sites per cell against 17.3 ms with 2,
+  // at reach 8 2.2 ms with 4 against 2.6 ms with 2.  The spacing pass sets its own
+  // (cell_sites): its first search covers half its reach, a few sites, best found in
+  // fine cells (I, N = 1e7, the whole pass: 4.24 ms with 0.75 sites per cell, 4.33 with
+  // 0.5, 4.40 with 1.0, 4.53 with 1.6, 4.83 with 3).
   const double reach = radius / mean_radius_;
-  const double per_cell = std::min(8.0, std::max(1.5, reach / 2.5));
+  const double per_cell = cell_sites > 0 ? cell_sites : std::min(8.0, std::max(1.5, reach / 2.5));
   sdim = uint32_t(std::min(320.0, std::max(1.0, std::round(std::cbrt(double(count) / per_cell)))));
   const size_t cells = size_t(sdim) * sdim * sdim;
   uq.ensure(size_t(capacity) * 16);
@@ -233,7 +241,7 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   scode.ensure(size_t(capacity) * 4);
   srank.ensure(size_t(capacity) * 4);
   offsets.ensure((cells + 1) * 4);
-  cursor.ensure((cells + 1) * 4);
+  uslot.ensure(size_t(capacity) * 4);
   order_buf.ensure(size_t(n) * 4);
   scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, tile_visits(sdim))) * 4);
   counters.ensure(4 * sizeof(unsigned long long));
@@ -244,9 +252,9 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
 }
 
 void NearField::build(Engine& e, const float4* q, uint32_t n_, uint32_t order_, uint32_t group_,
-                      double mean_radius_) {
+                      double mean_radius_, bool ordered) {
   if (!ready || n != n_ || order != order_ || group != group_) setup(e, q, n_, order_, group_, mean_radius_);
-  launch_near_build(e.prof, e.stream, q, n, e.ops_device(), order, view());
+  launch_near_build(e.prof, e.stream, q, n, e.ops_device(), order, view(), ordered);
 }
 
 bool NearField::overflowed(Engine& e) {
@@ -670,22 +678,24 @@ void Engine::measure() {
 double Engine::nearest_pass(RelaxState& rs) {
   for (int attempt = 0; attempt < 4; ++attempt) {
     rs.current = false;
-    rs.nn_chord.ensure(size_t(n_) * 4);
     rs.nn_rank.ensure(size_t(n_) * 4);
-    float4* q = q_.as<float4>();
-    if (rs.has_near) {
-      rs.near.build(*this, q, n_, order(), group_, mean_radius(uint64_t(n_) * order()));
-      launch_near_nearest(prof, stream, q, n_, order(), rs.near.view(), rs.near.cut, rs.nn_chord.as<float>(),
-                          rs.nn_rank.as<uint32_t>());
-    } else {
-      rs.nn_partial.ensure(size_t(order()) * n_ * 8);
-      launch_allpairs_nearest(prof, stream, q, n_, ops_device(), order(), rs.nn_partial.as<uint64_t>(),
-                              rs.nn_chord.as<float>(), rs.nn_rank.as<uint32_t>());
-    }
     rs.nn_degrees.ensure(size_t(n_) * 4 + 4);
     float* degrees = rs.nn_degrees.as<float>();
     uint32_t* least_bits = reinterpret_cast<uint32_t*>(degrees + n_);
-    launch_degrees(prof, stream, rs.nn_chord.as<float>(), n_, degrees, least_bits);
+    float4* q = q_.as<float4>();
+    if (rs.has_near) {
+      // The nearest orbit point is a minimum over keys: the sites' order within a cell
+      // does not matter.
+      rs.near.build(*this, q, n_, order(), group_, mean_radius(uint64_t(n_) * order()), false);
+      launch_near_nearest(prof, stream, n_, order(), rs.near.view(), rs.near.cut, degrees, rs.nn_rank.as<uint32_t>(),
+                          least_bits);
+    } else {
+      rs.nn_chord.ensure(size_t(n_) * 4);
+      rs.nn_partial.ensure(size_t(order()) * n_ * 8);
+      launch_allpairs_nearest(prof, stream, q, n_, ops_device(), order(), rs.nn_partial.as<uint64_t>(),
+                              rs.nn_chord.as<float>(), rs.nn_rank.as<uint32_t>());
+      launch_degrees(prof, stream, rs.nn_chord.as<float>(), n_, degrees, least_bits);
+    }
     auto& out = separation_out();
     uint32_t least_key = 0;
     check(hipMemcpyAsync(out.data(), degrees, size_t(n_) * 4, hipMemcpyDeviceToHost, stream), "download");
@@ -706,6 +716,7 @@ void Engine::spacing() {
   if (n_ < 2) fail("TooFewOrientations");
   const double delta = kSpacingDelta;
   spacing_state_.set_near(group_, n_ > kSpacingAllPairs ? &delta : nullptr);
+  spacing_state_.near.cell_sites = kSpacingCellSites;
   const double least = nearest_pass(spacing_state_);
   if (measured_ && measure_iteration_ == iteration_) {
     measures_.separation_deg = least;
@@ -830,7 +841,7 @@ size_t Engine::device_bytes() const {
       b += d->bytes;
     const NearField& f = rs->near;
     for (const DevBuf* d : {&f.planes, &f.info, &f.rim, &f.uq, &f.ucode, &f.urank, &f.ucell, &f.perm, &f.sq, &f.scode,
-                            &f.srank, &f.offsets, &f.cursor,
+                            &f.srank, &f.offsets, &f.uslot,
                             &f.order_buf, &f.scan_tmp, &f.counters, &f.counts, &f.crowded, &f.huge})
       b += d->bytes;
   }
@@ -1362,6 +1373,7 @@ void Engine::optimize_covering(uint32_t steps, uint32_t mode, bool finish, float
     if (floor_deg <= 0.0f) return true;
     const double delta = kSpacingDelta;
     spacing_state_.set_near(group_, n_ > kSpacingAllPairs ? &delta : nullptr);
+    spacing_state_.near.cell_sites = kSpacingCellSites;
     return nearest_pass(spacing_state_) >= double(floor_deg);
   };
   uint64_t rng = 0x9E3779B97F4A7C15ull ^ (uint64_t(opt_.kicks + 1) << 17);

@@ -114,6 +114,7 @@ class Engine;
 struct NearField {
   double delta = 0;
   double taper = 0.5;
+  double cell_sites = 0;  // sites per cell the grid aims for (0: chosen from the reach)
   uint32_t n = 0, order = 0, group = 0;
   bool ready = false;
   double radius = 0;
@@ -122,12 +123,14 @@ struct NearField {
   float band = 0;
   // The last pass's sites, shell and overflow flag (valid after a wait).
   NearInfo status{};
-  DevBuf planes, info, rim, uq, ucode, urank, ucell, perm, sq, scode, srank, offsets, cursor, order_buf,
+  DevBuf planes, info, rim, uq, ucode, urank, ucell, perm, sq, scode, srank, offsets, uslot, order_buf,
       scan_tmp, counters, counts, crowded, huge;
   NearBuffers view() const;
   void setup(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius);
-  // Records sites and sorts for the positions q (setting up first when needed).
-  void build(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius);
+  // Records sites and sorts for the positions q (setting up first when needed); `ordered`
+  // as launch_near_build takes it.
+  void build(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius,
+             bool ordered = true);
   // After a wait: whether the last pass overflowed (then the next pass sizes again).
   bool overflowed(Engine& e);
 };

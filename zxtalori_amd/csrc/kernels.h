@@ -30,6 +30,11 @@ struct NearCut {
   float c0;           // x.y >= c0 bounds every accepted site (with margin)
   float s0;           // sqrt(1 - c0^2)
   uint32_t own;       // the source's own non-identity images can lie within the cutoff
+  // The spacing pass first searches the cap of half the reach: inner_c0, inner_s0 bound
+  // every site whose chord^2 passes inner_chord2, as c0, s0 do for chord2.
+  float inner_chord2;
+  float inner_c0;
+  float inner_s0;
 };
 
 // Device-resident status of one near-field pass.  lo/hi are order-preserving u32 keys of
@@ -197,8 +202,9 @@ struct NearBuffers {
   uint32_t* scode = nullptr;     // capacity
   uint32_t* srank = nullptr;     // capacity
   uint32_t* offsets = nullptr;   // sdim^3 + 1
-  uint32_t* cursor = nullptr;    // sdim^3 + 1
+  uint32_t* uslot = nullptr;     // capacity: each site's arrival slot in its cell
   uint32_t* order = nullptr;     // n, sources in the tile-by-tile visit order of their identity images
+  float4* opos = nullptr;        // n, their positions in that order (the identity images' sites; shares uq)
   uint32_t* scan_tmp = nullptr;  // scan_tmp_words(max(cells, capacity))
   unsigned long long* counters = nullptr;  // [0] accepted pairs, [1] sites tested (force pass)
   uint32_t* counts = nullptr;    // 2n: per source, pairs and sites tested of the last force pass
@@ -226,14 +232,18 @@ size_t scan_tmp_words(size_t count);
 void launch_near_sites(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                        const NearBuffers& b);
 // Builds the sites (core + shell), sorts them into the cell grid and orders the sources
-// by the tile-by-tile visit order of their identity images.  Reads nothing back.
+// by the tile-by-tile visit order of their identity images.  Reads nothing back.  With
+// `ordered`, the sites of every cell are in rank order (sums over them then run in the same
+// order every pass); without it, in arrival order (enough for minima, such as spacing).
 void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
-                       const NearBuffers& b);
+                       const NearBuffers& b, bool ordered = true);
 // Force rows (mode 0) or nearest orbit point with rank (mode 1) over the built sites.
 void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, const NearBuffers& b, const NearCut& cut,
                        Rows out);
-void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, uint32_t order, const NearBuffers& b,
-                         const NearCut& cut, float* chord_out, uint32_t* rank_out);
+// Spacing over the built sites: per source the SO(3) degrees of its nearest orbit point
+// and that point's rank, and the least degree's bits (*least_bits).
+void launch_near_nearest(Prof& p, hipStream_t s, uint32_t n, uint32_t order, const NearBuffers& b, const NearCut& cut,
+                         float* degrees_out, uint32_t* rank_out, uint32_t* least_bits);
 
 // SO(3) degrees of nearest chords (out) and their minimum's bits (*least).
 // Covering radius per source (S3 angle of its Voronoi cell's farthest vertex), the vertex,

@@ -998,6 +998,7 @@ __device__ inline uint32_t cell_axis(float v, float lower, float cpu, uint32_t d
   return uint32_t(fminf(fmaxf(c, 0.0f), float(dim - 1)));
 }
 
+// Each site's cell and its arrival slot in that cell (the count before its own).
 __global__ void k_site_count(NearBuffers b) {
   const uint32_t S = min(b.info->sites, b.capacity);
   const float lx = b.info->lower[0], ly = b.info->lower[1], lz = b.info->lower[2], cpu = b.info->cpu;
@@ -1006,48 +1007,65 @@ __global__ void k_site_count(NearBuffers b) {
     const float3 p = chart(b.uq[s]);
     const uint32_t c = cell_axis(p.x, lx, cpu, d) + d * (cell_axis(p.y, ly, cpu, d) + d * cell_axis(p.z, lz, cpu, d));
     b.ucell[s] = c;
-    atomicAdd(&b.offsets[c], 1u);
+    b.uslot[s] = atomicAdd(&b.offsets[c], 1u);
   }
 }
 
-__global__ void k_copy_u32(const uint32_t* __restrict__ a, uint32_t* __restrict__ b, size_t count) {
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
-    b[i] = a[i];
-}
-
-__global__ void k_site_scatter(NearBuffers b) {
-  const uint32_t S = min(b.info->sites, b.capacity);
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x)
-    b.perm[atomicAdd(&b.cursor[b.ucell[s]], 1u)] = s;
-}
-
-// In-cell order by a unique key (the site's rank, or the item itself), so sums run in
-// the same order every pass (near.wgsl:70-82).  One thread sorts a cell of up to
-// kThreadCell items; larger cells are listed (when `crowded` is given) for
-// k_cell_sort_waves.
+// Cells of up to kThreadCell sites are sorted by one thread, in place; larger ones through
+// an index list (see k_cell_sort_sites).
 constexpr uint32_t kThreadCell = 16;
-__global__ void k_cell_sort(const uint32_t* __restrict__ offsets, uint32_t cells, uint32_t* __restrict__ perm,
-                            const uint32_t* __restrict__ key, uint32_t* __restrict__ crowded_list,
-                            uint32_t* __restrict__ crowded) {
+
+// With the counts scanned into offsets, each site moves to its cell's start plus its slot:
+// the site itself into a cell of at most `direct` sites, its index (perm) into a larger
+// one, whose sites the crowded-cell sorts move once ordered.
+__global__ void k_site_place(NearBuffers b, uint32_t direct) {
+  const uint32_t S = min(b.info->sites, b.capacity);
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
+    const uint32_t c = b.ucell[s];
+    const uint32_t begin = b.offsets[c], dst = begin + b.uslot[s];
+    if (b.offsets[c + 1] - begin > direct) {
+      b.perm[dst] = s;
+    } else {
+      b.sq[dst] = b.uq[s];
+      b.scode[dst] = b.ucode[s];
+      b.srank[dst] = b.urank[s];
+    }
+  }
+}
+
+// Moves unsorted site s to sorted position k.
+__device__ inline void move_site(const NearBuffers& b, uint32_t k, uint32_t s) {
+  b.sq[k] = b.uq[s];
+  b.scode[k] = b.ucode[s];
+  b.srank[k] = b.urank[s];
+}
+
+// In-cell order by the site's unique rank, so sums run in the same order every pass
+// (near.wgsl:70-82).  One thread sorts a cell of up to kThreadCell sites in place (by
+// insertion); larger cells are listed for k_cell_sort_waves.
+__global__ void k_cell_sort_sites(NearBuffers b, uint32_t cells) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cells) return;
-  const uint32_t begin = offsets[c], end = offsets[c + 1];
+  const uint32_t begin = b.offsets[c], end = b.offsets[c + 1];
   if (end - begin < 2) return;
   if (end - begin > kThreadCell) {
-    if (crowded) crowded_list[atomicAdd(crowded, 1u)] = c;
+    b.crowded[atomicAdd(&b.info->crowded, 1u)] = c;
     return;
   }
   for (uint32_t at = begin + 1; at < end; ++at) {
-    const uint32_t v = perm[at];
-    const uint32_t kv = key ? key[v] : v;
+    const uint32_t kv = b.srank[at];
+    const float4 qv = b.sq[at];
+    const uint32_t cv = b.scode[at];
     uint32_t before = at;
-    while (before > begin) {
-      const uint32_t u = perm[before - 1];
-      if ((key ? key[u] : u) <= kv) break;
-      perm[before] = u;
+    while (before > begin && b.srank[before - 1] > kv) {
+      b.sq[before] = b.sq[before - 1];
+      b.scode[before] = b.scode[before - 1];
+      b.srank[before] = b.srank[before - 1];
       --before;
     }
-    perm[before] = v;
+    b.sq[before] = qv;
+    b.scode[before] = cv;
+    b.srank[before] = kv;
   }
 }
 
@@ -1065,7 +1083,8 @@ __global__ __launch_bounds__(256) void k_cell_sort_waves(const uint32_t* __restr
                                                          uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
                                                          const uint32_t* __restrict__ crowded_list,
                                                          const uint32_t* __restrict__ crowded, uint32_t list_capacity,
-                                                         uint32_t* __restrict__ huge_list, uint32_t* __restrict__ huge) {
+                                                         uint32_t* __restrict__ huge_list, uint32_t* __restrict__ huge,
+                                                         NearBuffers sites) {
   __shared__ SortLds lds[kBlock / 64];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   SortLds& L = lds[wave];
@@ -1087,7 +1106,10 @@ __global__ __launch_bounds__(256) void k_cell_sort_waves(const uint32_t* __restr
       wave_sync();
       uint32_t rank = 0;
       for (uint32_t j = 0; j < m; ++j) rank += L.k[j] < k ? 1u : 0u;
-      if (lane < m) perm[begin + rank] = v;
+      if (lane < m) {
+        perm[begin + rank] = v;
+        move_site(sites, begin + rank, v);
+      }
       wave_sync();
       continue;
     }
@@ -1116,7 +1138,10 @@ __global__ __launch_bounds__(256) void k_cell_sort_waves(const uint32_t* __restr
         wave_sync();
       }
     }
-    for (uint32_t t = lane; t < m; t += 64) perm[begin + t] = L.v[t];
+    for (uint32_t t = lane; t < m; t += 64) {
+      perm[begin + t] = L.v[t];
+      move_site(sites, begin + t, L.v[t]);
+    }
     wave_sync();
   }
 }
@@ -1130,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_cell_sort_block(const uint32_t* __restr
                                                          uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
                                                          const uint32_t* __restrict__ huge_list,
                                                          const uint32_t* __restrict__ huge, uint32_t list_capacity,
-                                                         uint32_t* __restrict__ tmp) {
+                                                         uint32_t* __restrict__ tmp, NearBuffers sites) {
   __shared__ uint32_t sk[kBlockChunk], sv[kBlockChunk];
   const uint32_t total = min(*huge, list_capacity);
   for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
@@ -1193,17 +1218,8 @@ __global__ __launch_bounds__(256) void k_cell_sort_block(const uint32_t* __restr
       for (uint32_t at = threadIdx.x; at < m; at += blockDim.x) perm[begin + at] = tmp[begin + at];
       __syncthreads();
     }
-  }
-}
-
-// Sites in cell order.
-__global__ void k_site_gather(NearBuffers b) {
-  const uint32_t S = min(b.info->sites, b.capacity);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < S; k += gridDim.x * blockDim.x) {
-    const uint32_t s = b.perm[k];
-    b.sq[k] = b.uq[s];
-    b.scode[k] = b.ucode[s];
-    b.srank[k] = b.urank[s];
+    for (uint32_t at = threadIdx.x; at < m; at += blockDim.x) move_site(sites, begin + at, perm[begin + at]);
+    __syncthreads();
   }
 }
 
@@ -1242,7 +1258,10 @@ __global__ void k_core_order(NearBuffers b) {
     uint32_t at = b.ucell[v];
     for (uint32_t k = b.offsets[cell]; k < b.offsets[cell + 1]; ++k) {
       const uint32_t code = b.scode[k];
-      if (code >> 31) b.order[at++] = code & 0x7fffffffu;
+      if (code >> 31) {
+        b.order[at] = code & 0x7fffffffu;
+        b.opos[at++] = b.sq[k];  // the identity image: the source's position, bit for bit
+      }
     }
   }
 }
@@ -1263,28 +1282,23 @@ void launch_near_sites(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 }
 
 void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
-                       const NearBuffers& b) {
+                       const NearBuffers& b, bool ordered) {
   const size_t cells = size_t(b.sdim) * b.sdim * b.sdim;
   launch_fill_u32(p, s, b.offsets, 0u, cells + 1);
   launch_near_sites(p, s, q, n, ops, order, b);
   const uint32_t g = stride_grid(b.capacity);
   p.run("site_count", [&] { hipLaunchKernelGGL(k_site_count, dim3(g), dim3(kBlock), 0, s, b); });
   launch_scan(p, s, b.offsets, cells, b.scan_tmp);
-  p.run("copy", [&] {
-    hipLaunchKernelGGL(k_copy_u32, dim3(stride_grid(cells)), dim3(kBlock), 0, s, b.offsets, b.cursor, cells);
+  p.run("site_place", [&] {
+    hipLaunchKernelGGL(k_site_place, dim3(g), dim3(kBlock), 0, s, b, ordered ? kThreadCell : 0xffffffffu);
   });
-  p.run("site_scatter", [&] { hipLaunchKernelGGL(k_site_scatter, dim3(g), dim3(kBlock), 0, s, b); });
-  p.run("site_sort", [&] {
-    hipLaunchKernelGGL(k_cell_sort, dim3(blocks_for(cells, kBlock)), dim3(kBlock), 0, s, b.offsets, uint32_t(cells),
-                       b.perm, b.urank, b.crowded, b.crowded ? &b.info->crowded : nullptr);
-    if (b.crowded) {
-      hipLaunchKernelGGL(k_cell_sort_waves, dim3(kCrowdedBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank,
-                         b.crowded, &b.info->crowded, b.crowded_capacity, b.huge, &b.info->huge);
-      hipLaunchKernelGGL(k_cell_sort_block, dim3(kHugeBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank,
-                         b.huge, &b.info->huge, b.huge_capacity, b.ucell);
-    }
+  if (ordered) p.run("site_sort", [&] {
+    hipLaunchKernelGGL(k_cell_sort_sites, dim3(blocks_for(cells, kBlock)), dim3(kBlock), 0, s, b, uint32_t(cells));
+    hipLaunchKernelGGL(k_cell_sort_waves, dim3(kCrowdedBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank,
+                       b.crowded, &b.info->crowded, b.crowded_capacity, b.huge, &b.info->huge, b);
+    hipLaunchKernelGGL(k_cell_sort_block, dim3(kHugeBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank, b.huge,
+                       &b.info->huge, b.huge_capacity, b.ucell, b);
   });
-  p.run("site_gather", [&] { hipLaunchKernelGGL(k_site_gather, dim3(g), dim3(kBlock), 0, s, b); });
   const uint32_t visits = tile_visits(b.sdim), gv = stride_grid(visits);
   p.run("core_order", [&] { hipLaunchKernelGGL(k_core_count, dim3(gv), dim3(kBlock), 0, s, b); });
   launch_scan(p, s, b.ucell, visits, b.scan_tmp);
@@ -1550,91 +1564,136 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 // one per lane, then walks them (group-uniform), its lanes testing consecutive sites of a
 // row; the key (chord^2 bits, rank) makes the minimum take the lowest rank on ties.  The
 // source's own identity image is skipped by its rank.
+//
+// The search runs first over the cap of half the reach (an eighth of the volume, about 7
+// orbit points on average): when its least chord^2 is within that cap's cutoff, every site
+// that could be nearer (or tie) lies in the cells it scanned, so it is the minimum over the
+// whole cutoff, the reference's result.  Only the rare sources with nothing that near are
+// searched again over the full cap.
 constexpr uint32_t kSpacingLanes = 8;
-__global__ __launch_bounds__(256) void k_spacing(const float4* __restrict__ q, uint32_t n,
-                                                 const uint32_t* __restrict__ order, const float4* __restrict__ sq,
+__global__ __launch_bounds__(256) void k_spacing(uint32_t n, const uint32_t* __restrict__ order,
+                                                 const float4* __restrict__ sq,
                                                  const uint32_t* __restrict__ srank,
                                                  const uint32_t* __restrict__ offsets,
                                                  const NearInfo* __restrict__ info, uint32_t sdim, NearCut cut,
                                                  uint32_t two_order, uint32_t real_blocks,
-                                                 float* __restrict__ chord_out, uint32_t* __restrict__ rank_out) {
+                                                 const float4* __restrict__ opos, uint64_t* __restrict__ key_out) {
   constexpr uint32_t G = kSpacingLanes;
-  // Workgroups in launch order: for these short per-source scans that measured faster (4.1
-  // against 4.2 ms at N = 1e7) than one contiguous source range per XCD.
   const uint32_t vb = blockIdx.x;
   if (vb >= real_blocks) return;
   const uint32_t sub = threadIdx.x % G;
   const uint32_t slot = vb * (kBlock / G) + threadIdx.x / G;
   if (slot >= n) return;  // the whole group leaves (groups never straddle a wave)
   const uint32_t src = order[slot];
-  const float4 x = q[src];
+  const float4 x = opos[slot];
   const float cpu = info->cpu;
   const float fdim = float(sdim);
-  const float denom = cut.c0 + x.x;
-  const bool full = !(denom > 1e-3f);
-  float cx = 0.5f * fdim, cy = 0.5f * fdim, cz = 0.5f * fdim, rc = 3e18f;
-  if (!full) {
-    const float inv = 1.0f / denom;
-    cx = (x.y * inv - info->lower[0]) * cpu;
-    cy = (x.z * inv - info->lower[1]) * cpu;
-    cz = (x.w * inv - info->lower[2]) * cpu;
-    rc = (cut.s0 * inv * 1.001f + 1e-6f) * cpu + 1e-3f;
-  }
+  const float lower0 = info->lower[0], lower1 = info->lower[1], lower2 = info->lower[2];
   auto clampc = [&](float v) -> int { return int(fminf(fmaxf(floorf(v), 0.0f), fdim - 1.0f)); };
-  const int zlo = clampc(cz - rc), zhi = clampc(cz + rc), ylo = clampc(cy - rc), yhi = clampc(cy + rc);
-  const uint32_t ny = uint32_t(yhi - ylo + 1), nrows = ny * uint32_t(zhi - zlo + 1);
-  const float rc2 = rc * rc;
   const uint32_t self_rank = src * two_order;
   uint64_t best = ~uint64_t(0);
-  for (uint32_t rb = 0; rb < nrows; rb += G) {
-    const uint32_t r = rb + sub;
-    uint32_t start = 0, len = 0;
-    if (r < nrows) {
-      const int yy = ylo + int(r % ny), zz = zlo + int(r / ny);
-      const float gy = row_gap(cy, yy, sdim), gz = row_gap(cz, zz, sdim);
-      const float rest = rc2 - gy * gy - gz * gz;
-      if (rest >= 0.0f) {
-        const float span = full ? 3e18f : sqrtf(rest);
-        const int x0 = clampc(cx - span), x1 = clampc(cx + span);
-        const uint32_t row = sdim * (uint32_t(yy) + sdim * uint32_t(zz));
-        start = offsets[row + x0];
-        len = offsets[row + x1 + 1] - start;
-      }
+  // Every site in the cells the chart ball of the cap {y : x.y >= c0} (radius s0 / (c0 + w))
+  // crosses, rows trimmed in x.
+  auto scan = [&](float c0, float s0) {
+    const float denom = c0 + x.x;
+    const bool full = !(denom > 1e-3f);
+    float cx = 0.5f * fdim, cy = 0.5f * fdim, cz = 0.5f * fdim, rc = 3e18f;
+    if (!full) {
+      const float inv = 1.0f / denom;
+      cx = (x.y * inv - lower0) * cpu;
+      cy = (x.z * inv - lower1) * cpu;
+      cz = (x.w * inv - lower2) * cpu;
+      rc = (s0 * inv * 1.001f + 1e-6f) * cpu + 1e-3f;
     }
-    const uint32_t rows_here = min(G, nrows - rb);
-    for (uint32_t k = 0; k < rows_here; ++k) {
-      const uint32_t n_row = uint32_t(__shfl(int(len), int(k), int(G)));
-      const uint32_t s_row = uint32_t(__shfl(int(start), int(k), int(G)));
-      for (uint32_t c = sub; c < n_row; c += G) {
-        const uint32_t site = s_row + c;
-        const float4 y = sq[site];
-        const uint32_t rank = srank[site];
-        const float dm2 = chord2(x, y);
-        if (dm2 <= cut.chord2 && rank != self_rank) {
-          const uint64_t key = (uint64_t(__float_as_uint(dm2)) << 32) | rank;
-          best = key < best ? key : best;
+    const int zlo = clampc(cz - rc), zhi = clampc(cz + rc), ylo = clampc(cy - rc), yhi = clampc(cy + rc);
+    const uint32_t ny = uint32_t(yhi - ylo + 1), nrows = ny * uint32_t(zhi - zlo + 1);
+    const float rc2 = rc * rc;
+    for (uint32_t rb = 0; rb < nrows; rb += G) {
+      const uint32_t r = rb + sub;
+      uint32_t start = 0, len = 0;
+      if (r < nrows) {
+        const int yy = ylo + int(r % ny), zz = zlo + int(r / ny);
+        const float gy = row_gap(cy, yy, sdim), gz = row_gap(cz, zz, sdim);
+        const float rest = rc2 - gy * gy - gz * gz;
+        if (rest >= 0.0f) {
+          const float span = full ? 3e18f : sqrtf(rest);
+          const int x0 = clampc(cx - span), x1 = clampc(cx + span);
+          const uint32_t row = sdim * (uint32_t(yy) + sdim * uint32_t(zz));
+          start = offsets[row + x0];
+          len = offsets[row + x1 + 1] - start;
+        }
+      }
+      // The group's rows as one run of candidates: candidate t lies in the row k whose
+      // inclusive prefix is the first above t (a binary search over the group's lanes), so
+      // each lane's candidates load independently of the others'.
+      uint32_t incl = len;
+      for (uint32_t d = 1; d < G; d <<= 1) {
+        const uint32_t below = uint32_t(__shfl_up(int(incl), d, int(G)));
+        if (sub >= d) incl += below;
+      }
+      const uint32_t total = uint32_t(__shfl(int(incl), int(G - 1), int(G)));
+      const uint32_t first = start - (incl - len);  // start of the row minus the candidates before it
+      // Group-uniform trip count: every lane of the group takes part in the shuffles.
+      for (uint32_t base = 0; base < total; base += G) {
+        const uint32_t t = base + sub;
+        uint32_t k = 0;
+        for (uint32_t step = G / 2; step > 0; step >>= 1)
+          if (uint32_t(__shfl(int(incl), int(k + step - 1), int(G))) <= t) k += step;
+        const uint32_t site = uint32_t(__shfl(int(first), int(k), int(G))) + t;
+        if (t < total) {
+          const float4 y = sq[site];
+          const uint32_t rank = srank[site];
+          const float dm2 = chord2(x, y);
+          if (dm2 <= cut.chord2 && rank != self_rank) {
+            const uint64_t key = (uint64_t(__float_as_uint(dm2)) << 32) | rank;
+            best = key < best ? key : best;
+          }
         }
       }
     }
-  }
-  for (uint32_t d = G / 2; d > 0; d >>= 1) {
-    const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(best)), int(d), int(G)));
-    const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(best >> 32)), int(d), int(G)));
-    const uint64_t o = (uint64_t(hi) << 32) | lo;
-    best = o < best ? o : best;
-  }
-  if (sub == 0) {
-    chord_out[src] = best == ~uint64_t(0) ? 4.0f : __uint_as_float(uint32_t(best >> 32));
-    rank_out[src] = best == ~uint64_t(0) ? 0xffffffffu : uint32_t(best);
-  }
+    for (uint32_t d = G / 2; d > 0; d >>= 1) {
+      const uint32_t lo = uint32_t(__shfl_xor(int(uint32_t(best)), int(d), int(G)));
+      const uint32_t hi = uint32_t(__shfl_xor(int(uint32_t(best >> 32)), int(d), int(G)));
+      const uint64_t o = (uint64_t(hi) << 32) | lo;
+      best = o < best ? o : best;
+    }
+  };
+  scan(cut.inner_c0, cut.inner_s0);
+  // Chords are nonnegative, so their bits order as unsigned integers; best is group-uniform.
+  if (uint32_t(best >> 32) > __float_as_uint(cut.inner_chord2)) scan(cut.c0, cut.s0);
+  // One 8-byte store per source (the sources' indices are scattered): chord^2 bits and rank.
+  if (sub == 0) key_out[src] = best;
 }
 
-void launch_near_nearest(Prof& p, hipStream_t s, const float4* q, uint32_t n, uint32_t order, const NearBuffers& b,
-                         const NearCut& cut, float* chord_out, uint32_t* rank_out) {
+// Spacing keys (chord^2 bits, rank; all ones for none within the cutoff) to SO(3) degrees
+// (as k_degrees computes them) and ranks, and the least degree's bits into *least.
+__global__ void k_nearest_unpack(const uint64_t* __restrict__ keys, uint32_t n, float* __restrict__ degrees,
+                                 uint32_t* __restrict__ rank_out, uint32_t* __restrict__ least) {
+  uint32_t lo = 0xffffffffu;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t key = keys[i];
+    const float chord = key == ~uint64_t(0) ? 4.0f : __uint_as_float(uint32_t(key >> 32));
+    const float d = 4.0f * asinf(fminf(1.0f, sqrtf(chord) * 0.5f)) * (180.0f / kPiF);
+    degrees[i] = d;
+    rank_out[i] = key == ~uint64_t(0) ? 0xffffffffu : uint32_t(key);
+    lo = min(lo, __float_as_uint(d));
+  }
+  lo = wave_min_u32(lo);
+  if ((threadIdx.x & 63) == 0) atomicMin(least, lo);
+}
+
+void launch_near_nearest(Prof& p, hipStream_t s, uint32_t n, uint32_t order, const NearBuffers& b, const NearCut& cut,
+                         float* degrees_out, uint32_t* rank_out, uint32_t* least_bits) {
   const uint32_t real = blocks_for(n, kBlock / kSpacingLanes);
+  uint64_t* keys = reinterpret_cast<uint64_t*>(b.counts);  // 2n words: one key per source
   p.run("near_nearest", [&] {
-    hipLaunchKernelGGL(k_spacing, dim3(real), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.srank, b.offsets,
-                       b.info, b.sdim, cut, 2 * order, real, chord_out, rank_out);
+    hipLaunchKernelGGL(k_spacing, dim3(real), dim3(kBlock), 0, s, n, b.order, b.sq, b.srank, b.offsets, b.info,
+                       b.sdim, cut, 2 * order, real, b.opos, keys);
+  });
+  launch_fill_u32(p, s, least_bits, 0xffffffffu, 1);
+  p.run("degrees", [&] {
+    hipLaunchKernelGGL(k_nearest_unpack, dim3(std::min<uint32_t>(2048, blocks_for(n, kBlock))), dim3(kBlock), 0, s, keys,
+                       n, degrees_out, rank_out, least_bits);
   });
 }
 
