@@ -284,6 +284,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--serving-requests", type=int, default=40, help="0 skips the serving run")
     ap.add_argument("--serving-rate", type=float, default=4.0)
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the small all-pairs relax (O, 1e3, 30 steps) and near relax (O, 1e5, 100 steps)")
     ap.add_argument("--no-widened", action="store_true",
                     help="skip the generation, covering and exact-count measurements")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -384,6 +386,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(q, ops, cut, radius, args.cpu_sample)
     if args.spacing_n > 0:
         out["spacing"] = spacing_section(args.spacing_n)
+    if not args.no_small:
+        out["small_relax"] = small_relax_section(with_cpu=not args.no_cpu)
+        out["near_relax"] = near_relax_section(with_cpu=not args.no_cpu)
     if args.batched_jobs > 0:
         out["batched"] = batched_section(args.batched_jobs, args.batched_steps)
     if args.serving_requests > 0:
@@ -482,6 +487,115 @@ def spacing_section(n):
             "points_per_s": n / dt, "alg_bytes_per_point": 76,
             "alg_gbps_device": 76 * n / (device_ms * 1e-3) / 1e9, "kernels_ms": kernels,
             "sites": info["sites"], "separation_deg": stats["separation_deg"]}
+
+
+def _median_of(run, repeats):
+    """Median wall time (s) of `repeats` calls of run() after one warm-up call, and the last
+    call's result."""
+    run()
+    times, last = [], None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        last = run()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), last
+
+
+def small_relax_section(repeats=5, with_cpu=True):
+    """The small all-pairs workload: group O, N=1,000 normals (default_rng(193)), 30 BB steps
+    (neighbourhood 0), FP32 pair terms with FP64 sums; wall time of relax(30) (median of 5
+    after a warm-up), energy, separation, spacing, launches per step; the CPU restatement of
+    the same 30 steps on all host cores beside it."""
+    import oracle
+    import zxtalori_amd as zo
+    print("[bench] small relax O N=1000 x 30", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(193)
+    q0 = rng.standard_normal((1000, 4))
+    q0 = (q0 / np.linalg.norm(q0, axis=1, keepdims=True)).astype(np.float32)
+
+    def gpu():
+        with zo.from_quaternions(q0, "O") as grid:
+            t0 = time.perf_counter()
+            grid.relax(30)
+            dt = time.perf_counter() - t0
+            return dt, grid.stats
+    gpu()
+    runs = [gpu() for _ in range(repeats)]
+    wall = float(np.median([r[0] for r in runs]))
+    stats = runs[-1][1]
+    with zo.from_quaternions(q0, "O") as grid:
+        start, ops = grid.quaternions, grid.operations
+        grid.profile(True)
+        grid.relax(30)
+        launches = sum(c for c, _ in grid.profile_report().values())
+        grid.profile(False)
+        spacing = grid.spacing()["separation_deg"]
+    out = {"workload": "O, N=1000 (default_rng(193) normals), 30 BB steps, all pairs", "wall_ms": wall * 1e3,
+           "steps_per_s": 30 / wall, "energy": stats["energy"], "separation_deg": stats["separation_deg"],
+           "spacing_separation_deg": spacing, "launches_per_step": launches / 30,
+           "pair_evals_per_s": 31 * 1000 * 999 * 24 / wall}
+    if with_cpu:
+        cores = oracle.threads(0)
+        cpu_s, (_, rows, _) = _median_of(lambda: oracle.relax(start, ops, 9, 30), repeats)
+        out["cpu_baseline"] = {"wall_ms": cpu_s * 1e3, "cores": cores, "kind": "port",
+                               "energy": rows[:, 4].sum() / (1000 * 24 * 999),
+                               "sample": "the CPU restatement (oracle.relax: orc_allpairs, orc_bb_step, orc_fold) "
+                                         "of the same 30 steps, median of 5 after a warm-up"}
+    return out
+
+
+def near_relax_section(steps=100, cpu_repeats=5, with_cpu=True):
+    """Group O, N=1e5 Haar (seed 193), neighbourhood 20, 100 BB steps: per-step time, then the
+    rows of the first 10,000 sources against FP64 pair terms over the same pair set (energy,
+    force, separation errors); beside it the CPU restatement's per-step time (one near-field
+    pass over all sources, the BB step and the fold), median of 5 single steps after a
+    warm-up step, on all host cores."""
+    import oracle
+    import zxtalori_amd as zo
+    from oracle import dense_f64 as dense
+    print(f"[bench] near relax O N=1e5 x {steps}", file=sys.stderr, flush=True)
+    with zo.haar("O", 100_000, 193) as grid:
+        grid.relax(2, neighbourhood=20)  # warm-up (sizing, code paths)
+    with zo.haar("O", 100_000, 193) as grid:
+        start, ops = grid.quaternions, grid.operations
+        grid.relax(0, neighbourhood=20)
+        t0 = time.perf_counter()
+        grid.relax(steps)
+        dt = time.perf_counter() - t0
+        stats = grid.stats
+        force, energy, nearest = grid.force_rows()
+        q = grid.quaternions
+    print("[bench] near relax: fp64 subsample", file=sys.stderr, flush=True)
+    with np.load(ROOT / "tests" / "golden" / "pair-operators.npz", allow_pickle=False) as data:
+        ops64 = np.asarray(data["O"], np.float64)
+    cut, r = dense.near_cut(len(q), len(ops), 20)
+    ref = oracle.near_grid64(q, ops, ops64, cut, r, 0, 10_000)
+    e_err = abs(energy[:10_000].sum() - ref[:, 4].sum()) / ref[:, 4].sum()
+    f_err = np.linalg.norm(force[:10_000, :3] - ref[:, :3]) / np.linalg.norm(ref[:, 6])
+    sep = oracle.degrees(nearest[:10_000])
+    sep64 = 4 * np.degrees(np.arcsin(np.minimum(1, np.sqrt(ref[:, 5]) / 2)))
+    out = {"workload": f"O, N=1e5 (zo.haar seed 193), neighbourhood 20, {steps} BB steps",
+           "ms_per_step": dt * 1e3 / steps, "energy": stats["energy"], "separation_deg": stats["separation_deg"],
+           "fp64_subsample": {"sources": 10_000, "energy_sum_rel_err": e_err,
+                              "force_err_vs_pair_force_scale": f_err,
+                              "separation_max_abs_err_deg": float(np.abs(sep - sep64).max())}}
+    if with_cpu:
+        print("[bench] near relax: cpu steps", file=sys.stderr, flush=True)
+        cores = oracle.threads(0)
+        n, order = len(start), len(ops)
+        radius = oracle.mean_radius(n * order)
+        state = {"q": start.copy(), "bb": oracle.BB(n)}
+
+        def cpu_step():
+            rows, _ = oracle.near_grid(state["q"], ops, cut, r)
+            state["bb"].step(state["q"], rows, radius)
+            state["q"] = oracle.fold(state["q"], ops, 9)
+        step_s, _ = _median_of(cpu_step, cpu_repeats)
+        out["cpu_baseline"] = {"ms_per_step": step_s * 1e3, "cores": cores, "kind": "port",
+                               "sample": "the CPU restatement (orc_near_grid over all sources, orc_bb_step, "
+                                         "orc_fold) of single steps from the same start, median of 5 after a "
+                                         "warm-up step"}
+    return out
 
 
 def _same(a, b):

@@ -243,8 +243,7 @@ uint32_t Batch::step() {
       }
     }
     if (!ap.empty()) launch_allpairs(prof, stream, ap_set);
-    launch_bb_step(prof, stream, step_set);
-    launch_fold(prof, stream, step_set);
+    launch_bb_step(prof, stream, step_set);  // move and fold included
   };
   try {
     if (steady && graph_ && graph_key_ == key) {

@@ -783,9 +783,7 @@ void Engine::relax(uint32_t steps) {
     JobDesc ap;
     if (step_forces(ap)) launch_allpairs(prof, stream, single_job(ap));
     const JobSet one = single_job(step_job());
-    launch_bb_step(prof, stream, one);
-    // Right-action folding keeps the left tangent frame of the BB history (engine.zig:914-916).
-    launch_fold(prof, stream, one);
+    launch_bb_step(prof, stream, one);  // move and fold included
     stepped();
     recorded = true;
     if ((index + 1) % sync_every == 0 && index + 1 < count) sync();

@@ -53,18 +53,23 @@ struct NearInfo {
 };
 
 // BB step state, device-resident (relax.wgsl:17-22 keeps the same in 16 words).
+constexpr uint32_t kDigitBins = 2048;  // radix-select bins per middle rank (11-bit digits)
 struct BBState {
-  uint32_t iteration;
+  uint32_t iteration;  // steps begun
   uint32_t flags;      // 1 non-finite force, 2 duplicate orientation
   uint32_t step_bits;  // f32 bits of the largest move this step
-  uint32_t round;
+  uint32_t step;       // the running step's index (iteration before it began)
   double sy, yy, ss, ff, energy;
-  uint32_t rank[2];    // remaining rank of the two middle ranks
-  uint32_t prefix[2];  // selected high bits so far
+  uint32_t rank[2];    // remaining rank of the two middle ranks after the first two digits
+  uint32_t prefix[2];  // their first two digits
   float median;
   float alpha;
   double alpha64;
-  uint32_t hist[512];
+  // Radix-select histograms of the median key: the first digit (both middle ranks share
+  // it), then per rank the second and third digits.
+  uint32_t hist0[kDigitBins];
+  uint32_t hist1[2][kDigitBins];
+  uint32_t hist2[2][kDigitBins / 2];
 };
 
 // Scratch of an all-pairs pass: partial rows of `splits` (operator batch x site chunk).
@@ -204,7 +209,7 @@ struct NearBuffers {
   uint32_t* offsets = nullptr;   // sdim^3 + 1
   uint32_t* uslot = nullptr;     // capacity: each site's arrival slot in its cell
   uint32_t* order = nullptr;     // n, sources in the tile-by-tile visit order of their identity images
-  float4* opos = nullptr;        // n, their positions in that order (the identity images' sites; shares uq)
+  float4* opos = nullptr;        // n, their positions in that order, unordered builds only (shares uq)
   uint32_t* scan_tmp = nullptr;  // scan_tmp_words(max(cells, capacity))
   unsigned long long* counters = nullptr;  // [0] accepted pairs, [1] sites tested (force pass)
   uint32_t* counts = nullptr;    // 2n: per source, pairs and sites tested of the last force pass

@@ -315,14 +315,10 @@ __device__ inline uint32_t find_job(const uint32_t* __restrict__ start, uint32_t
 __device__ inline JobDesc load_job(const JobSet& s, uint32_t j) { return s.table ? s.table[j] : s.one; }
 __device__ inline uint32_t job_start(const uint32_t* __restrict__ start, uint32_t j) { return start ? start[j] : 0u; }
 
-__global__ void k_fold(JobSet s) {
-  const uint32_t j = find_job(s.point_start, s.count, blockIdx.x);
-  const JobDesc d = load_job(s, j);
-  const uint32_t i = (blockIdx.x - job_start(s.point_start, j)) * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
-  float4* __restrict__ q = d.q;
-  const float4* __restrict__ ops = d.ops;
-  const float4 a = q[i];
+// One point into the fundamental zone (state.wgsl:1-51): rescaled by exponent bits (no
+// division can overflow), normalised, scalar part made nonnegative, then, outside the zone,
+// the orbit point of largest scalar part.
+__device__ float4 fold_point(const float4 a, uint32_t group, const float4* __restrict__ ops, uint32_t order) {
   const float scale = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w)));
   const int maximum = int((__float_as_uint(scale) >> 23) & 255u);
   float c[4] = {a.x, a.y, a.z, a.w};
@@ -337,20 +333,25 @@ __global__ void k_fold(JobSet s) {
   }
   float4 p = normalize4(make_float4(c[0], c[1], c[2], c[3]));
   if (p.x < 0.0f) p = neg4(p);
-  if (in_zone(p, d.group, ops, d.order)) {
-    q[i] = p;
-    return;
-  }
+  if (in_zone(p, group, ops, order)) return p;
   float4 best = p;
   float w = p.x;
-  for (uint32_t k = 0; k < d.order; ++k) {
+  for (uint32_t k = 0; k < order; ++k) {
     const float4 cq = qmul(p, ops[k]);
     if (fabsf(cq.x) > w) {
       best = cq.x >= 0.0f ? cq : neg4(cq);
       w = fabsf(cq.x);
     }
   }
-  q[i] = normalize4(best);
+  return normalize4(best);
+}
+
+__global__ void k_fold(JobSet s) {
+  const uint32_t j = find_job(s.point_start, s.count, blockIdx.x);
+  const JobDesc d = load_job(s, j);
+  const uint32_t i = (blockIdx.x - job_start(s.point_start, j)) * blockDim.x + threadIdx.x;
+  if (i >= d.n) return;
+  d.q[i] = fold_point(d.q[i], d.group, d.ops, d.order);
 }
 
 JobDesc describe_job(float4* q, const float4* ops, uint32_t n, uint32_t order, uint32_t group) {
@@ -1011,24 +1012,25 @@ __global__ void k_site_count(NearBuffers b) {
   }
 }
 
-// Cells of up to kThreadCell sites are sorted by one thread, in place; larger ones through
-// an index list (see k_cell_sort_sites).
+// Cells of up to kThreadCell sites are put in rank order by counting (k_site_rank_place);
+// larger ones by sorting an index list (k_cell_sort_waves, k_cell_sort_block).
 constexpr uint32_t kThreadCell = 16;
 
-// With the counts scanned into offsets, each site moves to its cell's start plus its slot:
-// the site itself into a cell of at most `direct` sites, its index (perm) into a larger
-// one, whose sites the crowded-cell sorts move once ordered.
-__global__ void k_site_place(NearBuffers b, uint32_t direct) {
+// With the counts scanned into offsets, each site takes its cell's start plus its slot.
+// Unordered (spacing): the site itself goes there.  Ordered: a cell of at most kThreadCell
+// sites gets the site's rank there (k_site_rank_place then moves each site to its place
+// in rank order); a larger one the site's index (perm), for the crowded-cell sorts.
+__global__ void k_site_place(NearBuffers b, uint32_t ordered) {
   const uint32_t S = min(b.info->sites, b.capacity);
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
     const uint32_t c = b.ucell[s];
     const uint32_t begin = b.offsets[c], dst = begin + b.uslot[s];
-    if (b.offsets[c + 1] - begin > direct) {
-      b.perm[dst] = s;
-    } else {
+    if (!ordered) {
       b.sq[dst] = b.uq[s];
       b.scode[dst] = b.ucode[s];
       b.srank[dst] = b.urank[s];
+    } else {
+      b.perm[dst] = b.offsets[c + 1] - begin > kThreadCell ? s : b.urank[s];
     }
   }
 }
@@ -1041,31 +1043,23 @@ __device__ inline void move_site(const NearBuffers& b, uint32_t k, uint32_t s) {
 }
 
 // In-cell order by the site's unique rank, so sums run in the same order every pass
-// (near.wgsl:70-82).  One thread sorts a cell of up to kThreadCell sites in place (by
-// insertion); larger cells are listed for k_cell_sort_waves.
-__global__ void k_cell_sort_sites(NearBuffers b, uint32_t cells) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cells) return;
-  const uint32_t begin = b.offsets[c], end = b.offsets[c + 1];
-  if (end - begin < 2) return;
-  if (end - begin > kThreadCell) {
-    b.crowded[atomicAdd(&b.info->crowded, 1u)] = c;
-    return;
-  }
-  for (uint32_t at = begin + 1; at < end; ++at) {
-    const uint32_t kv = b.srank[at];
-    const float4 qv = b.sq[at];
-    const uint32_t cv = b.scode[at];
-    uint32_t before = at;
-    while (before > begin && b.srank[before - 1] > kv) {
-      b.sq[before] = b.sq[before - 1];
-      b.scode[before] = b.scode[before - 1];
-      b.srank[before] = b.srank[before - 1];
-      --before;
+// (near.wgsl:70-82).  In a cell of at most kThreadCell sites each site's place is the count
+// of its cell's ranks below its own (read from the ranks k_site_place laid out; the loads
+// are independent, no sorting chain); larger cells are listed for k_cell_sort_waves (by
+// the site that arrived first).
+__global__ void k_site_rank_place(NearBuffers b) {
+  const uint32_t S = min(b.info->sites, b.capacity);
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
+    const uint32_t c = b.ucell[s];
+    const uint32_t begin = b.offsets[c], m = b.offsets[c + 1] - begin;
+    if (m > kThreadCell) {
+      if (b.uslot[s] == 0) b.crowded[atomicAdd(&b.info->crowded, 1u)] = c;
+      continue;
     }
-    b.sq[before] = qv;
-    b.scode[before] = cv;
-    b.srank[before] = kv;
+    const uint32_t key = b.urank[s];
+    uint32_t below = 0;
+    for (uint32_t k = 0; k < m; ++k) below += b.perm[begin + k] < key ? 1u : 0u;
+    move_site(b, begin + below, s);
   }
 }
 
@@ -1249,8 +1243,9 @@ __global__ void k_core_count(NearBuffers b) {
   }
 }
 
-// With ucell scanned: each visited cell writes its sources from its offset on.
-__global__ void k_core_order(NearBuffers b) {
+// With ucell scanned: each visited cell writes its sources from its offset on (and, for
+// the spacing pass, their positions).
+__global__ void k_core_order(NearBuffers b, uint32_t positions) {
   const uint32_t visits = tile_visits(b.sdim);
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < visits; v += gridDim.x * blockDim.x) {
     uint32_t cell;
@@ -1260,7 +1255,8 @@ __global__ void k_core_order(NearBuffers b) {
       const uint32_t code = b.scode[k];
       if (code >> 31) {
         b.order[at] = code & 0x7fffffffu;
-        b.opos[at++] = b.sq[k];  // the identity image: the source's position, bit for bit
+        if (positions) b.opos[at] = b.sq[k];  // the identity image: the source's position, bit for bit
+        ++at;
       }
     }
   }
@@ -1289,11 +1285,9 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
   const uint32_t g = stride_grid(b.capacity);
   p.run("site_count", [&] { hipLaunchKernelGGL(k_site_count, dim3(g), dim3(kBlock), 0, s, b); });
   launch_scan(p, s, b.offsets, cells, b.scan_tmp);
-  p.run("site_place", [&] {
-    hipLaunchKernelGGL(k_site_place, dim3(g), dim3(kBlock), 0, s, b, ordered ? kThreadCell : 0xffffffffu);
-  });
+  p.run("site_place", [&] { hipLaunchKernelGGL(k_site_place, dim3(g), dim3(kBlock), 0, s, b, ordered ? 1u : 0u); });
   if (ordered) p.run("site_sort", [&] {
-    hipLaunchKernelGGL(k_cell_sort_sites, dim3(blocks_for(cells, kBlock)), dim3(kBlock), 0, s, b, uint32_t(cells));
+    hipLaunchKernelGGL(k_site_rank_place, dim3(g), dim3(kBlock), 0, s, b);
     hipLaunchKernelGGL(k_cell_sort_waves, dim3(kCrowdedBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank,
                        b.crowded, &b.info->crowded, b.crowded_capacity, b.huge, &b.info->huge, b);
     hipLaunchKernelGGL(k_cell_sort_block, dim3(kHugeBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank, b.huge,
@@ -1302,7 +1296,7 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
   const uint32_t visits = tile_visits(b.sdim), gv = stride_grid(visits);
   p.run("core_order", [&] { hipLaunchKernelGGL(k_core_count, dim3(gv), dim3(kBlock), 0, s, b); });
   launch_scan(p, s, b.ucell, visits, b.scan_tmp);
-  p.run("core_order", [&] { hipLaunchKernelGGL(k_core_order, dim3(gv), dim3(kBlock), 0, s, b); });
+  p.run("core_order", [&] { hipLaunchKernelGGL(k_core_order, dim3(gv), dim3(kBlock), 0, s, b, ordered ? 0u : 1u); });
   // The order only schedules the waves: each source's row is its own.
 }
 
@@ -2234,7 +2228,7 @@ __global__ void k_bb_reset(BBState* st) {
     st->iteration = 0;
     st->flags = 0;
     st->step_bits = 0;
-    st->round = 0;
+    st->step = 0;
     st->sy = st->yy = st->ss = st->ff = st->energy = 0.0;
     st->rank[0] = st->rank[1] = 0;
     st->prefix[0] = st->prefix[1] = 0;
@@ -2242,7 +2236,11 @@ __global__ void k_bb_reset(BBState* st) {
     st->alpha = 0.0f;
     st->alpha64 = 0.0;
   }
-  for (uint32_t k = t; k < 512; k += blockDim.x) st->hist[k] = 0;
+  for (uint32_t k = t; k < kDigitBins; k += blockDim.x) {
+    st->hist0[k] = 0;
+    st->hist1[0][k] = st->hist1[1][k] = 0;
+  }
+  for (uint32_t k = t; k < kDigitBins / 2; k += blockDim.x) st->hist2[0][k] = st->hist2[1][k] = 0;
 }
 void launch_bb_reset(Prof& p, hipStream_t s, BBState* state) {
   p.run("bb_reset", [&] { hipLaunchKernelGGL(k_bb_reset, dim3(1), dim3(kBlock), 0, s, state); });
@@ -2264,12 +2262,81 @@ __device__ inline void block_sum5(double v[5], double* out) {
   }
 }
 
+// The exact median of max(curvature, 0) (relax.wgsl:79-113) by radix selection over both
+// middle ranks, in three digits of 11, 11 and 10 bits of the key's FP32 bits (nonnegative
+// floats order as their bits).  A digit's histogram is complete only when the launch that
+// counts it has ended, so each later launch's workgroups pick the digits they need from
+// the finished histograms themselves (the same arithmetic in every workgroup): no
+// workgroup waits for another within a launch, and no device-wide fence is needed.
+__device__ inline uint32_t curvature_bits(float w) { return __float_as_uint(w > 0.0f ? w : 0.0f); }
+
+// One workgroup finds, in a histogram of `bins` counts, the bin holding rank `want`: each
+// thread sums bins / 256 of them, a block scan finds the thread whose run holds the rank,
+// and that thread the bin.  Returns (bin, rank within the bin) to every thread.
+__device__ uint2 select_bin(const uint32_t* __restrict__ hist, uint32_t bins, uint32_t want) {
+  __shared__ uint32_t wave_total[kBlock / 64];
+  __shared__ uint2 found;
+  const uint32_t per = bins / kBlock;  // 8 or 4
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t counts[8], mine = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    counts[k] = k < per ? hist[threadIdx.x * per + k] : 0u;
+    mine += counts[k];
+  }
+  const uint32_t incl = wave_incl_scan(mine);
+  if (lane == 63) wave_total[wave] = incl;
+  if (threadIdx.x == 0) found = make_uint2(bins - 1, 0u);  // not kept when the counts hold the rank
+  __syncthreads();
+  uint32_t before = incl - mine;
+  for (uint32_t w = 0; w < wave; ++w) before += wave_total[w];
+  if (want >= before && want < before + mine) {
+    uint32_t left = want - before, bin = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+      if (bin == k && k + 1 < per && left >= counts[k]) {
+        left -= counts[k];
+        bin = k + 1;
+      }
+    found = make_uint2(threadIdx.x * per + bin, left);
+  }
+  __syncthreads();
+  const uint2 out = found;
+  __syncthreads();
+  return out;
+}
+
+// The first digit of both middle ranks (from hist0), with their remaining ranks.
+__device__ inline void first_digits(const BBState* st, uint32_t n, uint32_t prefix[2], uint32_t rank[2]) {
+  const uint32_t want[2] = {(n - 1) / 2, n / 2};
+  for (uint32_t r = 0; r < 2; ++r) {
+    const uint2 b = select_bin(st->hist0, kDigitBins, want[r]);
+    prefix[r] = b.x << 21;
+    rank[r] = b.y;
+  }
+}
+// The second digit of both middle ranks (from hist1), after the first.
+__device__ inline void second_digits(const BBState* st, uint32_t prefix[2], uint32_t rank[2]) {
+  for (uint32_t r = 0; r < 2; ++r) {
+    const uint2 b = select_bin(st->hist1[r], kDigitBins, rank[r]);
+    prefix[r] |= b.x << 10;
+    rank[r] = b.y;
+  }
+}
+
+// BB stage 1, one thread per point: the step sums' partials (s.y, y.y, s.s, f.f, E) per
+// workgroup, the flags and the first digit's histogram of the median key.
 __global__ __launch_bounds__(256) void k_bb_stats(JobSet s) {
+  __shared__ uint32_t local[kDigitBins];
   const uint32_t jb = find_job(s.point_start, s.count, blockIdx.x);
   const JobDesc d = load_job(s, jb);
   const uint32_t block = blockIdx.x - job_start(s.point_start, jb);
   const uint32_t i = block * kBlock + threadIdx.x;
   BBState* __restrict__ st = d.state;
+  for (uint32_t k = threadIdx.x; k < kDigitBins; k += kBlock) local[k] = 0;
+  if (block == 0)  // the third digit's histogram, used last step, is counted again in stage 3
+    for (uint32_t k = threadIdx.x; k < kDigitBins / 2; k += kBlock) st->hist2[0][k] = st->hist2[1][k] = 0;
+  __syncthreads();
   const bool first = st->iteration == 0;
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   uint32_t fl = 0;
@@ -2290,122 +2357,125 @@ __global__ __launch_bounds__(256) void k_bb_stats(JobSet s) {
       v[1] = yx * yx + yy_ * yy_ + yz * yz;
       v[2] = sx * sx + sy_ * sy_ + sz * sz;
     }
+    atomicAdd(&local[curvature_bits(f.w) >> 21], 1u);
   }
   if (fl) atomicOr(&st->flags, fl);
   block_sum5(v, d.bb_partials + 5 * size_t(block));
-}
-
-// One workgroup per job: partial sums in a fixed order, then the median's rank bookkeeping.
-__global__ __launch_bounds__(256) void k_bb_sums(JobSet s) {
-  const JobDesc d = load_job(s, blockIdx.x);
-  BBState* __restrict__ st = d.state;
-  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (uint32_t b = threadIdx.x; b < d.point_blocks; b += kBlock)
-    for (int c = 0; c < 5; ++c) v[c] += d.bb_partials[5 * size_t(b) + c];
-  __shared__ double out[5];
-  block_sum5(v, out);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    st->sy = out[0];
-    st->yy = out[1];
-    st->ss = out[2];
-    st->ff = out[3];
-    st->energy = out[4];
-    st->rank[0] = (d.n - 1) / 2;
-    st->rank[1] = d.n / 2;
-    st->prefix[0] = st->prefix[1] = 0;
-    st->step_bits = 0;
-  }
-  for (uint32_t k = threadIdx.x; k < 512; k += kBlock) st->hist[k] = 0;
+  for (uint32_t k = threadIdx.x; k < kDigitBins; k += kBlock)
+    if (local[k]) atomicAdd(&st->hist0[k], local[k]);
 }
 
-__device__ inline uint32_t curvature_bits(const float4 f) { return __float_as_uint(f.w > 0.0f ? f.w : 0.0f); }
-
-// Exact median of max(curvature, 0) by four 8-bit radix rounds over both middle ranks.
-__global__ __launch_bounds__(256) void k_median_count(JobSet s, uint32_t round) {
-  __shared__ uint32_t local[512];
+// BB stages 2 and 3 (median_blocks workgroups per job, reading the curvature word of each
+// force row): the keys matching each rank's digits so far are counted by their next digit.
+// Stage 2's first workgroup also adds the partials in a fixed order (the step sums) and
+// opens the step; stage 3's first workgroup records both ranks' first two digits.
+__global__ __launch_bounds__(256) void k_median_round(JobSet s, uint32_t round) {
+  __shared__ uint32_t local[2 * kDigitBins];
   const uint32_t jb = find_job(s.median_start, s.count, blockIdx.x);
   const JobDesc d = load_job(s, jb);
   const uint32_t block = blockIdx.x - job_start(s.median_start, jb);
-  local[threadIdx.x] = 0;
-  local[threadIdx.x + 256] = 0;
-  __syncthreads();
   BBState* __restrict__ st = d.state;
-  const uint32_t shift = 24 - 8 * round;
-  const uint32_t high = round > 0 ? (0xffffffffu << (32 - 8 * round)) : 0u;
-  const uint32_t p0 = st->prefix[0], p1 = st->prefix[1];
+  uint32_t prefix[2], rank[2];
+  first_digits(st, d.n, prefix, rank);
+  if (round == 2) second_digits(st, prefix, rank);
+  const uint32_t bins = round == 1 ? kDigitBins : kDigitBins / 2, shift = round == 1 ? 10u : 0u;
+  const uint32_t fixed = round == 1 ? 0xffe00000u : 0xfffffc00u;  // the key bits chosen so far
+  for (uint32_t k = threadIdx.x; k < 2 * bins; k += kBlock) local[k] = 0;
+  __syncthreads();
+  const float* __restrict__ curvature = reinterpret_cast<const float*>(d.rows.force) + 3;
   for (uint32_t i = block * kBlock + threadIdx.x; i < d.n; i += d.median_blocks * kBlock) {
-    const uint32_t bits = curvature_bits(d.rows.force[i]);
-    const uint32_t digit = (bits >> shift) & 255u;
-    if ((bits & high) == p0) atomicAdd(&local[digit], 1u);
-    if ((bits & high) == p1) atomicAdd(&local[256 + digit], 1u);
+    const uint32_t bits = curvature_bits(curvature[4 * size_t(i)]);
+    const uint32_t digit = (bits >> shift) & (bins - 1);
+    if ((bits & fixed) == prefix[0]) atomicAdd(&local[digit], 1u);
+    if ((bits & fixed) == prefix[1]) atomicAdd(&local[bins + digit], 1u);
   }
   __syncthreads();
-  for (uint32_t k = threadIdx.x; k < 512; k += kBlock)
-    if (local[k]) atomicAdd(&st->hist[k], local[k]);
-}
-
-__global__ __launch_bounds__(256) void k_median_pick(JobSet s, uint32_t round) {
-  const JobDesc d = load_job(s, blockIdx.x);
-  BBState* __restrict__ st = d.state;
-  if (threadIdx.x == 0) {
-    const uint32_t shift = 24 - 8 * round;
-    for (int r = 0; r < 2; ++r) {
-      uint32_t remaining = st->rank[r];
-      uint32_t digit = 255;
-      for (uint32_t dg = 0; dg < 256; ++dg) {
-        const uint32_t c = st->hist[256 * r + dg];
-        if (remaining < c) {
-          digit = dg;
-          break;
-        }
-        remaining -= c;
-      }
-      st->rank[r] = remaining;
-      st->prefix[r] |= digit << shift;
-    }
-    if (round == 3) {
-      const float median =
-          fmaxf(1e-12f, 0.5f * (__uint_as_float(st->prefix[0]) + __uint_as_float(st->prefix[1])));
-      st->median = median;
-      const uint32_t it = st->iteration;
-      const double radius = d.radius;
-      const double alpha0 = (0.8 / 3.4) * radius * radius * radius;
-      double alpha = alpha0;
-      if (it > 0) {
-        const double a1 = fabs(st->yy) > 1e-20 ? fabs(st->sy / st->yy) : alpha0;
-        const double a2 = fabs(st->sy) > 1e-20 ? fabs(st->ss / st->sy) : alpha0;
-        alpha = fmin(fmax(it % 2 == 1 ? a1 : a2, 1e-12), fmax(1.0, 10.0 * radius));
-      }
-      st->alpha64 = alpha;
-      st->alpha = float(alpha);
-      st->iteration = it + 1;
-    }
+  uint32_t* out0 = round == 1 ? st->hist1[0] : st->hist2[0];
+  uint32_t* out1 = round == 1 ? st->hist1[1] : st->hist2[1];
+  for (uint32_t k = threadIdx.x; k < bins; k += kBlock) {
+    if (local[k]) atomicAdd(&out0[k], local[k]);
+    if (local[bins + k]) atomicAdd(&out1[k], local[bins + k]);
   }
-  __syncthreads();
-  st->hist[threadIdx.x] = 0;
-  st->hist[threadIdx.x + 256] = 0;
+  if (block != 0) return;
+  if (round == 1) {
+    double w[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = threadIdx.x; b < d.point_blocks; b += kBlock)
+      for (int c = 0; c < 5; ++c) w[c] += d.bb_partials[5 * size_t(b) + c];
+    __shared__ double sums[5];
+    block_sum5(w, sums);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      st->sy = sums[0];
+      st->yy = sums[1];
+      st->ss = sums[2];
+      st->ff = sums[3];
+      st->energy = sums[4];
+      st->step_bits = 0;
+      st->step = st->iteration;
+      st->iteration = st->step + 1;
+    }
+  } else if (threadIdx.x == 0) {
+    st->prefix[0] = prefix[0];
+    st->prefix[1] = prefix[1];
+    st->rank[0] = rank[0];
+    st->rank[1] = rank[1];
+  }
 }
 
+// BB stage 4, one thread per point: every workgroup takes the third digit from the finished
+// histogram, the median = max(1e-12, mean of the two middle keys) and the step length
+// alpha (relax.wgsl:115-128: alpha0 = (0.8/3.4) r^3 first, then BB1 on odd and BB2 on even
+// steps, clamped to [1e-12, max(1, 10 r)]); then the move (relax.wgsl:140-165): scale
+// alpha / max(1e-6, c / median + 1e-2), clipped to half the mean radius, q <- exp(d) q
+// (sinc series below 1e-4), sign fixed, the held identity exact; then the fold into the
+// zone.  A flagged step moves nothing (the points are folded as they are).  The first
+// workgroup clears the first two digits' histograms for the next step.
 __global__ __launch_bounds__(256) void k_bb_move(JobSet s) {
   const uint32_t jb = find_job(s.point_start, s.count, blockIdx.x);
   const JobDesc d = load_job(s, jb);
-  const uint32_t i = (blockIdx.x - job_start(s.point_start, jb)) * kBlock + threadIdx.x;
+  const uint32_t block = blockIdx.x - job_start(s.point_start, jb);
+  const uint32_t i = block * kBlock + threadIdx.x;
   BBState* __restrict__ st = d.state;
-  if (st->flags != 0) return;
+  uint32_t key[2];
+  for (uint32_t r = 0; r < 2; ++r) key[r] = st->prefix[r] | select_bin(st->hist2[r], kDigitBins / 2, st->rank[r]).x;
+  const float median = fmaxf(1e-12f, 0.5f * (__uint_as_float(key[0]) + __uint_as_float(key[1])));
+  const uint32_t it = st->step;
+  const double radius = d.radius;
+  const double alpha0 = (0.8 / 3.4) * radius * radius * radius;
+  double alpha64 = alpha0;
+  if (it > 0) {
+    const double a1 = fabs(st->yy) > 1e-20 ? fabs(st->sy / st->yy) : alpha0;
+    const double a2 = fabs(st->sy) > 1e-20 ? fabs(st->ss / st->sy) : alpha0;
+    alpha64 = fmin(fmax(it % 2 == 1 ? a1 : a2, 1e-12), fmax(1.0, 10.0 * radius));
+  }
+  const float alpha = float(alpha64);
+  if (block == 0) {
+    if (threadIdx.x == 0) {
+      st->median = median;
+      st->alpha = alpha;
+      st->alpha64 = alpha64;
+    }
+    for (uint32_t k = threadIdx.x; k < kDigitBins; k += kBlock) {
+      st->hist0[k] = 0;
+      st->hist1[0][k] = st->hist1[1][k] = 0;
+    }
+  }
   float moved_size = 0.0f;
   if (i < d.n) {
-    if (i + 1 == d.fixed) {
+    float4 m = d.q[i];
+    if (st->flags != 0) {
+      // no move
+    } else if (i + 1 == d.fixed) {
       d.prev[2 * size_t(i)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       d.prev[2 * size_t(i) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      d.q[i] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+      m = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
     } else {
-      const float radius = float(d.radius);
-      const float alpha = st->alpha, median = st->median;
+      const float r = float(d.radius);
       const float4 f = d.rows.force[i];
       const float scale = alpha * d.step_scale / fmaxf(1e-6f, f.w / median + 1e-2f);
       const float size = sqrtf(f.x * f.x + f.y * f.y + f.z * f.z) * scale;
-      const float clip = fminf(1.0f, 0.5f * radius / fmaxf(size, 1e-12f));
+      const float clip = fminf(1.0f, 0.5f * r / fmaxf(size, 1e-12f));
       const float3 dv = make_float3(f.x * (scale * clip), f.y * (scale * clip), f.z * (scale * clip));
       d.prev[2 * size_t(i)] = make_float4(f.x, f.y, f.z, 0.0f);
       d.prev[2 * size_t(i) + 1] = make_float4(dv.x, dv.y, dv.z, 0.0f);
@@ -2413,26 +2483,25 @@ __global__ __launch_bounds__(256) void k_bb_move(JobSet s) {
       const float angle = sqrtf(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
       const float sinc = angle > 1e-4f ? sinf(angle) / angle : 1.0f - angle * angle / 6.0f;
       const float4 rot = make_float4(cosf(angle), dv.x * sinc, dv.y * sinc, dv.z * sinc);
-      float4 m = normalize4(qmul(rot, d.q[i]));
+      m = normalize4(qmul(rot, m));
       if (m.x < 0.0f) m = neg4(m);
-      d.q[i] = m;
     }
+    // Right-action folding keeps the left tangent frame of the BB history (engine.zig:914-916).
+    d.q[i] = fold_point(m, d.group, d.ops, d.order);
   }
   const uint32_t mx = wave_max_u32(__float_as_uint(moved_size));
   if ((threadIdx.x & 63) == 0 && mx) atomicMax(&st->step_bits, mx);
 }
 
+// One BB step of every job in the set, resident, in four launches: stats with the first
+// median digit, two radix rounds (with the step sums), move and fold.
 void launch_bb_step(Prof& p, hipStream_t s, const JobSet& jobs) {
   if (!jobs.count) return;
   p.run("bb_stats", [&] { hipLaunchKernelGGL(k_bb_stats, dim3(jobs.point_total), dim3(kBlock), 0, s, jobs); });
-  p.run("bb_sums", [&] { hipLaunchKernelGGL(k_bb_sums, dim3(jobs.count), dim3(kBlock), 0, s, jobs); });
-  for (uint32_t round = 0; round < 4; ++round) {
-    p.run("median_count", [&] {
-      hipLaunchKernelGGL(k_median_count, dim3(jobs.median_total), dim3(kBlock), 0, s, jobs, round);
+  for (uint32_t round = 1; round <= 2; ++round)
+    p.run("median_round", [&] {
+      hipLaunchKernelGGL(k_median_round, dim3(jobs.median_total), dim3(kBlock), 0, s, jobs, round);
     });
-    p.run("median_pick",
-          [&] { hipLaunchKernelGGL(k_median_pick, dim3(jobs.count), dim3(kBlock), 0, s, jobs, round); });
-  }
   p.run("bb_move", [&] { hipLaunchKernelGGL(k_bb_move, dim3(jobs.point_total), dim3(kBlock), 0, s, jobs); });
 }
 
