@@ -284,6 +284,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--serving-requests", type=int, default=40, help="0 skips the serving run")
     ap.add_argument("--serving-rate", type=float, default=4.0)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="batched and serving runs launch every step eagerly (for kernel-trace profiles)")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the small all-pairs relax (O, 1e3, 30 steps) and near relax (O, 1e5, 100 steps)")
     ap.add_argument("--no-widened", action="store_true",
@@ -390,9 +392,9 @@ def main():
         out["small_relax"] = small_relax_section(with_cpu=not args.no_cpu)
         out["near_relax"] = near_relax_section(with_cpu=not args.no_cpu)
     if args.batched_jobs > 0:
-        out["batched"] = batched_section(args.batched_jobs, args.batched_steps)
+        out["batched"] = batched_section(args.batched_jobs, args.batched_steps, graphs=not args.no_graphs)
     if args.serving_requests > 0:
-        out["serving"] = serving_section(args.serving_requests, args.serving_rate)
+        out["serving"] = serving_section(args.serving_requests, args.serving_rate, graphs=not args.no_graphs)
     if not args.no_widened:
         out["widened"] = widened_section()
     print(json.dumps(out), flush=True)
@@ -603,14 +605,14 @@ def _same(a, b):
     return all(x.state == y.state == "done" and x.stats == y.stats for x, y in zip(a, b))
 
 
-def batched_section(total, steps):
+def batched_section(total, steps, graphs=True):
     """Mixed-size relax jobs (12 groups with C1, N log-uniform in [500, 2e5], all pairs below
     5,000 points, neighbourhood 20 above, Haar seed = job index) run in one batch with graph
     replay, then one after another on their own Grids (the plain per-sample path)."""
     from zxtalori_amd import serve
     jobs = serve.mixed_relax_jobs(total, seed=193, steps=steps)
     print(f"[bench] batched {total} jobs x {steps} steps", file=sys.stderr, flush=True)
-    batched, b = serve.run_batched(jobs)
+    batched, b = serve.run_batched(jobs, graphs=graphs)
     print(f"[bench] batched {b['elapsed_s']:.2f} s; plain loop", file=sys.stderr, flush=True)
     plain, p = serve.serve_plain(jobs, realtime=False)
     points = sum(j.count for j in jobs)
@@ -624,20 +626,20 @@ def batched_section(total, steps):
             "failed": b["failed"] + p["failed"]}
 
 
-def serving_section(total, rate):
+def serving_section(total, rate, graphs=True):
     """Poisson request stream (relax 20-100 steps, measure, spacing; N log-uniform in
     [1e3, 1e6]; 12 groups) served by the continuous-batching loop and by the plain path, at
     the arrival rate (latency) and with every request queued at the start (throughput)."""
     from zxtalori_amd import serve
     reqs = serve.poisson_requests(total, rate, seed=5)
     print(f"[bench] serving {total} requests at {rate}/s", file=sys.stderr, flush=True)
-    with serve.Server(hbm_budget=64 << 30) as server:
+    with serve.Server(hbm_budget=64 << 30, graphs=graphs) as server:
         served, s = server.run(reqs, realtime=True)
         max_live = server.max_live
     print(f"[bench] served p50 {s['p50_ms']:.1f} ms p99 {s['p99_ms']:.1f} ms; plain", file=sys.stderr, flush=True)
     plain, p = serve.serve_plain(reqs, realtime=True)
     print("[bench] saturated throughput", file=sys.stderr, flush=True)
-    with serve.Server(hbm_budget=64 << 30) as server:
+    with serve.Server(hbm_budget=64 << 30, graphs=graphs) as server:
         _, s_sat = server.run(reqs, realtime=False)
     _, p_sat = serve.serve_plain(reqs, realtime=False)
     return {"workload": f"{total} requests, Poisson rate {rate}/s (seed 5): 70% relax (20-100 steps), 15% "

@@ -29,9 +29,13 @@ Batch::Batch(bool use_graphs, uint32_t streams) : use_graphs_(use_graphs) {
 
 Batch::~Batch() {
   (void)hipStreamSynchronize(stream);
-  if (graph_) (void)hipGraphExecDestroy(graph_);
   jobs_.clear();
   CaptureLock lock(false);
+  if (graph_) (void)hipGraphExecDestroy(graph_);
+  for (const Retired& r : retired_) {
+    (void)hipGraphExecDestroy(r.exec);
+    (void)hipEventDestroy(r.done);
+  }
   for (Table* t : {&ap_table_, &step_table_}) {
     if (t->host_desc) (void)hipHostFree(t->host_desc);
     if (t->host_start) (void)hipHostFree(t->host_start);
@@ -81,19 +85,19 @@ static void check_desc(const JobDesc& d, bool allpairs) {
 // Uploads a job table (only when it changed) and returns its JobSet.
 JobSet Batch::upload(Table& t, const std::vector<JobDesc>& descs) {
   const size_t count = descs.size();
-  for (const JobDesc& d : descs) check_desc(d, &t == &ap_table_);
-  bool same = t.last.size() == count;
-  for (size_t k = 0; same && k < count; ++k) same = std::memcmp(&t.last[k], &descs[k], sizeof(JobDesc)) == 0;
-  if (same && count) return t.set;
-  // A captured step must replay with the tables it was captured with.
-  if (capturing_) throw GraphTableChanged{};
   JobSet s;
-  s.count = uint32_t(count);
-  if (count == 0) {
+  if (count == 0) {  // nothing to launch, nothing to upload (also under capture)
     t.last.clear();
     t.set = s;
     return s;
   }
+  for (const JobDesc& d : descs) check_desc(d, &t == &ap_table_);
+  bool same = t.last.size() == count;
+  for (size_t k = 0; same && k < count; ++k) same = std::memcmp(&t.last[k], &descs[k], sizeof(JobDesc)) == 0;
+  if (same) return t.set;
+  // A captured step must replay with the tables it was captured with.
+  if (capturing_) throw GraphTableChanged{};
+  s.count = uint32_t(count);
   if (count > t.capacity) {
     Engine::check(hipStreamSynchronize(stream), "wait");
     CaptureLock lock(false);
@@ -148,6 +152,7 @@ void Batch::finish(Job& j, bool recorded) {
 }
 
 uint32_t Batch::step() {
+  reap_graphs();
   uint32_t completed = 0;
   // Measure and spacing jobs run as they come (each ends with a wait).
   for (auto& jp : jobs_) {
@@ -220,6 +225,14 @@ uint32_t Batch::step() {
       Engine::check(hipEventRecord(fork_, stream), "fork");
       for (hipStream_t s : aux_) Engine::check(hipStreamWaitEvent(s, fork_, 0), "fork");
     }
+    // Every extra stream joins the batch stream again, also when a pass throws: under
+    // capture, a stream left forked would keep the capture from ending.
+    auto join = [&]() {
+      for (size_t k = 0; k < aux_.size(); ++k) {
+        Engine::check(hipEventRecord(join_[k], aux_[k]), "join");
+        Engine::check(hipStreamWaitEvent(stream, join_[k], 0), "join");
+      }
+    };
     size_t next = 0;
     for (Engine* e : near) {
       JobDesc unused;
@@ -232,16 +245,12 @@ uint32_t Batch::step() {
         e->step_forces(unused);
       } catch (...) {
         e->use_stream(own);
+        join();
         throw;
       }
       e->use_stream(own);
     }
-    if (split) {
-      for (size_t k = 0; k < aux_.size(); ++k) {
-        Engine::check(hipEventRecord(join_[k], aux_[k]), "join");
-        Engine::check(hipStreamWaitEvent(stream, join_[k], 0), "join");
-      }
-    }
+    if (split) join();
     if (!ap.empty()) launch_allpairs(prof, stream, ap_set);
     launch_bb_step(prof, stream, step_set);  // move and fold included
   };
@@ -250,10 +259,7 @@ uint32_t Batch::step() {
       Engine::check(hipGraphLaunch(graph_, stream), "graph");
       ++replays_;
     } else if (steady) {
-      if (graph_) {
-        (void)hipGraphExecDestroy(graph_);
-        graph_ = nullptr;
-      }
+      retire_graph();
       graph_key_.clear();
       hipGraph_t g = nullptr;
       bool changed = false;
@@ -282,8 +288,14 @@ uint32_t Batch::step() {
         record();
       } else {
         Engine::check(ended, "capture");
-        const hipError_t made = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
+        hipError_t made;
+        {
+          // Instantiation allocates the graph's argument memory and the streams of its
+          // parallel branches: like any allocation, not while another thread captures.
+          CaptureLock lock(false);
+          made = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+        }
         Engine::check(made, "graph");
         graph_key_ = key;
         Engine::check(hipGraphLaunch(graph_, stream), "graph");
@@ -329,8 +341,7 @@ void Batch::remove(uint64_t id) {
   fail("UnknownJob");
 }
 
-// Steps in flight: an event recorded after each step's launches.
-void Batch::mark_step() {
+hipEvent_t Batch::take_event() {
   hipEvent_t ev = nullptr;
   if (!spare_.empty()) {
     ev = spare_.back();
@@ -338,11 +349,42 @@ void Batch::mark_step() {
   } else {
     Engine::check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "event");
   }
+  return ev;
+}
+
+// Steps in flight: an event recorded after each step's launches.
+void Batch::mark_step() {
+  hipEvent_t ev = take_event();
   Engine::check(hipEventRecord(ev, stream), "event");
   inflight_.push_back(ev);
 }
 
+// Called outside capture only: the event marks the end of everything the stream holds,
+// the graph's last launch included.
+void Batch::retire_graph() {
+  if (!graph_) return;
+  hipEvent_t ev = take_event();
+  Engine::check(hipEventRecord(ev, stream), "event");
+  retired_.push_back({graph_, ev});
+  graph_ = nullptr;
+}
+
+void Batch::reap_graphs() {
+  size_t kept = 0;
+  for (const Retired& r : retired_) {
+    if (hipEventQuery(r.done) == hipSuccess) {
+      CaptureLock lock(false);  // frees what instantiation allocated
+      (void)hipGraphExecDestroy(r.exec);
+      spare_.push_back(r.done);
+    } else {
+      retired_[kept++] = r;
+    }
+  }
+  retired_.resize(kept);
+}
+
 uint32_t Batch::in_flight() {
+  reap_graphs();
   while (!inflight_.empty() && hipEventQuery(inflight_.front()) == hipSuccess) {
     spare_.push_back(inflight_.front());
     inflight_.pop_front();

@@ -77,6 +77,16 @@ class Batch {
   };
   Job* find(uint64_t id) const;
   void mark_step();
+  hipEvent_t take_event();
+  // A replaced graph is destroyed only after the stream has passed its last launch: HIP
+  // does not promise to keep an executable graph alive while a launch of it still runs.
+  void retire_graph();
+  void reap_graphs();
+  struct Retired {
+    hipGraphExec_t exec;
+    hipEvent_t done;
+  };
+  std::vector<Retired> retired_;
   std::deque<hipEvent_t> inflight_;
   std::vector<hipEvent_t> spare_;
   JobSet upload(Table& t, const std::vector<JobDesc>& descs);
