@@ -40,6 +40,8 @@ Batch::~Batch() {
     if (t->host_desc) (void)hipHostFree(t->host_desc);
     if (t->host_start) (void)hipHostFree(t->host_start);
   }
+  if (near_table_.host_desc) (void)hipHostFree(near_table_.host_desc);
+  if (near_table_.host_start) (void)hipHostFree(near_table_.host_start);
   for (hipEvent_t e : inflight_) (void)hipEventDestroy(e);
   for (hipEvent_t e : spare_) (void)hipEventDestroy(e);
   for (hipStream_t s : aux_) (void)hipStreamDestroy(s);
@@ -141,6 +143,65 @@ JobSet Batch::upload(Table& t, const std::vector<JobDesc>& descs) {
   return s;
 }
 
+// The near-field job table (sorted by force variant) and its stage prefix sums, uploaded
+// only when it changed, as upload does for the other tables.
+NearSet Batch::upload_near(const std::vector<NearJob>& jobs) {
+  NearTable& t = near_table_;
+  const size_t count = jobs.size();
+  if (count == 0) {
+    t.last.clear();
+    t.set = NearSet{};
+    return t.set;
+  }
+  bool same = t.last.size() == count;
+  for (size_t k = 0; same && k < count; ++k) same = std::memcmp(&t.last[k], &jobs[k], sizeof(NearJob)) == 0;
+  if (same) return t.set;
+  if (capturing_) throw GraphTableChanged{};
+  const size_t words = size_t(kNearStages) * (count + 1);
+  Engine::check(hipStreamSynchronize(stream), "wait");  // the last upload may still be in flight
+  if (count > t.capacity) {
+    CaptureLock lock(false);
+    if (t.host_desc) (void)hipHostFree(t.host_desc);
+    if (t.host_start) (void)hipHostFree(t.host_start);
+    t.host_desc = nullptr;
+    t.host_start = nullptr;
+    t.capacity = std::max<size_t>(count, 2 * t.capacity);
+    const size_t cap_words = size_t(kNearStages) * (t.capacity + 1);
+    Engine::check(hipHostMalloc(reinterpret_cast<void**>(&t.host_desc), t.capacity * sizeof(NearJob)), "pinned");
+    Engine::check(hipHostMalloc(reinterpret_cast<void**>(&t.host_start), cap_words * sizeof(uint32_t)), "pinned");
+    t.desc.ensure(t.capacity * sizeof(NearJob));
+    t.start.ensure(cap_words * sizeof(uint32_t));
+  }
+  NearSet s;
+  s.count = uint32_t(count);
+  for (uint32_t st = 0; st < kNearStages; ++st) {
+    uint32_t* row = t.host_start + size_t(st) * (count + 1);
+    row[0] = 0;
+    for (size_t k = 0; k < count; ++k) row[k + 1] = row[k] + near_stage_blocks(jobs[k], st);
+    s.total[st] = row[count];
+  }
+  const uint32_t* force = t.host_start + size_t(kNsForce) * (count + 1);
+  for (size_t k = 0; k < count; ++k) {
+    if (s.keys == 0 || s.key[s.keys - 1] != jobs[k].key) {
+      if (s.keys == kNearKeys) fail("TooManyNearVariants");
+      s.key[s.keys] = jobs[k].key;
+      s.key_first[s.keys] = force[k];
+      ++s.keys;
+    }
+    s.key_blocks[s.keys - 1] = force[k + 1] - s.key_first[s.keys - 1];
+  }
+  std::memcpy(t.host_desc, jobs.data(), count * sizeof(NearJob));
+  Engine::check(hipMemcpyAsync(t.desc.p, t.host_desc, count * sizeof(NearJob), hipMemcpyHostToDevice, stream),
+                "table");
+  Engine::check(hipMemcpyAsync(t.start.p, t.host_start, words * sizeof(uint32_t), hipMemcpyHostToDevice, stream),
+                "table");
+  s.table = t.desc.as<NearJob>();
+  s.start = t.start.as<uint32_t>();
+  t.last = jobs;
+  t.set = s;
+  return s;
+}
+
 void Batch::finish(Job& j, bool recorded) {
   try {
     j.engine->finish_relax(recorded);
@@ -202,56 +263,40 @@ uint32_t Batch::step() {
   const bool steady = use_graphs_ && !prof.on() && !fresh && key == last_key_;
   last_key_ = key;
   auto record = [&]() {
-    // Tables first (a captured step cannot upload), then the near-field passes, forked
-    // onto the extra streams after the last step's folds and joined back before this
-    // step's all-pairs pass, BB step and fold.
+    // Tables first (a captured step cannot upload), then the near-field launches, forked
+    // onto an extra stream after the last step's moves, beside the all-pairs launch, and
+    // joined back before this step's BB step.
     std::vector<JobDesc> ap, steps;
-    std::vector<Engine*> near;
+    std::vector<NearJob> near;
     for (Job* j : active) {
       Engine& e = *j->engine;
       e.prepare_step();
       if (e.relax_state_.has_near) {
-        near.push_back(&e);
+        NearJob nj;
+        if (e.near_job(nj)) near.push_back(nj);
       } else {
         JobDesc d;
         if (e.step_forces(d)) ap.push_back(d);
       }
     }
     for (Job* j : active) steps.push_back(j->engine->step_job());
+    // Jobs of one force variant share a launch.
+    std::stable_sort(near.begin(), near.end(), [](const NearJob& a, const NearJob& b) { return a.key < b.key; });
     const JobSet ap_set = upload(ap_table_, ap);
     const JobSet step_set = upload(step_table_, steps);
-    const bool split = !aux_.empty() && !near.empty();
+    const NearSet near_set = upload_near(near);
+    // (Profiled steps stay on one stream: the profiler's events are on the batch stream.)
+    const bool split = !aux_.empty() && !near.empty() && !ap.empty() && !prof.on();
     if (split) {
       Engine::check(hipEventRecord(fork_, stream), "fork");
-      for (hipStream_t s : aux_) Engine::check(hipStreamWaitEvent(s, fork_, 0), "fork");
+      Engine::check(hipStreamWaitEvent(aux_[0], fork_, 0), "fork");
     }
-    // Every extra stream joins the batch stream again, also when a pass throws: under
-    // capture, a stream left forked would keep the capture from ending.
-    auto join = [&]() {
-      for (size_t k = 0; k < aux_.size(); ++k) {
-        Engine::check(hipEventRecord(join_[k], aux_[k]), "join");
-        Engine::check(hipStreamWaitEvent(stream, join_[k], 0), "join");
-      }
-    };
-    size_t next = 0;
-    for (Engine* e : near) {
-      JobDesc unused;
-      if (!split) {
-        e->step_forces(unused);
-        continue;
-      }
-      const hipStream_t own = e->use_stream(aux_[next++ % aux_.size()]);
-      try {
-        e->step_forces(unused);
-      } catch (...) {
-        e->use_stream(own);
-        join();
-        throw;
-      }
-      e->use_stream(own);
-    }
-    if (split) join();
+    launch_near_jobs(prof, split ? aux_[0] : stream, near_set);
     if (!ap.empty()) launch_allpairs(prof, stream, ap_set);
+    if (split) {
+      Engine::check(hipEventRecord(join_[0], aux_[0]), "join");
+      Engine::check(hipStreamWaitEvent(stream, join_[0], 0), "join");
+    }
     launch_bb_step(prof, stream, step_set);  // move and fold included
   };
   try {

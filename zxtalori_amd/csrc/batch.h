@@ -1,9 +1,10 @@
 // Many samples relaxed together (no counterpart in the reference, which runs one sample
 // per engine): every job is an Engine on the batch's stream; per step, the all-pairs
-// force passes of all jobs run as one launch, near-field jobs record their own site build
-// and pair kernel, and the BB step and fold of all jobs run as one launch per stage.  A
-// job's arithmetic is the same as alone (the multi-sample kernels are the ones a single
-// engine uses), so its result is bit-identical alone and in the mix.
+// force passes of all jobs run as one launch, the near-field passes of all jobs as one
+// launch per build stage and per force-kernel variant, and the BB step and fold of all
+// jobs as one launch per stage.  A job's arithmetic is the same as alone (the
+// multi-sample kernels share the single-sample kernels' code), so its result is
+// bit-identical alone and in the mix.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -21,9 +22,9 @@ enum class JobKind : uint32_t { Relax = 0, Measure = 1, Spacing = 2 };
 
 class Batch {
  public:
-  // use_graphs: replay steady-state steps from a captured graph.  streams: near-field
-  // passes of different jobs go round-robin to this many extra streams so that small
-  // passes overlap on the device (0: all on the batch stream).
+  // use_graphs: replay steady-state steps from a captured graph.  streams > 0: the
+  // near-field launches of a step run on an extra stream, beside the all-pairs launch
+  // (0: all on the batch stream).
   Batch(bool use_graphs, uint32_t streams);
   ~Batch();
 
@@ -90,6 +91,16 @@ class Batch {
   std::deque<hipEvent_t> inflight_;
   std::vector<hipEvent_t> spare_;
   JobSet upload(Table& t, const std::vector<JobDesc>& descs);
+  struct NearTable {
+    DevBuf desc, start;
+    NearJob* host_desc = nullptr;  // pinned
+    uint32_t* host_start = nullptr;
+    size_t capacity = 0;
+    std::vector<NearJob> last;
+    NearSet set;
+  };
+  NearTable near_table_;
+  NearSet upload_near(const std::vector<NearJob>& jobs);
   void finish(Job& j, bool recorded);
 
   std::vector<std::unique_ptr<Job>> jobs_;

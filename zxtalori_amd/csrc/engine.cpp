@@ -251,9 +251,14 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   ready = true;
 }
 
+void NearField::ready_for(Engine& e, const float4* q, uint32_t n_, uint32_t order_, uint32_t group_,
+                          double mean_radius_) {
+  if (!ready || n != n_ || order != order_ || group != group_) setup(e, q, n_, order_, group_, mean_radius_);
+}
+
 void NearField::build(Engine& e, const float4* q, uint32_t n_, uint32_t order_, uint32_t group_,
                       double mean_radius_, bool ordered) {
-  if (!ready || n != n_ || order != order_ || group != group_) setup(e, q, n_, order_, group_, mean_radius_);
+  ready_for(e, q, n_, order_, group_, mean_radius_);
   launch_near_build(e.prof, e.stream, q, n, e.ops_device(), order, view(), ordered);
 }
 
@@ -757,6 +762,25 @@ bool Engine::step_forces(JobDesc& allpairs) {
     return false;
   }
   allpairs = rs.allpairs_desc(*this, n_, order());
+  return true;
+}
+
+bool Engine::near_job(NearJob& job) {
+  prepare_step();
+  RelaxState& rs = relax_state_;
+  if (rs.current) return false;
+  const uint32_t ord = order();
+  float4* q = static_cast<float4*>(q_device());
+  rs.near.ready_for(*this, q, n_, ord, group_, mean_radius(uint64_t(n_) * ord));
+  job = NearJob{};
+  job.q = q;
+  job.ops = ops_device();
+  job.n = n_;
+  job.order = ord;
+  job.b = rs.near.view();
+  job.cut = rs.near.cut;
+  job.key = near_force_key(job.cut);
+  job.rows = rs.rows();
   return true;
 }
 

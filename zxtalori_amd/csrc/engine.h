@@ -131,6 +131,8 @@ struct NearField {
   // as launch_near_build takes it.
   void build(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius,
              bool ordered = true);
+  // Sets up (sizing pass, allocation, one wait) unless set up for this sample already.
+  void ready_for(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius);
   // After a wait: whether the last pass overflowed (then the next pass sizes again).
   bool overflowed(Engine& e);
 };
@@ -275,6 +277,9 @@ class Engine {
   void begin_relax();
   void prepare_step();
   bool step_forces(JobDesc& allpairs);
+  // A near-field sample's force pass of one step as a job of a multi-sample launch (the
+  // near field set up first); false when the last pass's forces are current.
+  bool near_job(NearJob& job);
   JobDesc step_job();
   void stepped();
   void finish_relax(bool recorded);

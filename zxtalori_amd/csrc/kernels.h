@@ -196,6 +196,7 @@ struct NearBuffers {
   float reach = 0;       // r + 1e-4: the angle within which a site can matter
   uint32_t capacity = 0;
   uint32_t sdim = 1;
+  uint32_t pad0 = 0;             // (no padding bytes: job tables compare bytewise)
   NearInfo* info = nullptr;
   uint32_t* rim = nullptr;       // n
   float4* uq = nullptr;          // capacity, unsorted sites
@@ -215,9 +216,12 @@ struct NearBuffers {
   uint32_t* counts = nullptr;    // 2n: per source, pairs and sites tested of the last force pass
   uint32_t* crowded = nullptr;   // crowded_capacity: cells with more sites than one thread sorts
   uint32_t crowded_capacity = 0;
+  uint32_t pad1 = 0;
   uint32_t* huge = nullptr;      // huge_capacity: cells with more sites than one wave sorts
   uint32_t huge_capacity = 0;
+  uint32_t pad2 = 0;
 };
+static_assert(sizeof(NearBuffers) == 200, "NearBuffers has no padding bytes");
 // The sources are visited by tiles of kTile^3 cells; the padded grid of tiles has this many
 // visits (at least sdim^3).
 constexpr uint32_t kTile = 8;
@@ -249,6 +253,54 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 // and that point's rank, and the least degree's bits (*least_bits).
 void launch_near_nearest(Prof& p, hipStream_t s, uint32_t n, uint32_t order, const NearBuffers& b, const NearCut& cut,
                          float* degrees_out, uint32_t* rank_out, uint32_t* least_bits);
+
+// The force kernel variant a cutoff takes: own images in reach (8) plus arctangent terms.
+uint32_t near_force_key(const NearCut& cut);
+
+// One sample of a multi-sample near-field pass (a batch step): every build stage and the
+// force pass run as one launch for all samples (one per force variant), each workgroup
+// finding its sample by the stage's prefix sums of workgroups.  Same arithmetic per sample
+// as launch_near_build + launch_near_force, so the rows are bit-identical.
+struct NearJob {
+  const float4* q = nullptr;
+  const float4* ops = nullptr;
+  uint32_t n = 0, order = 0;
+  NearBuffers b;
+  NearCut cut{};
+  uint32_t key = 0;  // near_force_key(cut)
+  Rows rows;
+};
+static_assert(sizeof(NearJob) == 288, "NearJob has no padding bytes");
+enum NearStage : uint32_t {
+  kNsFill,        // cell offsets cleared
+  kNsRim,         // core sites and rim list
+  kNsShell,       // shell sites
+  kNsSites,       // per-site count, place, rank place
+  kNsCellTiles,   // scan of the cell counts: tiles, then totals (one workgroup), then adds
+  kNsCellAdd,
+  kNsCrowded,     // crowded-cell sorts
+  kNsHuge,
+  kNsVisits,      // source visit counts and order
+  kNsVisitTiles,  // scan of the visit counts
+  kNsVisitAdd,
+  kNsForce,       // one wave per source
+  kNsReduce,      // pair and candidate counters
+  kNearStages
+};
+uint32_t near_stage_blocks(const NearJob& j, uint32_t stage);
+constexpr uint32_t kNearKeys = 16;
+struct NearSet {
+  const NearJob* table = nullptr;  // sorted by key
+  uint32_t count = 0;
+  const uint32_t* start = nullptr;  // kNearStages x (count + 1) workgroup prefix sums
+  uint32_t total[kNearStages] = {};
+  // Force launches: per variant, the first force workgroup and how many.
+  uint32_t keys = 0;
+  uint32_t key[kNearKeys] = {};
+  uint32_t key_first[kNearKeys] = {};
+  uint32_t key_blocks[kNearKeys] = {};
+};
+void launch_near_jobs(Prof& p, hipStream_t s, const NearSet& set);
 
 // SO(3) degrees of nearest chords (out) and their minimum's bits (*least).
 // Covering radius per source (S3 angle of its Voronoi cell's farthest vertex), the vertex,

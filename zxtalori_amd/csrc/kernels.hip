@@ -117,6 +117,21 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
   return v;
 }
 
+// Loads through a pointer of the global address space are global loads with a scalar
+// base; through a generic pointer read from memory (a job table's), the compiler cannot
+// tell where it points and emits flat loads, which take 64-bit VGPR addresses.
+template <typename T>
+using global_t = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ inline global_t<T>* to_global(T* p) {
+  return (global_t<T>*)p;
+}
+__device__ inline float4 load4(const global_t<const float4>* p) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = *(const global_t<const v4>*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Workgroup b runs on XCD b % 8: give each XCD a contiguous range of virtual blocks.
 __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t grid) {
   const uint32_t per = grid / 8;  // grid is a multiple of 8
@@ -705,10 +720,10 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& 
   return r;
 }
 
-__global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* __restrict__ data, size_t count,
-                                                    uint32_t* __restrict__ tile_sums) {
+__device__ __attribute__((always_inline)) inline void scan_tiles_body(uint32_t* __restrict__ data, size_t count,
+                                                    uint32_t* __restrict__ tile_sums, uint32_t block, uint32_t grid) {
   __shared__ uint32_t lds[kBlock];
-  const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
+  const size_t base = size_t(block) * kScanTile + size_t(threadIdx.x) * kScanItems;
   uint32_t v[kScanItems];
   uint32_t sum = 0;
 #pragma unroll
@@ -723,11 +738,13 @@ __global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* __restrict__ data,
 #pragma unroll
   for (uint32_t k = 0; k < kScanItems; ++k)
     if (base + k < count) data[base + k] = v[k] + off;
-  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+  if (threadIdx.x == 0) tile_sums[block] = total;
 }
+__global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* __restrict__ data, size_t count,
+                                                    uint32_t* __restrict__ tile_sums) { scan_tiles_body(data, count, tile_sums, blockIdx.x, gridDim.x); }
 
-__global__ __launch_bounds__(256) void k_scan_sums(uint32_t* __restrict__ tile_sums, uint32_t ntiles,
-                                                   uint32_t* __restrict__ total_out) {
+__device__ __attribute__((always_inline)) inline void scan_sums_body(uint32_t* __restrict__ tile_sums, uint32_t ntiles,
+                                                   uint32_t* __restrict__ total_out, uint32_t block, uint32_t grid) {
   __shared__ uint32_t lds[kBlock];
   const uint32_t per = (ntiles + kBlock - 1) / kBlock;
   const uint32_t b0 = threadIdx.x * per, b1 = min(ntiles, b0 + per);
@@ -742,11 +759,14 @@ __global__ __launch_bounds__(256) void k_scan_sums(uint32_t* __restrict__ tile_s
   }
   if (threadIdx.x == 0) *total_out = total;
 }
+__global__ __launch_bounds__(256) void k_scan_sums(uint32_t* __restrict__ tile_sums, uint32_t ntiles,
+                                                   uint32_t* __restrict__ total_out) { scan_sums_body(tile_sums, ntiles, total_out, blockIdx.x, gridDim.x); }
 
-__global__ void k_scan_add(uint32_t* __restrict__ data, size_t count, const uint32_t* __restrict__ tile_sums) {
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x)
+__device__ __attribute__((always_inline)) inline void scan_add_body(uint32_t* __restrict__ data, size_t count, const uint32_t* __restrict__ tile_sums, uint32_t block, uint32_t grid) {
+  for (size_t i = size_t(block) * blockDim.x + threadIdx.x; i < count; i += size_t(grid) * blockDim.x)
     data[i] += tile_sums[i / kScanTile];
 }
+__global__ void k_scan_add(uint32_t* __restrict__ data, size_t count, const uint32_t* __restrict__ tile_sums) { scan_add_body(data, count, tile_sums, blockIdx.x, gridDim.x); }
 
 // Exclusive scan of data[0..count) in place, data[count] = total.  tmp: scan_tmp_words(count).
 static void launch_scan(Prof& p, hipStream_t s, uint32_t* data, size_t count, uint32_t* tmp) {
@@ -760,7 +780,7 @@ static void launch_scan(Prof& p, hipStream_t s, uint32_t* data, size_t count, ui
 }
 
 // ---- near-field sites (shell.wgsl:19-130, near.wgsl:22-82) -----------------------------
-__global__ void k_near_reset(NearInfo* info, uint32_t n) {
+__device__ inline void near_reset_body(NearInfo* info, uint32_t n) {
   info->sites = n;
   info->overflow = 0;
   info->rim = 0;
@@ -772,6 +792,7 @@ __global__ void k_near_reset(NearInfo* info, uint32_t n) {
     info->hi[c] = 0u;
   }
 }
+__global__ void k_near_reset(NearInfo* info, uint32_t n) { near_reset_body(info, n); }
 
 __device__ inline void box_reduce_commit(NearInfo* info, uint32_t lo[3], uint32_t hi[3]) {
   __shared__ uint32_t slo[3][kBlock / 64], shi[3][kBlock / 64];
@@ -831,14 +852,14 @@ __device__ inline void block_min_commit(uint32_t* dst, uint32_t v) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, uint32_t n,
+__device__ __attribute__((always_inline)) inline void core_rim_body(const float4* __restrict__ q, uint32_t n,
                                                   const float4* __restrict__ planes, uint32_t nplanes, float band,
-                                                  uint32_t two_order, NearBuffers b) {
+                                                  uint32_t two_order, NearBuffers b, uint32_t block, uint32_t grid) {
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   uint32_t wmin = 0xffffffffu;
-  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t stride = grid * blockDim.x;
   const uint32_t rounds = (n + stride - 1) / stride;  // at most 32 (core_rim_blocks)
-  const uint32_t first = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t first = block * blockDim.x + threadIdx.x;
   uint32_t rim_mask = 0;  // bit r: source r * stride + first is a rim source
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint32_t i = r * stride + first;
@@ -880,6 +901,9 @@ __global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, 
   box_reduce_commit(b.info, lo, hi);
   block_min_commit(&b.info->wmin_key, wmin);
 }
+__global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, uint32_t n,
+                                                  const float4* __restrict__ planes, uint32_t nplanes, float band,
+                                                  uint32_t two_order, NearBuffers b) { core_rim_body(q, n, planes, nplanes, band, two_order, b, blockIdx.x, gridDim.x); }
 
 // Workgroups of k_core_rim: at least 512 (measured at I, N = 1e6: 0.06 ms against 0.17 ms
 // with 2048), and enough that no thread takes more than 32 sources (its rim flags fit one
@@ -919,20 +943,20 @@ __device__ inline bool shell_image(const float4 x, uint32_t img, const float4* _
 // wave reserves room for all of its lanes' sites with one atomic, then each thread writes
 // its own.  The append order varies; the cell sort's in-cell key order makes the result
 // deterministic.
-__global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uint32_t n,
+__device__ __attribute__((always_inline)) inline void shell_body(const float4* __restrict__ q, uint32_t n,
                                                const float4* __restrict__ ops, uint32_t order,
                                                const float4* __restrict__ planes, uint32_t nplanes, float band,
-                                               NearBuffers b) {
+                                               NearBuffers b, uint32_t block, uint32_t grid) {
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   const uint32_t m = b.info->rim;
   const uint32_t per = 2 * order - 1;
   const float w_floor =
       cosf(fminf(3.14159265f, acosf(fminf(1.0f, unordered_key(b.info->wmin_key))) + b.reach)) - 1e-6f;
-  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t stride = grid * blockDim.x;
   const uint32_t rounds = (m + stride - 1) / stride;
   const uint32_t lane = lane_id();
   for (uint32_t r = 0; r < rounds; ++r) {
-    const uint32_t t = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = r * stride + block * blockDim.x + threadIdx.x;
     const bool live = t < m;
     const uint32_t src = live ? b.rim[t] : 0u;
     const float4 x = live ? q[src] : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
@@ -980,8 +1004,12 @@ __global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uin
   }
   box_reduce_commit(b.info, lo, hi);
 }
+__global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uint32_t n,
+                                               const float4* __restrict__ ops, uint32_t order,
+                                               const float4* __restrict__ planes, uint32_t nplanes, float band,
+                                               NearBuffers b) { shell_body(q, n, ops, order, planes, nplanes, band, b, blockIdx.x, gridDim.x); }
 
-__global__ void k_box_finalize(NearInfo* info, uint32_t sdim) {
+__device__ inline void box_finalize_body(NearInfo* info, uint32_t sdim) {
   float lo[3], hi[3];
   float width = 1e-6f;
   for (int c = 0; c < 3; ++c) {
@@ -993,6 +1021,7 @@ __global__ void k_box_finalize(NearInfo* info, uint32_t sdim) {
   for (int c = 0; c < 3; ++c) info->lower[c] = lo[c] - 1e-5f * width;
   info->cpu = float(sdim) / (width * 1.0001f);
 }
+__global__ void k_box_finalize(NearInfo* info, uint32_t sdim) { box_finalize_body(info, sdim); }
 
 __device__ inline uint32_t cell_axis(float v, float lower, float cpu, uint32_t dim) {
   const float c = floorf((v - lower) * cpu);
@@ -1000,17 +1029,18 @@ __device__ inline uint32_t cell_axis(float v, float lower, float cpu, uint32_t d
 }
 
 // Each site's cell and its arrival slot in that cell (the count before its own).
-__global__ void k_site_count(NearBuffers b) {
+__device__ __attribute__((always_inline)) inline void site_count_body(NearBuffers b, uint32_t block, uint32_t grid) {
   const uint32_t S = min(b.info->sites, b.capacity);
   const float lx = b.info->lower[0], ly = b.info->lower[1], lz = b.info->lower[2], cpu = b.info->cpu;
   const uint32_t d = b.sdim;
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
+  for (uint32_t s = block * blockDim.x + threadIdx.x; s < S; s += grid * blockDim.x) {
     const float3 p = chart(b.uq[s]);
     const uint32_t c = cell_axis(p.x, lx, cpu, d) + d * (cell_axis(p.y, ly, cpu, d) + d * cell_axis(p.z, lz, cpu, d));
     b.ucell[s] = c;
     b.uslot[s] = atomicAdd(&b.offsets[c], 1u);
   }
 }
+__global__ void k_site_count(NearBuffers b) { site_count_body(b, blockIdx.x, gridDim.x); }
 
 // Cells of up to kThreadCell sites are put in rank order by counting (k_site_rank_place);
 // larger ones by sorting an index list (k_cell_sort_waves, k_cell_sort_block).
@@ -1020,9 +1050,9 @@ constexpr uint32_t kThreadCell = 16;
 // Unordered (spacing): the site itself goes there.  Ordered: a cell of at most kThreadCell
 // sites gets the site's rank there (k_site_rank_place then moves each site to its place
 // in rank order); a larger one the site's index (perm), for the crowded-cell sorts.
-__global__ void k_site_place(NearBuffers b, uint32_t ordered) {
+__device__ __attribute__((always_inline)) inline void site_place_body(NearBuffers b, uint32_t ordered, uint32_t block, uint32_t grid) {
   const uint32_t S = min(b.info->sites, b.capacity);
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
+  for (uint32_t s = block * blockDim.x + threadIdx.x; s < S; s += grid * blockDim.x) {
     const uint32_t c = b.ucell[s];
     const uint32_t begin = b.offsets[c], dst = begin + b.uslot[s];
     if (!ordered) {
@@ -1034,6 +1064,7 @@ __global__ void k_site_place(NearBuffers b, uint32_t ordered) {
     }
   }
 }
+__global__ void k_site_place(NearBuffers b, uint32_t ordered) { site_place_body(b, ordered, blockIdx.x, gridDim.x); }
 
 // Moves unsorted site s to sorted position k.
 __device__ inline void move_site(const NearBuffers& b, uint32_t k, uint32_t s) {
@@ -1047,9 +1078,9 @@ __device__ inline void move_site(const NearBuffers& b, uint32_t k, uint32_t s) {
 // of its cell's ranks below its own (read from the ranks k_site_place laid out; the loads
 // are independent, no sorting chain); larger cells are listed for k_cell_sort_waves (by
 // the site that arrived first).
-__global__ void k_site_rank_place(NearBuffers b) {
+__device__ __attribute__((always_inline)) inline void site_rank_place_body(NearBuffers b, uint32_t block, uint32_t grid) {
   const uint32_t S = min(b.info->sites, b.capacity);
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < S; s += gridDim.x * blockDim.x) {
+  for (uint32_t s = block * blockDim.x + threadIdx.x; s < S; s += grid * blockDim.x) {
     const uint32_t c = b.ucell[s];
     const uint32_t begin = b.offsets[c], m = b.offsets[c + 1] - begin;
     if (m > kThreadCell) {
@@ -1062,6 +1093,7 @@ __global__ void k_site_rank_place(NearBuffers b) {
     move_site(b, begin + below, s);
   }
 }
+__global__ void k_site_rank_place(NearBuffers b) { site_rank_place_body(b, blockIdx.x, gridDim.x); }
 
 // Crowded cells (more than kThreadCell items: coarse cells, or a wide cutoff over a small
 // sample crowding the chart's centre), one wave each: up to 64 items placed by rank (the
@@ -1073,18 +1105,18 @@ struct SortLds {
   uint32_t k[kWaveChunk];
   uint32_t v[kWaveChunk];
 };
-__global__ __launch_bounds__(256) void k_cell_sort_waves(const uint32_t* __restrict__ offsets,
+__device__ __attribute__((always_inline)) inline void cell_sort_waves_body(const uint32_t* __restrict__ offsets,
                                                          uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
                                                          const uint32_t* __restrict__ crowded_list,
                                                          const uint32_t* __restrict__ crowded, uint32_t list_capacity,
                                                          uint32_t* __restrict__ huge_list, uint32_t* __restrict__ huge,
-                                                         NearBuffers sites) {
+                                                         NearBuffers sites, uint32_t block, uint32_t grid) {
   __shared__ SortLds lds[kBlock / 64];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   SortLds& L = lds[wave];
   const uint32_t total = min(*crowded, list_capacity);
-  const uint32_t waves = gridDim.x * (kBlock / 64);
-  for (uint32_t b = blockIdx.x * (kBlock / 64) + wave; b < total; b += waves) {
+  const uint32_t waves = grid * (kBlock / 64);
+  for (uint32_t b = block * (kBlock / 64) + wave; b < total; b += waves) {
     const uint32_t c = crowded_list[b];
     const uint32_t begin = offsets[c], m = offsets[c + 1] - begin;
     if (m > kWaveChunk) {
@@ -1139,20 +1171,26 @@ __global__ __launch_bounds__(256) void k_cell_sort_waves(const uint32_t* __restr
     wave_sync();
   }
 }
+__global__ __launch_bounds__(256) void k_cell_sort_waves(const uint32_t* __restrict__ offsets,
+                                                         uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
+                                                         const uint32_t* __restrict__ crowded_list,
+                                                         const uint32_t* __restrict__ crowded, uint32_t list_capacity,
+                                                         uint32_t* __restrict__ huge_list, uint32_t* __restrict__ huge,
+                                                         NearBuffers sites) { cell_sort_waves_body(offsets, perm, key, crowded_list, crowded, list_capacity, huge_list, huge, sites, blockIdx.x, gridDim.x); }
 
 // Cells of more than kWaveChunk items, one workgroup each: chunks of up to kBlockChunk
 // sorted in LDS (bitonic), then, for cells of more than one chunk, every item placed at
 // its rank (its index in its chunk plus, per other chunk, the count of smaller keys found
 // by binary search) through tmp.
 constexpr uint32_t kBlockChunk = 8192;
-__global__ __launch_bounds__(256) void k_cell_sort_block(const uint32_t* __restrict__ offsets,
+__device__ __attribute__((always_inline)) inline void cell_sort_block_body(const uint32_t* __restrict__ offsets,
                                                          uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
                                                          const uint32_t* __restrict__ huge_list,
                                                          const uint32_t* __restrict__ huge, uint32_t list_capacity,
-                                                         uint32_t* __restrict__ tmp, NearBuffers sites) {
+                                                         uint32_t* __restrict__ tmp, NearBuffers sites, uint32_t block, uint32_t grid) {
   __shared__ uint32_t sk[kBlockChunk], sv[kBlockChunk];
   const uint32_t total = min(*huge, list_capacity);
-  for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+  for (uint32_t b = block; b < total; b += grid) {
     const uint32_t c = huge_list[b];
     const uint32_t begin = offsets[c], m = offsets[c + 1] - begin;
     for (uint32_t cb = 0; cb < m; cb += kBlockChunk) {
@@ -1216,6 +1254,11 @@ __global__ __launch_bounds__(256) void k_cell_sort_block(const uint32_t* __restr
     __syncthreads();
   }
 }
+__global__ __launch_bounds__(256) void k_cell_sort_block(const uint32_t* __restrict__ offsets,
+                                                         uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
+                                                         const uint32_t* __restrict__ huge_list,
+                                                         const uint32_t* __restrict__ huge, uint32_t list_capacity,
+                                                         uint32_t* __restrict__ tmp, NearBuffers sites) { cell_sort_block_body(offsets, perm, key, huge_list, huge, list_capacity, tmp, sites, blockIdx.x, gridDim.x); }
 
 // The sources are visited tile by tile (8x8x8 cells, tiles in row-major order, cells
 // row-major within a tile), each cell's identity images in their sorted order: waves of
@@ -1233,21 +1276,22 @@ __device__ inline bool visit_cell(uint32_t v, uint32_t sdim, uint32_t& cell) {
 }
 
 // ucell (free once the scatter has run) takes each visit's count of identity images.
-__global__ void k_core_count(NearBuffers b) {
+__device__ __attribute__((always_inline)) inline void core_count_body(NearBuffers b, uint32_t block, uint32_t grid) {
   const uint32_t visits = tile_visits(b.sdim);
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < visits; v += gridDim.x * blockDim.x) {
+  for (uint32_t v = block * blockDim.x + threadIdx.x; v < visits; v += grid * blockDim.x) {
     uint32_t cell, count = 0;
     if (visit_cell(v, b.sdim, cell))
       for (uint32_t k = b.offsets[cell]; k < b.offsets[cell + 1]; ++k) count += b.scode[k] >> 31;
     b.ucell[v] = count;
   }
 }
+__global__ void k_core_count(NearBuffers b) { core_count_body(b, blockIdx.x, gridDim.x); }
 
 // With ucell scanned: each visited cell writes its sources from its offset on (and, for
 // the spacing pass, their positions).
-__global__ void k_core_order(NearBuffers b, uint32_t positions) {
+__device__ __attribute__((always_inline)) inline void core_order_body(NearBuffers b, uint32_t positions, uint32_t block, uint32_t grid) {
   const uint32_t visits = tile_visits(b.sdim);
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < visits; v += gridDim.x * blockDim.x) {
+  for (uint32_t v = block * blockDim.x + threadIdx.x; v < visits; v += grid * blockDim.x) {
     uint32_t cell;
     if (!visit_cell(v, b.sdim, cell)) continue;
     uint32_t at = b.ucell[v];
@@ -1261,6 +1305,7 @@ __global__ void k_core_order(NearBuffers b, uint32_t positions) {
     }
   }
 }
+__global__ void k_core_order(NearBuffers b, uint32_t positions) { core_order_body(b, positions, blockIdx.x, gridDim.x); }
 
 static uint32_t stride_grid(size_t work) { return uint32_t(std::min<size_t>(8192, (work + kBlock - 1) / kBlock + 1)); }
 
@@ -1329,19 +1374,20 @@ __device__ inline float row_gap(float c, int idx, uint32_t dim) {
 // all beyond the cutoff, the source's own identity image (chord exactly 0) being dropped
 // when its pair is evaluated, and the nearest chord is taken over the compacted pairs.
 // TERMS: the arctangent series (atan_small).
+// vb: the workgroup's index in the source order (4 sources each).
 template <bool OWN, int TERMS>
-__global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint32_t n,
-                                              const uint32_t* __restrict__ order, const float4* __restrict__ sq,
-                                              const uint32_t* __restrict__ scode,
-                                              const uint32_t* __restrict__ offsets, const NearInfo* __restrict__ info,
-                                              uint32_t sdim, NearCut cut,
-                                              uint32_t real_blocks, Rows rows, uint32_t* __restrict__ counts) {
+__device__ __attribute__((always_inline)) inline void near_body(
+    const float4* __restrict__ q_in, uint32_t n, const uint32_t* __restrict__ order_in,
+    const float4* __restrict__ sq_in, const uint32_t* __restrict__ scode_in, const uint32_t* __restrict__ offsets_in,
+    const NearInfo* __restrict__ info_in, uint32_t sdim, NearCut cut, Rows rows, uint32_t* __restrict__ counts,
+    uint32_t vb) {
+  const auto* __restrict__ q = to_global(q_in);
+  const auto* __restrict__ order = to_global(order_in);
+  const auto* __restrict__ sq = to_global(sq_in);
+  const auto* __restrict__ scode = to_global(scode_in);
+  const auto* __restrict__ offsets = to_global(offsets_in);
+  const auto* __restrict__ info = to_global(info_in);
   __shared__ WaveLds lds[kBlock / 64];
-  // Each XCD takes a contiguous range of the tile-by-tile source order, so its L2 holds
-  // that range's sites: 420 MB of HBM traffic per pass at I, N = 1e6, against 1061 MB
-  // with the workgroups dealt round-robin over the XCDs (at the same kernel time).
-  const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
-  if (vb >= real_blocks) return;
   // The wave index is uniform: the source, its position and every per-source constant
   // below then live in scalar registers, leaving the VGPRs to the candidate and pair work.
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -1349,7 +1395,7 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   if (slot >= n) return;  // the whole wave leaves; no workgroup barrier follows
   WaveLds& L = lds[wave];
   const uint32_t src = order[slot];
-  const float4 x = q[src];
+  const float4 x = load4(q + src);
   const float cpu = info->cpu;
   const float fdim = float(sdim);
   const float denom = cut.c0 + x.x;
@@ -1430,15 +1476,15 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
       if (n_row == 0) continue;
       const uint32_t s_row = uint32_t(__builtin_amdgcn_readlane(int(start), int(rr)));
       // The row's sites and codes from a uniform base: lane offsets stay 32-bit.
-      const float4* __restrict__ row_sites = sq + s_row;
-      const uint32_t* __restrict__ row_codes = scode + s_row;
+      const auto* __restrict__ row_sites = sq + s_row;
+      const auto* __restrict__ row_codes = scode + s_row;
       for (uint32_t c = 0; c < n_row; c += 64) {
         const uint32_t k = c + lane;
         float4 d;
         float dm2 = 8.0f;  // beyond every cutoff: the lanes past the row's end
         bool own = false;  // the source's own images exert no force (pairs.wgsl:404-408)
         if (k < n_row) {   // the lanes past the row's end load nothing
-          const float4 y = row_sites[k];
+          const float4 y = load4(row_sites + k);
           d = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
           dm2 = dot4(d, d);  // chord2(x, y), the same operations
           if (OWN) {
@@ -1502,11 +1548,26 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
   }
 }
 
+template <bool OWN, int TERMS>
+__global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint32_t n,
+                                              const uint32_t* __restrict__ order, const float4* __restrict__ sq,
+                                              const uint32_t* __restrict__ scode,
+                                              const uint32_t* __restrict__ offsets, const NearInfo* __restrict__ info,
+                                              uint32_t sdim, NearCut cut,
+                                              uint32_t real_blocks, Rows rows, uint32_t* __restrict__ counts) {
+  // Each XCD takes a contiguous range of the tile-by-tile source order, so its L2 holds
+  // that range's sites: 420 MB of HBM traffic per pass at I, N = 1e6, against 1061 MB
+  // with the workgroups dealt round-robin over the XCDs (at the same kernel time).
+  const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
+  if (vb >= real_blocks) return;
+  near_body<OWN, TERMS>(q, n, order, sq, scode, offsets, info, sdim, cut, rows, counts, vb);
+}
+
 // Sums the per-source pair and candidate counts of a force pass into counters[0..1].
-__global__ __launch_bounds__(256) void k_count_reduce(const uint32_t* __restrict__ counts, uint32_t n,
-                                                      unsigned long long* __restrict__ counters) {
+__device__ __attribute__((always_inline)) inline void count_reduce_body(const uint32_t* __restrict__ counts, uint32_t n,
+                                                      unsigned long long* __restrict__ counters, uint32_t block, uint32_t grid) {
   unsigned long long a = 0, b = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = block * blockDim.x + threadIdx.x; i < n; i += grid * blockDim.x) {
     a += counts[2 * size_t(i)];
     b += counts[2 * size_t(i) + 1];
   }
@@ -1517,6 +1578,8 @@ __global__ __launch_bounds__(256) void k_count_reduce(const uint32_t* __restrict
     atomicAdd(&counters[1], b);
   }
 }
+__global__ __launch_bounds__(256) void k_count_reduce(const uint32_t* __restrict__ counts, uint32_t n,
+                                                      unsigned long long* __restrict__ counters) { count_reduce_body(counts, n, counters, blockIdx.x, gridDim.x); }
 
 void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, const NearBuffers& b, const NearCut& cut,
                        Rows out) {
@@ -1550,6 +1613,204 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
     hipLaunchKernelGGL(k_count_reduce, dim3(std::min<uint32_t>(256, blocks_for(n, kBlock))), dim3(kBlock), 0, s,
                        b.counts, n, b.counters);
   });
+}
+
+// ---- near field of many samples in one launch per stage (a batch step) ------------
+uint32_t near_force_key(const NearCut& cut) { return (cut.own ? 8u : 0u) + uint32_t(atan_terms(cut.chord2)); }
+
+// Workgroups per job and stage.  Grid-stride stages take 8 items per thread: a workgroup
+// first finds its job (a search over the stage's prefix sums), so each should do more
+// than one round of work.
+uint32_t near_stage_blocks(const NearJob& j, uint32_t stage) {
+  const size_t cells = size_t(j.b.sdim) * j.b.sdim * j.b.sdim, visits = tile_visits(j.b.sdim);
+  const uint32_t per = kBlock * 8;
+  switch (stage) {
+    case kNsFill: return std::min<uint32_t>(4096, blocks_for(cells + 1, per));
+    case kNsRim: return std::min(core_rim_blocks(j.n), blocks_for(j.n, kBlock));
+    case kNsShell: return std::min<uint32_t>(2048, blocks_for(j.n, kBlock));  // rim sources <= n
+    case kNsSites: return std::min<uint32_t>(8192, blocks_for(j.b.capacity, per));
+    case kNsCellTiles: return blocks_for(cells, kScanTile);
+    case kNsCellAdd: return std::min<uint32_t>(8192, blocks_for(cells, per));
+    case kNsCrowded: return std::min<uint32_t>(kCrowdedBlocks, blocks_for(j.b.crowded_capacity, kBlock / 64));
+    case kNsHuge: return std::min<uint32_t>(kHugeBlocks, j.b.huge_capacity);
+    case kNsVisits: return std::min<uint32_t>(8192, blocks_for(visits, per));
+    case kNsVisitTiles: return blocks_for(visits, kScanTile);
+    case kNsVisitAdd: return std::min<uint32_t>(8192, blocks_for(visits, per));
+    case kNsForce: return blocks_for(j.n, kBlock / 64);
+    case kNsReduce: return std::min<uint32_t>(256, blocks_for(j.n, kBlock));
+    default: return 0;
+  }
+}
+
+namespace {
+// The job whose workgroup range of `stage` holds workgroup g (uniform), with the
+// workgroup's index within the job's range and the range's size.
+struct JobPart {
+  const NearJob* job;
+  uint32_t block, grid;
+};
+__device__ inline JobPart near_part(const NearSet& set, uint32_t stage, uint32_t g) {
+  const uint32_t* st = set.start + size_t(stage) * (set.count + 1);
+  uint32_t lo = 0, hi = set.count;  // st[lo] <= g < st[hi]; a job with no workgroups is never found
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (st[mid] <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return {set.table + lo, g - st[lo], st[lo + 1] - st[lo]};
+}
+}  // namespace
+
+// Per job: the pass's status and the force pass's counters start again.
+__global__ void k_jobs_reset(NearSet set) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= set.count) return;
+  const NearJob& job = set.table[j];
+  near_reset_body(job.b.info, job.n);
+  job.b.counters[0] = 0ull;
+  job.b.counters[1] = 0ull;
+}
+__global__ void k_jobs_fill_offsets(NearSet set) {
+  const JobPart w = near_part(set, kNsFill, blockIdx.x);
+  const size_t count = size_t(w.job->b.sdim) * w.job->b.sdim * w.job->b.sdim + 1;
+  for (size_t i = size_t(w.block) * blockDim.x + threadIdx.x; i < count; i += size_t(w.grid) * blockDim.x)
+    w.job->b.offsets[i] = 0u;
+}
+__global__ __launch_bounds__(256) void k_jobs_core_rim(NearSet set) {
+  const JobPart w = near_part(set, kNsRim, blockIdx.x);
+  const NearJob& j = *w.job;
+  core_rim_body(j.q, j.n, j.b.planes, j.b.nplanes, j.b.band, 2 * j.order, j.b, w.block, w.grid);
+}
+__global__ __launch_bounds__(256) void k_jobs_shell(NearSet set) {
+  const JobPart w = near_part(set, kNsShell, blockIdx.x);
+  const NearJob& j = *w.job;
+  shell_body(j.q, j.n, j.ops, j.order, j.b.planes, j.b.nplanes, j.b.band, j.b, w.block, w.grid);
+}
+__global__ void k_jobs_box(NearSet set) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < set.count) box_finalize_body(set.table[j].b.info, set.table[j].b.sdim);
+}
+// stage 0 count, 1 place (ordered), 2 rank place.
+__global__ void k_jobs_sites(NearSet set, uint32_t stage) {
+  const JobPart w = near_part(set, kNsSites, blockIdx.x);
+  if (stage == 0)
+    site_count_body(w.job->b, w.block, w.grid);
+  else if (stage == 1)
+    site_place_body(w.job->b, 1u, w.block, w.grid);
+  else
+    site_rank_place_body(w.job->b, w.block, w.grid);
+}
+// Scans of the cell counts (visits 0: offsets over sdim^3 cells) or of the visit counts
+// (visits 1: ucell over tile_visits(sdim)), each in its job's scan_tmp.
+__device__ inline void scan_of(const NearJob& j, uint32_t visits, uint32_t*& data, size_t& count) {
+  const uint32_t d = j.b.sdim;
+  data = visits ? j.b.ucell : j.b.offsets;
+  count = visits ? size_t(tile_visits(d)) : size_t(d) * d * d;
+}
+__global__ __launch_bounds__(256) void k_jobs_scan_tiles(NearSet set, uint32_t visits) {
+  const JobPart w = near_part(set, visits ? kNsVisitTiles : kNsCellTiles, blockIdx.x);
+  uint32_t* data;
+  size_t count;
+  scan_of(*w.job, visits, data, count);
+  scan_tiles_body(data, count, w.job->b.scan_tmp, w.block, w.grid);
+}
+__global__ __launch_bounds__(256) void k_jobs_scan_sums(NearSet set, uint32_t visits) {
+  const NearJob& j = set.table[blockIdx.x];
+  uint32_t* data;
+  size_t count;
+  scan_of(j, visits, data, count);
+  scan_sums_body(j.b.scan_tmp, uint32_t((count + kScanTile - 1) / kScanTile), data + count, 0u, 1u);
+}
+__global__ void k_jobs_scan_add(NearSet set, uint32_t visits) {
+  const JobPart w = near_part(set, visits ? kNsVisitAdd : kNsCellAdd, blockIdx.x);
+  uint32_t* data;
+  size_t count;
+  scan_of(*w.job, visits, data, count);
+  scan_add_body(data, count, w.job->b.scan_tmp, w.block, w.grid);
+}
+__global__ __launch_bounds__(256) void k_jobs_sort_waves(NearSet set) {
+  const JobPart w = near_part(set, kNsCrowded, blockIdx.x);
+  const NearBuffers& b = w.job->b;
+  cell_sort_waves_body(b.offsets, b.perm, b.urank, b.crowded, &b.info->crowded, b.crowded_capacity, b.huge,
+                       &b.info->huge, b, w.block, w.grid);
+}
+__global__ __launch_bounds__(256) void k_jobs_sort_block(NearSet set) {
+  const JobPart w = near_part(set, kNsHuge, blockIdx.x);
+  const NearBuffers& b = w.job->b;
+  cell_sort_block_body(b.offsets, b.perm, b.urank, b.huge, &b.info->huge, b.huge_capacity, b.ucell, b, w.block,
+                       w.grid);
+}
+// stage 0 visit counts, 1 source order.
+__global__ void k_jobs_visits(NearSet set, uint32_t stage) {
+  const JobPart w = near_part(set, kNsVisits, blockIdx.x);
+  if (stage == 0)
+    core_count_body(w.job->b, w.block, w.grid);
+  else
+    core_order_body(w.job->b, 0u, w.block, w.grid);
+}
+// The force workgroups of the jobs of one variant: [first, first + blocks) of the stage.
+template <bool OWN, int TERMS>
+__global__ __launch_bounds__(256) void k_jobs_near(NearSet set, uint32_t first, uint32_t blocks) {
+  const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
+  if (vb >= blocks) return;
+  const JobPart w = near_part(set, kNsForce, first + vb);
+  const NearJob& j = *w.job;
+  near_body<OWN, TERMS>(j.q, j.n, j.b.order, j.b.sq, j.b.scode, j.b.offsets, j.b.info, j.b.sdim, j.cut, j.rows,
+                        j.b.counts, w.block);
+}
+__global__ __launch_bounds__(256) void k_jobs_count_reduce(NearSet set) {
+  const JobPart w = near_part(set, kNsReduce, blockIdx.x);
+  count_reduce_body(w.job->b.counts, w.job->n, w.job->b.counters, w.block, w.grid);
+}
+
+void launch_near_jobs(Prof& p, hipStream_t s, const NearSet& set) {
+  if (!set.count) return;
+  const uint32_t per_job = blocks_for(set.count, kBlock);
+  auto stage = [&](const char* name, uint32_t st, auto kernel, auto... args) {
+    if (!set.total[st]) return;
+    p.run(name, [&] { hipLaunchKernelGGL(kernel, dim3(set.total[st]), dim3(kBlock), 0, s, set, args...); });
+  };
+  p.run("near_reset", [&] { hipLaunchKernelGGL(k_jobs_reset, dim3(per_job), dim3(kBlock), 0, s, set); });
+  stage("fill", kNsFill, k_jobs_fill_offsets);
+  stage("core_rim", kNsRim, k_jobs_core_rim);
+  stage("shell", kNsShell, k_jobs_shell);
+  p.run("box", [&] { hipLaunchKernelGGL(k_jobs_box, dim3(per_job), dim3(kBlock), 0, s, set); });
+  stage("site_count", kNsSites, k_jobs_sites, 0u);
+  stage("scan", kNsCellTiles, k_jobs_scan_tiles, 0u);
+  p.run("scan", [&] { hipLaunchKernelGGL(k_jobs_scan_sums, dim3(set.count), dim3(kBlock), 0, s, set, 0u); });
+  stage("scan", kNsCellAdd, k_jobs_scan_add, 0u);
+  stage("site_place", kNsSites, k_jobs_sites, 1u);
+  stage("site_sort", kNsSites, k_jobs_sites, 2u);
+  stage("site_sort", kNsCrowded, k_jobs_sort_waves);
+  stage("site_sort", kNsHuge, k_jobs_sort_block);
+  stage("core_order", kNsVisits, k_jobs_visits, 0u);
+  stage("scan", kNsVisitTiles, k_jobs_scan_tiles, 1u);
+  p.run("scan", [&] { hipLaunchKernelGGL(k_jobs_scan_sums, dim3(set.count), dim3(kBlock), 0, s, set, 1u); });
+  stage("scan", kNsVisitAdd, k_jobs_scan_add, 1u);
+  stage("core_order", kNsVisits, k_jobs_visits, 1u);
+  for (uint32_t k = 0; k < set.keys; ++k) {
+    const uint32_t first = set.key_first[k], blocks = set.key_blocks[k];
+    p.run("near_force", [&] {
+      auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(xcd_grid(blocks)), dim3(kBlock), 0, s, set, first, blocks);
+      };
+      switch (set.key[k]) {
+        case 8 + 2: go(k_jobs_near<true, 2>); break;
+        case 8 + 3: go(k_jobs_near<true, 3>); break;
+        case 8 + 4: go(k_jobs_near<true, 4>); break;
+        case 8 + 5: go(k_jobs_near<true, 5>); break;
+        case 8 + 0: go(k_jobs_near<true, 0>); break;
+        case 2: go(k_jobs_near<false, 2>); break;
+        case 3: go(k_jobs_near<false, 3>); break;
+        case 4: go(k_jobs_near<false, 4>); break;
+        case 5: go(k_jobs_near<false, 5>); break;
+        default: go(k_jobs_near<false, 0>); break;
+      }
+    });
+  }
+  stage("count_reduce", kNsReduce, k_jobs_count_reduce);
 }
 
 // Nearest orbit point within the cutoff for spacing (pairs.zig:318-335): a small cap (4
