@@ -1047,7 +1047,7 @@ __global__ void k_site_count(NearBuffers b) { site_count_body(b, blockIdx.x, gri
 constexpr uint32_t kThreadCell = 16;
 
 // With the counts scanned into offsets, each site takes its cell's start plus its slot.
-// Unordered (spacing): the site itself goes there.  Ordered: a cell of at most kThreadCell
+// Unordered (spacing): the site itself (position and rank) goes there.  Ordered: a cell of at most kThreadCell
 // sites gets the site's rank there (k_site_rank_place then moves each site to its place
 // in rank order); a larger one the site's index (perm), for the crowded-cell sorts.
 __device__ __attribute__((always_inline)) inline void site_place_body(NearBuffers b, uint32_t ordered, uint32_t block, uint32_t grid) {
@@ -1055,9 +1055,8 @@ __device__ __attribute__((always_inline)) inline void site_place_body(NearBuffer
   for (uint32_t s = block * blockDim.x + threadIdx.x; s < S; s += grid * blockDim.x) {
     const uint32_t c = b.ucell[s];
     const uint32_t begin = b.offsets[c], dst = begin + b.uslot[s];
-    if (!ordered) {
+    if (!ordered) {  // no site codes: the spacing pass tells identity images by rank
       b.sq[dst] = b.uq[s];
-      b.scode[dst] = b.ucode[s];
       b.srank[dst] = b.urank[s];
     } else {
       b.perm[dst] = b.offsets[c + 1] - begin > kThreadCell ? s : b.urank[s];
@@ -1275,37 +1274,47 @@ __device__ inline bool visit_cell(uint32_t v, uint32_t sdim, uint32_t& cell) {
   return true;
 }
 
-// ucell (free once the scatter has run) takes each visit's count of identity images.
-__device__ __attribute__((always_inline)) inline void core_count_body(NearBuffers b, uint32_t block, uint32_t grid) {
+// ucell (free once the scatter has run) takes each visit's count of identity images: the
+// sites whose code has the core bit or, with two_order = 2|K| (unordered builds, which
+// keep no codes), whose rank src * 2|K| + image has image 0.
+__device__ __attribute__((always_inline)) inline void core_count_body(NearBuffers b, uint32_t two_order,
+                                                                      uint32_t block, uint32_t grid) {
   const uint32_t visits = tile_visits(b.sdim);
   for (uint32_t v = block * blockDim.x + threadIdx.x; v < visits; v += grid * blockDim.x) {
     uint32_t cell, count = 0;
     if (visit_cell(v, b.sdim, cell))
-      for (uint32_t k = b.offsets[cell]; k < b.offsets[cell + 1]; ++k) count += b.scode[k] >> 31;
+      for (uint32_t k = b.offsets[cell]; k < b.offsets[cell + 1]; ++k)
+        count += two_order ? uint32_t(b.srank[k] % two_order == 0) : b.scode[k] >> 31;
     b.ucell[v] = count;
   }
 }
-__global__ void k_core_count(NearBuffers b) { core_count_body(b, blockIdx.x, gridDim.x); }
+__global__ void k_core_count(NearBuffers b, uint32_t two_order) { core_count_body(b, two_order, blockIdx.x, gridDim.x); }
 
 // With ucell scanned: each visited cell writes its sources from its offset on (and, for
-// the spacing pass, their positions).
-__device__ __attribute__((always_inline)) inline void core_order_body(NearBuffers b, uint32_t positions, uint32_t block, uint32_t grid) {
+// the spacing pass, two_order != 0, their positions).
+__device__ __attribute__((always_inline)) inline void core_order_body(NearBuffers b, uint32_t two_order,
+                                                                      uint32_t block, uint32_t grid) {
   const uint32_t visits = tile_visits(b.sdim);
   for (uint32_t v = block * blockDim.x + threadIdx.x; v < visits; v += grid * blockDim.x) {
     uint32_t cell;
     if (!visit_cell(v, b.sdim, cell)) continue;
     uint32_t at = b.ucell[v];
     for (uint32_t k = b.offsets[cell]; k < b.offsets[cell + 1]; ++k) {
-      const uint32_t code = b.scode[k];
-      if (code >> 31) {
-        b.order[at] = code & 0x7fffffffu;
-        if (positions) b.opos[at] = b.sq[k];  // the identity image: the source's position, bit for bit
-        ++at;
+      if (two_order) {
+        const uint32_t rank = b.srank[k];
+        if (rank % two_order == 0) {
+          b.order[at] = rank / two_order;
+          b.opos[at] = b.sq[k];  // the identity image: the source's position, bit for bit
+          ++at;
+        }
+      } else {
+        const uint32_t code = b.scode[k];
+        if (code >> 31) b.order[at++] = code & 0x7fffffffu;
       }
     }
   }
 }
-__global__ void k_core_order(NearBuffers b, uint32_t positions) { core_order_body(b, positions, blockIdx.x, gridDim.x); }
+__global__ void k_core_order(NearBuffers b, uint32_t two_order) { core_order_body(b, two_order, blockIdx.x, gridDim.x); }
 
 static uint32_t stride_grid(size_t work) { return uint32_t(std::min<size_t>(8192, (work + kBlock - 1) / kBlock + 1)); }
 
@@ -1339,9 +1348,10 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
                        &b.info->huge, b.huge_capacity, b.ucell, b);
   });
   const uint32_t visits = tile_visits(b.sdim), gv = stride_grid(visits);
-  p.run("core_order", [&] { hipLaunchKernelGGL(k_core_count, dim3(gv), dim3(kBlock), 0, s, b); });
+  const uint32_t two_order = ordered ? 0u : 2 * order;
+  p.run("core_order", [&] { hipLaunchKernelGGL(k_core_count, dim3(gv), dim3(kBlock), 0, s, b, two_order); });
   launch_scan(p, s, b.ucell, visits, b.scan_tmp);
-  p.run("core_order", [&] { hipLaunchKernelGGL(k_core_order, dim3(gv), dim3(kBlock), 0, s, b, ordered ? 0u : 1u); });
+  p.run("core_order", [&] { hipLaunchKernelGGL(k_core_order, dim3(gv), dim3(kBlock), 0, s, b, two_order); });
   // The order only schedules the waves: each source's row is its own.
 }
 
@@ -1618,9 +1628,11 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 // ---- near field of many samples in one launch per stage (a batch step) ------------
 uint32_t near_force_key(const NearCut& cut) { return (cut.own ? 8u : 0u) + uint32_t(atan_terms(cut.chord2)); }
 
-// Workgroups per job and stage.  Grid-stride stages take 8 items per thread: a workgroup
-// first finds its job (a search over the stage's prefix sums), so each should do more
-// than one round of work.
+// Workgroups per job and stage.  The per-site and per-cell stages take 8 items per
+// thread: a workgroup first finds its job (a search over the stage's prefix sums), so
+// each should do more than one round of work (the site stages: 6.7 -> 4.6 ms per step of
+// the 256-job mix).  The visit stages keep one visit per thread: each walks its cell's
+// sites, and fewer threads leave too little parallelism (0.8 -> 1.3 ms per step).
 uint32_t near_stage_blocks(const NearJob& j, uint32_t stage) {
   const size_t cells = size_t(j.b.sdim) * j.b.sdim * j.b.sdim, visits = tile_visits(j.b.sdim);
   const uint32_t per = kBlock * 8;
@@ -1633,7 +1645,7 @@ uint32_t near_stage_blocks(const NearJob& j, uint32_t stage) {
     case kNsCellAdd: return std::min<uint32_t>(8192, blocks_for(cells, per));
     case kNsCrowded: return std::min<uint32_t>(kCrowdedBlocks, blocks_for(j.b.crowded_capacity, kBlock / 64));
     case kNsHuge: return std::min<uint32_t>(kHugeBlocks, j.b.huge_capacity);
-    case kNsVisits: return std::min<uint32_t>(8192, blocks_for(visits, per));
+    case kNsVisits: return stride_grid(visits);
     case kNsVisitTiles: return blocks_for(visits, kScanTile);
     case kNsVisitAdd: return std::min<uint32_t>(8192, blocks_for(visits, per));
     case kNsForce: return blocks_for(j.n, kBlock / 64);
@@ -1746,7 +1758,7 @@ __global__ __launch_bounds__(256) void k_jobs_sort_block(NearSet set) {
 __global__ void k_jobs_visits(NearSet set, uint32_t stage) {
   const JobPart w = near_part(set, kNsVisits, blockIdx.x);
   if (stage == 0)
-    core_count_body(w.job->b, w.block, w.grid);
+    core_count_body(w.job->b, 0u, w.block, w.grid);
   else
     core_order_body(w.job->b, 0u, w.block, w.grid);
 }
@@ -1834,7 +1846,9 @@ __global__ __launch_bounds__(256) void k_spacing(uint32_t n, const uint32_t* __r
                                                  uint32_t two_order, uint32_t real_blocks,
                                                  const float4* __restrict__ opos, uint64_t* __restrict__ key_out) {
   constexpr uint32_t G = kSpacingLanes;
-  const uint32_t vb = blockIdx.x;
+  // Each XCD takes a contiguous range of the tile-by-tile source order (as in k_near), so
+  // its L2 holds that range's sites.
+  const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
   if (vb >= real_blocks) return;
   const uint32_t sub = threadIdx.x % G;
   const uint32_t slot = vb * (kBlock / G) + threadIdx.x / G;
@@ -1888,22 +1902,36 @@ __global__ __launch_bounds__(256) void k_spacing(uint32_t n, const uint32_t* __r
       }
       const uint32_t total = uint32_t(__shfl(int(incl), int(G - 1), int(G)));
       const uint32_t first = start - (incl - len);  // start of the row minus the candidates before it
-      // Group-uniform trip count: every lane of the group takes part in the shuffles.
-      for (uint32_t base = 0; base < total; base += G) {
-        const uint32_t t = base + sub;
-        uint32_t k = 0;
-        for (uint32_t step = G / 2; step > 0; step >>= 1)
-          if (uint32_t(__shfl(int(incl), int(k + step - 1), int(G))) <= t) k += step;
-        const uint32_t site = uint32_t(__shfl(int(first), int(k), int(G))) + t;
-        if (t < total) {
-          const float4 y = sq[site];
-          const uint32_t rank = srank[site];
-          const float dm2 = chord2(x, y);
-          if (dm2 <= cut.chord2 && rank != self_rank) {
-            const uint64_t key = (uint64_t(__float_as_uint(dm2)) << 32) | rank;
-            best = key < best ? key : best;
-          }
+      // Group-uniform trip count: every lane of the group takes part in the shuffles.  Two
+      // candidates per lane per round, both loads issued before either is used.
+      auto test = [&](const float4 y, uint32_t rank) {
+        const float dm2 = chord2(x, y);
+        if (dm2 <= cut.chord2 && rank != self_rank) {
+          const uint64_t key = (uint64_t(__float_as_uint(dm2)) << 32) | rank;
+          best = key < best ? key : best;
         }
+      };
+      for (uint32_t base = 0; base < total; base += 2 * G) {
+        const uint32_t t0 = base + sub, t1 = t0 + G;
+        uint32_t k0 = 0, k1 = 0;
+        for (uint32_t step = G / 2; step > 0; step >>= 1) {
+          if (uint32_t(__shfl(int(incl), int(k0 + step - 1), int(G))) <= t0) k0 += step;
+          if (uint32_t(__shfl(int(incl), int(k1 + step - 1), int(G))) <= t1) k1 += step;
+        }
+        const uint32_t site0 = uint32_t(__shfl(int(first), int(k0), int(G))) + t0;
+        const uint32_t site1 = uint32_t(__shfl(int(first), int(k1), int(G))) + t1;
+        float4 y0 = x, y1 = x;
+        uint32_t r0 = self_rank, r1 = self_rank;
+        if (t0 < total) {
+          y0 = sq[site0];
+          r0 = srank[site0];
+        }
+        if (t1 < total) {
+          y1 = sq[site1];
+          r1 = srank[site1];
+        }
+        if (t0 < total) test(y0, r0);
+        if (t1 < total) test(y1, r1);
       }
     }
     for (uint32_t d = G / 2; d > 0; d >>= 1) {
@@ -1942,7 +1970,7 @@ void launch_near_nearest(Prof& p, hipStream_t s, uint32_t n, uint32_t order, con
   const uint32_t real = blocks_for(n, kBlock / kSpacingLanes);
   uint64_t* keys = reinterpret_cast<uint64_t*>(b.counts);  // 2n words: one key per source
   p.run("near_nearest", [&] {
-    hipLaunchKernelGGL(k_spacing, dim3(real), dim3(kBlock), 0, s, n, b.order, b.sq, b.srank, b.offsets, b.info,
+    hipLaunchKernelGGL(k_spacing, dim3(xcd_grid(real)), dim3(kBlock), 0, s, n, b.order, b.sq, b.srank, b.offsets, b.info,
                        b.sdim, cut, 2 * order, real, b.opos, keys);
   });
   launch_fill_u32(p, s, least_bits, 0xffffffffu, 1);
