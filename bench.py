@@ -237,13 +237,14 @@ def counters_from_profile(name):
 
 
 def issue_rates():
-    """The chip's measured VALU issue rates (zo_issue_probe): non-packed FP32 FMA and
-    transcendental wave64 instructions per second, each the best of three runs."""
+    """The chip's measured issue rates (zo_issue_probe): non-packed FP32 FMA,
+    transcendental and v_mfma_f32_16x16x4f32 wave64 instructions per second, each the best
+    of three runs."""
     import ctypes
     from zxtalori_amd.api import _library
     lib = _library()
     rates = {}
-    for kind, name in ((0, "fma"), (1, "trans")):
+    for kind, name in ((0, "fma"), (1, "trans"), (2, "mfma_f32_16x16x4")):
         best = 0.0
         for _ in range(3):
             value = ctypes.c_double(0.0)

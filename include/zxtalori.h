@@ -118,10 +118,11 @@ void zo_delaunay_mode(zxt_engine* e, uint32_t mode);
  * compiled-in tuned table. */
 void zo_tune_allpairs(uint32_t target, uint32_t ops_per_split);
 
-/* No counterpart in the reference: the chip's VALU issue rates, measured on device 0 with
- * long chains of one instruction kind on 8 waves per SIMD (kind 0 non-packed FP32 FMA, kind
- * 1 transcendentals v_rsq_f32/v_rcp_f32), in wave64 instructions per second; the peaks the
- * pair kernels' counters are read against.  0, or -1 on failure. */
+/* No counterpart in the reference: the chip's issue rates, measured on device 0 with long
+ * chains of one instruction kind on 8 waves per SIMD (kind 0 non-packed FP32 FMA, kind 1
+ * transcendentals v_rsq_f32/v_rcp_f32, kind 2 v_mfma_f32_16x16x4f32 FP32 Gram tiles), in
+ * wave64 instructions per second; the peaks the pair kernels' counters are read against.
+ * 0, or -1 on failure. */
 int32_t zo_issue_probe(uint32_t kind, uint32_t iters, double* rate_out);
 
 /* Additions of this build (no reference counterpart). */

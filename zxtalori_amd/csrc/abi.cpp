@@ -215,7 +215,7 @@ void zo_delaunay_mode(zxt_engine*, uint32_t) {}
 void zo_tune_allpairs(uint32_t target, uint32_t ops_per_split) { zxt::set_allpairs_override(target, ops_per_split); }
 
 int32_t zo_issue_probe(uint32_t kind, uint32_t iters, double* rate_out) {
-  if (kind > 1 || iters == 0 || !rate_out) return -1;
+  if (kind > 2 || iters == 0 || !rate_out) return -1;
   int sms = 0;
   if (hipDeviceGetAttribute(&sms, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || sms < 1) return -1;
   const uint32_t blocks = uint32_t(sms) * 8;  // 8 waves per SIMD

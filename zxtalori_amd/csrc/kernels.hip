@@ -2797,7 +2797,8 @@ void launch_bb_step(Prof& p, hipStream_t s, const JobSet& jobs) {
 // ---- issue-rate probe (zo_issue_probe) --------------------------------------------------
 // Long independent chains of one instruction kind per lane, on 8 waves per SIMD over the
 // whole chip, timed with events: the rate the VALU issues non-packed FP32 FMAs (8 chains)
-// or transcendentals (v_rsq_f32 and v_rcp_f32, 4 chains) in wave64 instructions per second.
+// or transcendentals (v_rsq_f32 and v_rcp_f32, 4 chains), or the matrix cores FP32 Gram
+// tiles (v_mfma_f32_16x16x4f32, 4 chains), in wave64 instructions per second.
 // The chains stay finite (an FMA contracting toward 1, reciprocals of values in [0.5, 2]).
 __global__ __launch_bounds__(256) void k_probe_fma(float* __restrict__ out, uint32_t iters) {
   const float m = 0.999f, c = 0.001f;
@@ -2819,14 +2820,32 @@ __global__ __launch_bounds__(256) void k_probe_trans(float* __restrict__ out, ui
   out[blockIdx.x * blockDim.x + threadIdx.x] = a0 + a1 + a2 + a3;
 }
 
+// FP32 Gram tiles: v_mfma_f32_16x16x4f32, 16 x 16 dot products of 4-vectors each (the
+// shape of a candidate prefilter's tile of 16 sources by 16 sites), 4 independent
+// accumulators per wave.
+__global__ __launch_bounds__(256) void k_probe_mfma(float* __restrict__ out, uint32_t iters) {
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  const float a = 1.0f + float(threadIdx.x) * 1e-3f, b = 0.5f - float(threadIdx.x) * 1e-4f;
+  f32x4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = c0, c2 = c0, c3 = c0;
+  for (uint32_t i = 0; i < iters; ++i) {
+    c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c0, 0, 0, 0);
+    c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(b, a, c1, 0, 0, 0);
+    c2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, a, c2, 0, 0, 0);
+    c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(b, b, c3, 0, 0, 0);
+  }
+  out[blockIdx.x * blockDim.x + threadIdx.x] = c0.x + c1.y + c2.z + c3.w;
+}
+
 double probe_issue_rate(hipStream_t s, uint32_t kind, uint32_t iters, float* scratch, uint32_t blocks) {
   hipEvent_t a = nullptr, b = nullptr;
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1.0;
   auto launch = [&](uint32_t it) {
     if (kind == 0)
       hipLaunchKernelGGL(k_probe_fma, dim3(blocks), dim3(kBlock), 0, s, scratch, it);
-    else
+    else if (kind == 1)
       hipLaunchKernelGGL(k_probe_trans, dim3(blocks), dim3(kBlock), 0, s, scratch, it);
+    else
+      hipLaunchKernelGGL(k_probe_mfma, dim3(blocks), dim3(kBlock), 0, s, scratch, it);
   };
   launch(64);  // warm-up
   (void)hipEventRecord(a, s);
@@ -2838,7 +2857,7 @@ double probe_issue_rate(hipStream_t s, uint32_t kind, uint32_t iters, float* scr
   (void)hipEventDestroy(b);
   if (!ok || ms <= 0.0f) return -1.0;
   const double waves = double(blocks) * kBlock / 64.0;
-  return waves * double(iters) * (kind == 0 ? 8.0 : 4.0) / (double(ms) * 1e-3);
+  return waves * double(iters) * (kind == 0 ? 8.0 : 4.0) / (double(ms) * 1e-3);  // 4 per round for 1 and 2
 }
 
 }  // namespace zxt
