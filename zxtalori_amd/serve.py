@@ -138,27 +138,28 @@ class Server:
     """Admits requests by HBM budget into batches and steps them until all are served.
 
     Requests go to a lane by the pair evaluations one of their steps costs: up to
-    `bounds[0]` the small lane, up to `bounds[1]` the medium lane (a near-field N of about
-    270,000 at neighbourhood 20), beyond it the large lane.  Each lane is a Batch stepped
-    by its own host thread and kept at most `ahead` steps ahead of the device, so the
-    lanes' kernels overlap on the GPU and a small job does not wait for a larger job's
-    steps.  Jobs within a lane advance together, one BB step per lane step, so a lane's
-    step lasts as long as its jobs' steps together: lanes of similar costs keep a job from
-    being slowed by much costlier ones.  A large job fills the device by itself, so the
-    large lane runs one relax job at a time, in arrival order: sharing steps among several
-    would only delay the first to finish.  Its single-pass requests (measure, spacing) join
-    beside the running relax job instead of waiting for all of its steps.
+    `bounds[0]` the small lane (by default 6e8: a near-field N of about 80,000 at
+    neighbourhood 20), up to each further bound a further lane, beyond the last the large
+    lane.  Each lane is a Batch stepped by its own host thread and kept at most `ahead`
+    steps ahead of the device, so the lanes' kernels overlap on the GPU and a small job does
+    not wait for a larger job's steps.  Jobs within a lane advance together, one BB step per
+    lane step, so a lane's step lasts as long as its jobs' steps together: lanes of similar
+    costs keep a job from being slowed by much costlier ones.  The large lane runs one relax
+    job at a time, in arrival order, as the per-sample path does: a large job nearly fills
+    the device by itself, and sharing it among several large jobs (a medium lane batching
+    jobs up to 2e9) made the slowest request slower than on the per-sample path.  Its
+    single-pass requests (measure, spacing) join beside the running relax job instead of
+    waiting for all of its steps.
     """
 
-    def __init__(self, hbm_budget=64 << 30, *, graphs=True, keep_results=False, bounds=(2e8, 2e9), ahead=1):
+    def __init__(self, hbm_budget=64 << 30, *, graphs=True, keep_results=False, bounds=(6e8,), ahead=1):
         self.hbm_budget = int(hbm_budget)
         self.keep_results = keep_results
         self.bounds = tuple(float(b) for b in bounds)
         self.ahead = int(ahead)
-        # The large lane's passes fill the device alone; the others spread their jobs'
-        # near-field passes over three extra streams (the 256-job mix: 9.4 s against 9.6 s
-        # with two).
-        self.lanes = [_Lane(f"lane{k}", graphs, 3) for k in range(len(self.bounds))]
+        # The large lane runs one job; the others run their near-field launches on an extra
+        # stream beside their all-pairs launch.
+        self.lanes = [_Lane(f"lane{k}", graphs, 1) for k in range(len(self.bounds))]
         self.lanes.append(_Lane("large", graphs, 0, limit=1))
         self.peak_bytes = 0
         self.max_live = 0
