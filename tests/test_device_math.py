@@ -158,8 +158,12 @@ def test_duplicate_pair_stays_finite():
 
 # ---- the near-field form of the pair (near_pair_terms, near_force_of) ---------------
 def series_coefficients():
-    body = SRC[SRC.index("__device__ __forceinline__ float atan_small"):SRC.index("struct NearSum")]
-    return [float(x) for x in re.findall(r"(-?\d+\.\d+)f", body[body.index("c[6]"):])][:6]
+    """The plain series coefficients c_k of atan2x_small, whose table holds 2 * c_k."""
+    body = SRC[SRC.index("__device__ __forceinline__ float atan2x_small"):SRC.index("struct NearSum")]
+    table = body[body.index("c[6]"):body.index("};")]
+    found = re.findall(r"2\.0f \* (-?\d+\.\d+)f", table)
+    assert len(found) == 6, table
+    return [float(x) for x in found]
 
 
 def atan_series(t, terms):
@@ -185,6 +189,21 @@ def test_atan_series_within_its_cutoff():
         # one term fewer would not do over the same range
         if terms > 2:
             assert (np.abs(atan_series(t, terms - 1) - np.arctan(t)) / np.arctan(t)).max() > 5e-9
+
+
+def test_doubled_series_is_twice_the_series_bit_for_bit():
+    """Scaling every coefficient by 2 scales every rounded Horner step and the final product
+    by 2 exactly (no overflow or underflow at these magnitudes), so the kernel's 2 atan(t)
+    is bit for bit twice the plain series."""
+    c = np.array(series_coefficients(), np.float32)
+    t = np.linspace(1e-6, 0.6, 200001, dtype=np.float32)
+    z = t * t
+    for terms in range(2, 6):
+        p, q = c[terms], np.float32(2) * c[terms]
+        for k in range(terms - 1, -1, -1):
+            p = (z * p + c[k]).astype(np.float32)
+            q = (z * q + np.float32(2) * c[k]).astype(np.float32)
+        np.testing.assert_array_equal((t * q).astype(np.float32), np.float32(2) * (t * p).astype(np.float32))
 
 
 def direction_M(x, e):
@@ -230,13 +249,14 @@ def kernel_near_pair(x, y, taper_start, inv_width, terms):
     sine = 0.5 * t * fr + 1e-9
     P1, P2, P3 = a2 * b2 * sine, a3 * b3, a * b
     D = 1 / (P1 * P2 * P3)
-    iP1, iP2, iP3 = D * P2 * P3, D * P1 * P3, D * P1 * P2
-    energy = math.pi * iP3
+    iP1, iP2, iP3 = D * P2 * P3, D * P1 * P3, D * P1 * P2  # the pair energy is pi iP3
     u = min(max((a - taper_start) * inv_width, 0.0), 1.0)
-    w = (u * u) * (2 * u - 3) + 1
-    slope = -6 * inv_width * (u * (1 - u))
-    coef = w * (a2 - b2) * iP1 + energy * slope * (2 * R)
-    return direction_M(x, coef * e), (a3 + b3) * iP2 * w, energy * w
+    uu = u * u
+    w = uu * (2 * u - 3) + 1
+    slope_pi = -12 * math.pi * inv_width  # per source: pi times twice the slope's factor
+    coef = w * (a2 - b2) * iP1 + iP3 * (slope_pi * (u - uu)) * R
+    # the kernel sums iP3 w and multiplies the source's sum by pi once
+    return direction_M(x, coef * e), (a3 + b3) * iP2 * w, math.pi * (iP3 * w)
 
 
 def test_near_pair_form_is_the_reference_pair():

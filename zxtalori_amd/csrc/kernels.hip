@@ -1383,7 +1383,7 @@ __device__ inline float row_gap(float c, int idx, uint32_t dim) {
 // image is ever a candidate: then the candidate stage needs no site codes and no test at
 // all beyond the cutoff, the source's own identity image (chord exactly 0) being dropped
 // when its pair is evaluated, and the nearest chord is taken over the compacted pairs.
-// TERMS: the arctangent series (atan_small).
+// TERMS: the arctangent series (atan2x_small).
 // vb: the workgroup's index in the source order (4 sources each).
 template <bool OWN, int TERMS>
 __device__ __attribute__((always_inline)) inline void near_body(
@@ -1439,6 +1439,7 @@ __device__ __attribute__((always_inline)) inline void near_body(
   // and the count is read from lane 0 after it.
   uint32_t zeros = 0;
   const float u_offset = -cut.taper_start * cut.inv_width;
+  const float slope_pi = -12.0f * kPiF * cut.inv_width;
   uint32_t qhead = 0, qcount = 0, rounds = 0;
   uint32_t candidates = 0;  // sites tested (uniform); accepted pairs are 64 rounds + the tail
 
@@ -1452,7 +1453,7 @@ __device__ __attribute__((always_inline)) inline void near_body(
       // 2 exceeds every accepted chord (< 2), so it stands for "no pair" in the minimum.
       nearest_bits = min(nearest_bits, __float_as_uint(dm2));
     }
-    near_pair_terms<TERMS>(d, dm2, cut.inv_width, u_offset, ps);
+    near_pair_terms<TERMS>(d, dm2, cut.inv_width, u_offset, slope_pi, ps);
   };
   // Every 64 rounds the lanes' FP32 sums (64 pairs each) go into the wave's FP64 sums.
   auto flush = [&]() {
@@ -1536,7 +1537,7 @@ __device__ __attribute__((always_inline)) inline void near_body(
       // Each duplicate of x is a pair at chord 0 (pairs.wgsl:404-408 keeps it): its
       // terms, clamped by the pair's regularisers, with no force (x - y = 0).
       NearSum z{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-      near_pair_terms<TERMS>(make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, cut.inv_width, u_offset, z);
+      near_pair_terms<TERMS>(make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, cut.inv_width, u_offset, slope_pi, z);
       acc[4] += double(zeros - 1) * double(z.curvature);
       acc[5] += double(zeros - 1) * double(z.energy);
       nearest_key = 0u;
@@ -1545,7 +1546,7 @@ __device__ __attribute__((always_inline)) inline void near_body(
     pairs -= min(zeros, 1u);  // the identity image is no pair
   }
   const double g[4] = {acc[0], acc[1], acc[2], acc[3]};
-  const double curvature = acc[4], e = acc[5];
+  const double curvature = acc[4], e = acc[5] * kPi64;  // the pairs' energies were summed without pi
   const float nearest = __uint_as_float(nearest_key);
   if (lane == 0) {
     double f[3];

@@ -23,6 +23,7 @@
 namespace zxt {
 
 constexpr float kPiF = 3.14159265358979323846f;
+constexpr double kPi64 = 3.14159265358979323846;
 
 __host__ __device__ inline float4 qmul(const float4 a, const float4 b) {
   float4 r;
@@ -125,16 +126,20 @@ __device__ __forceinline__ void pair_terms(const float4 x, const float4 y, float
   s.energy += energy;
 }
 
-// atan(t) for small t: t times the Taylor series 1 - z/3 + z^2/5 - ... in z = t^2 up to
-// z^TERMS.  Within a near-field cutoff z <= tan(r/2)^2, and the launcher picks the fewest
-// terms whose truncation, z^(TERMS+1)/(2 TERMS + 3), stays below 5e-9 there (atan_terms);
-// TERMS = 0 selects the minimax polynomial on [0, 1].
+// 2 atan(t) for small t: t times twice the Taylor series 1 - z/3 + z^2/5 - ... in z = t^2
+// up to z^TERMS.  Within a near-field cutoff z <= tan(r/2)^2, and the launcher picks the
+// fewest terms whose truncation, z^(TERMS+1)/(2 TERMS + 3), stays below 5e-9 there
+// (atan_terms); TERMS = 0 selects the minimax polynomial on [0, 1].  The factor 2 rides on
+// the coefficients: scaling by 2 commutes with every rounding of the Horner steps and of
+// the last product, so the result is bit for bit 2 * (t * p) of the plain series, one
+// multiplication fewer per pair.
 template <int TERMS>
-__device__ __forceinline__ float atan_small(float t) {
-  if (TERMS == 0) return atan01(t);
-  // Horner from the highest kept term: coefficients (-1)^k / (2k + 1), k = TERMS..1.
+__device__ __forceinline__ float atan2x_small(float t) {
+  if (TERMS == 0) return 2.0f * atan01(t);
+  // Horner from the highest kept term: coefficients 2 (-1)^k / (2k + 1), k = TERMS..1.
   const float z = t * t;
-  constexpr float c[6] = {1.0f, -0.33333334f, 0.2f, -0.14285715f, 0.11111111f, -0.09090909f};
+  constexpr float c[6] = {2.0f * 1.0f,         2.0f * -0.33333334f, 2.0f * 0.2f,
+                          2.0f * -0.14285715f, 2.0f * 0.11111111f,  2.0f * -0.09090909f};
   float p = c[TERMS];
 #pragma unroll
   for (int k = TERMS - 1; k >= 0; --k) p = fmaf(z, p, c[k]);
@@ -152,18 +157,22 @@ struct NearSum {
 // The near-field pair for a site y within the cutoff, from e = x - y and dm2 = |e|^2:
 // dm2 <= chord2 < 2, so y is the nearer of +-y (sign +) and dp2 = 4 - dm2 for unit
 // quaternions; then near + far = 4 and 1/sin a = 2 rsq(near far).  Otherwise the
-// arithmetic of pair_terms<true> (pairs.wgsl:246-262, :349-361), with two changes of
-// form: the window coordinate (a - taper_start) / width is one fused multiply-add,
+// arithmetic of pair_terms<true> (pairs.wgsl:246-262, :349-361), with changes of form:
+// the window coordinate (a - taper_start) / width is one fused multiply-add,
 // a * inv_width + u_offset with u_offset = -taper_start * inv_width (one rounding instead
-// of two), and the factor 2 of the slope term rides on its constant (exact).
+// of two); the pair energy pi / (a b) is summed without its factor pi, which the caller
+// applies once per source in FP64, and rides with the window's slope factor -12 on the
+// per-source constant slope_pi = -12 pi inv_width (twice the slope, times pi); and the
+// slope's u (1 - u) is u - u^2 from the u^2 the window already has.  Each saves one
+// instruction per pair; the rounding differs from the reference's form in the last bit.
 template <int TERMS>
 __device__ __forceinline__ void near_pair_terms(const float4 e, float dm2, float inv_width, float u_offset,
-                                                NearSum& s) {
+                                                float slope_pi, NearSum& s) {
   const float nr = __builtin_amdgcn_fmed3f(dm2, 1e-30f, 4.0f);
   const float fr = 4.0f - dm2;
   const float R = __builtin_amdgcn_rsqf(nr * fr);
   const float t = nr * R;
-  const float a = fmaxf(2.0f * atan_small<TERMS>(t), 1e-12f);
+  const float a = fmaxf(atan2x_small<TERMS>(t), 1e-12f);
   const float b = kPiF - a;
   const float aa = a * a, bb = b * b;
   // Within a cutoff of at most 1.5 rad, b >= pi - 1.5, so b^2 >= 2.6 and b^2 + 1e-9 rounds
@@ -174,18 +183,18 @@ __device__ __forceinline__ void near_pair_terms(const float4 e, float dm2, float
   const float P1 = a2 * b2 * sine, P2 = a3 * b3, P3 = a * b;
   const float P12 = P1 * P2;
   const float D = __builtin_amdgcn_rcpf(P12 * P3);
-  const float iP1 = D * (P2 * P3), iP2 = D * (P1 * P3), iP3 = D * P12;
-  const float energy = kPiF * iP3;
+  const float iP1 = D * (P2 * P3), iP2 = D * (P1 * P3), iP3 = D * P12;  // energy / pi = iP3
   const float u = fminf(fmaxf(fmaf(a, inv_width, u_offset), 0.0f), 1.0f);
-  const float w = fmaf(u * u, fmaf(2.0f, u, -3.0f), 1.0f);
-  const float slope2 = -12.0f * inv_width * (u * (1.0f - u));  // twice the window's slope
-  const float coef = fmaf(w, (a2 - b2) * iP1, energy * slope2 * R);
+  const float uu = u * u;
+  const float w = fmaf(uu, fmaf(2.0f, u, -3.0f), 1.0f);
+  const float slope2pi = slope_pi * (u - uu);  // pi times twice the window's slope
+  const float coef = fmaf(w, (a2 - b2) * iP1, iP3 * slope2pi * R);
   s.gx = fmaf(coef, e.x, s.gx);
   s.gy = fmaf(coef, e.y, s.gy);
   s.gz = fmaf(coef, e.z, s.gz);
   s.gw = fmaf(coef, e.w, s.gw);
   s.curvature = fmaf((a3 + b3) * iP2, w, s.curvature);
-  s.energy = fmaf(energy, w, s.energy);
+  s.energy = fmaf(iP3, w, s.energy);  // times pi once per source
 }
 
 // The force of a source x from the summed g = sum coef (x - y): -M(x) g, in FP64.
@@ -196,7 +205,7 @@ __host__ __device__ inline void near_force_of(const float4 x, const double g[4],
   f[2] = x3 * g[0] + x2 * g[1] - x1 * g[2] - x0 * g[3];
 }
 
-// Terms of atan_small for a cutoff chord^2: the fewest with truncation error below 5e-9
+// Terms of atan2x_small for a cutoff chord^2: the fewest with truncation error below 5e-9
 // for z up to chord2 / (4 - chord2) = tan(r/2)^2 (0: the minimax polynomial).
 inline int atan_terms(float chord2) {
   const double z = double(chord2) / (4.0 - double(chord2)) * 1.0001;
