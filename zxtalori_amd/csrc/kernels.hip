@@ -720,8 +720,8 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t& 
   return r;
 }
 
-__device__ __attribute__((always_inline)) inline void scan_tiles_body(uint32_t* __restrict__ data, size_t count,
-                                                    uint32_t* __restrict__ tile_sums, uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void scan_tiles_body(
+    uint32_t* __restrict__ data, size_t count, uint32_t* __restrict__ tile_sums, uint32_t block, uint32_t grid) {
   __shared__ uint32_t lds[kBlock];
   const size_t base = size_t(block) * kScanTile + size_t(threadIdx.x) * kScanItems;
   uint32_t v[kScanItems];
@@ -740,11 +740,14 @@ __device__ __attribute__((always_inline)) inline void scan_tiles_body(uint32_t* 
     if (base + k < count) data[base + k] = v[k] + off;
   if (threadIdx.x == 0) tile_sums[block] = total;
 }
-__global__ __launch_bounds__(256) void k_scan_tiles(uint32_t* __restrict__ data, size_t count,
-                                                    uint32_t* __restrict__ tile_sums) { scan_tiles_body(data, count, tile_sums, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_scan_tiles(
+    uint32_t* __restrict__ data, size_t count, uint32_t* __restrict__ tile_sums) {
+  scan_tiles_body(data, count, tile_sums, blockIdx.x, gridDim.x);
+}
 
-__device__ __attribute__((always_inline)) inline void scan_sums_body(uint32_t* __restrict__ tile_sums, uint32_t ntiles,
-                                                   uint32_t* __restrict__ total_out, uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void scan_sums_body(
+    uint32_t* __restrict__ tile_sums, uint32_t ntiles, uint32_t* __restrict__ total_out, uint32_t block,
+    uint32_t grid) {
   __shared__ uint32_t lds[kBlock];
   const uint32_t per = (ntiles + kBlock - 1) / kBlock;
   const uint32_t b0 = threadIdx.x * per, b1 = min(ntiles, b0 + per);
@@ -759,14 +762,20 @@ __device__ __attribute__((always_inline)) inline void scan_sums_body(uint32_t* _
   }
   if (threadIdx.x == 0) *total_out = total;
 }
-__global__ __launch_bounds__(256) void k_scan_sums(uint32_t* __restrict__ tile_sums, uint32_t ntiles,
-                                                   uint32_t* __restrict__ total_out) { scan_sums_body(tile_sums, ntiles, total_out, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_scan_sums(
+    uint32_t* __restrict__ tile_sums, uint32_t ntiles, uint32_t* __restrict__ total_out) {
+  scan_sums_body(tile_sums, ntiles, total_out, blockIdx.x, gridDim.x);
+}
 
-__device__ __attribute__((always_inline)) inline void scan_add_body(uint32_t* __restrict__ data, size_t count, const uint32_t* __restrict__ tile_sums, uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void scan_add_body(
+    uint32_t* __restrict__ data, size_t count, const uint32_t* __restrict__ tile_sums, uint32_t block,
+    uint32_t grid) {
   for (size_t i = size_t(block) * blockDim.x + threadIdx.x; i < count; i += size_t(grid) * blockDim.x)
     data[i] += tile_sums[i / kScanTile];
 }
-__global__ void k_scan_add(uint32_t* __restrict__ data, size_t count, const uint32_t* __restrict__ tile_sums) { scan_add_body(data, count, tile_sums, blockIdx.x, gridDim.x); }
+__global__ void k_scan_add(uint32_t* __restrict__ data, size_t count, const uint32_t* __restrict__ tile_sums) {
+  scan_add_body(data, count, tile_sums, blockIdx.x, gridDim.x);
+}
 
 // Exclusive scan of data[0..count) in place, data[count] = total.  tmp: scan_tmp_words(count).
 static void launch_scan(Prof& p, hipStream_t s, uint32_t* data, size_t count, uint32_t* tmp) {
@@ -792,7 +801,9 @@ __device__ inline void near_reset_body(NearInfo* info, uint32_t n) {
     info->hi[c] = 0u;
   }
 }
-__global__ void k_near_reset(NearInfo* info, uint32_t n) { near_reset_body(info, n); }
+__global__ void k_near_reset(NearInfo* info, uint32_t n) {
+  near_reset_body(info, n);
+}
 
 __device__ inline void box_reduce_commit(NearInfo* info, uint32_t lo[3], uint32_t hi[3]) {
   __shared__ uint32_t slo[3][kBlock / 64], shi[3][kBlock / 64];
@@ -852,9 +863,9 @@ __device__ inline void block_min_commit(uint32_t* dst, uint32_t v) {
   }
 }
 
-__device__ __attribute__((always_inline)) inline void core_rim_body(const float4* __restrict__ q, uint32_t n,
-                                                  const float4* __restrict__ planes, uint32_t nplanes, float band,
-                                                  uint32_t two_order, NearBuffers b, uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void core_rim_body(
+    const float4* __restrict__ q, uint32_t n, const float4* __restrict__ planes, uint32_t nplanes, float band,
+    uint32_t two_order, NearBuffers b, uint32_t block, uint32_t grid) {
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   uint32_t wmin = 0xffffffffu;
   const uint32_t stride = grid * blockDim.x;
@@ -901,9 +912,11 @@ __device__ __attribute__((always_inline)) inline void core_rim_body(const float4
   box_reduce_commit(b.info, lo, hi);
   block_min_commit(&b.info->wmin_key, wmin);
 }
-__global__ __launch_bounds__(256) void k_core_rim(const float4* __restrict__ q, uint32_t n,
-                                                  const float4* __restrict__ planes, uint32_t nplanes, float band,
-                                                  uint32_t two_order, NearBuffers b) { core_rim_body(q, n, planes, nplanes, band, two_order, b, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_core_rim(
+    const float4* __restrict__ q, uint32_t n, const float4* __restrict__ planes, uint32_t nplanes, float band,
+    uint32_t two_order, NearBuffers b) {
+  core_rim_body(q, n, planes, nplanes, band, two_order, b, blockIdx.x, gridDim.x);
+}
 
 // Workgroups of k_core_rim: at least 512 (measured at I, N = 1e6: 0.06 ms against 0.17 ms
 // with 2048), and enough that no thread takes more than 32 sources (its rim flags fit one
@@ -943,10 +956,9 @@ __device__ inline bool shell_image(const float4 x, uint32_t img, const float4* _
 // wave reserves room for all of its lanes' sites with one atomic, then each thread writes
 // its own.  The append order varies; the cell sort's in-cell key order makes the result
 // deterministic.
-__device__ __attribute__((always_inline)) inline void shell_body(const float4* __restrict__ q, uint32_t n,
-                                               const float4* __restrict__ ops, uint32_t order,
-                                               const float4* __restrict__ planes, uint32_t nplanes, float band,
-                                               NearBuffers b, uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void shell_body(
+    const float4* __restrict__ q, uint32_t n, const float4* __restrict__ ops, uint32_t order,
+    const float4* __restrict__ planes, uint32_t nplanes, float band, NearBuffers b, uint32_t block, uint32_t grid) {
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   const uint32_t m = b.info->rim;
   const uint32_t per = 2 * order - 1;
@@ -1004,10 +1016,11 @@ __device__ __attribute__((always_inline)) inline void shell_body(const float4* _
   }
   box_reduce_commit(b.info, lo, hi);
 }
-__global__ __launch_bounds__(256) void k_shell(const float4* __restrict__ q, uint32_t n,
-                                               const float4* __restrict__ ops, uint32_t order,
-                                               const float4* __restrict__ planes, uint32_t nplanes, float band,
-                                               NearBuffers b) { shell_body(q, n, ops, order, planes, nplanes, band, b, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_shell(
+    const float4* __restrict__ q, uint32_t n, const float4* __restrict__ ops, uint32_t order,
+    const float4* __restrict__ planes, uint32_t nplanes, float band, NearBuffers b) {
+  shell_body(q, n, ops, order, planes, nplanes, band, b, blockIdx.x, gridDim.x);
+}
 
 __device__ inline void box_finalize_body(NearInfo* info, uint32_t sdim) {
   float lo[3], hi[3];
@@ -1021,7 +1034,9 @@ __device__ inline void box_finalize_body(NearInfo* info, uint32_t sdim) {
   for (int c = 0; c < 3; ++c) info->lower[c] = lo[c] - 1e-5f * width;
   info->cpu = float(sdim) / (width * 1.0001f);
 }
-__global__ void k_box_finalize(NearInfo* info, uint32_t sdim) { box_finalize_body(info, sdim); }
+__global__ void k_box_finalize(NearInfo* info, uint32_t sdim) {
+  box_finalize_body(info, sdim);
+}
 
 __device__ inline uint32_t cell_axis(float v, float lower, float cpu, uint32_t dim) {
   const float c = floorf((v - lower) * cpu);
@@ -1040,7 +1055,9 @@ __device__ __attribute__((always_inline)) inline void site_count_body(NearBuffer
     b.uslot[s] = atomicAdd(&b.offsets[c], 1u);
   }
 }
-__global__ void k_site_count(NearBuffers b) { site_count_body(b, blockIdx.x, gridDim.x); }
+__global__ void k_site_count(NearBuffers b) {
+  site_count_body(b, blockIdx.x, gridDim.x);
+}
 
 // Cells of up to kThreadCell sites are put in rank order by counting (k_site_rank_place);
 // larger ones by sorting an index list (k_cell_sort_waves, k_cell_sort_block).
@@ -1050,7 +1067,8 @@ constexpr uint32_t kThreadCell = 16;
 // Unordered (spacing): the site itself (position and rank) goes there.  Ordered: a cell of at most kThreadCell
 // sites gets the site's rank there (k_site_rank_place then moves each site to its place
 // in rank order); a larger one the site's index (perm), for the crowded-cell sorts.
-__device__ __attribute__((always_inline)) inline void site_place_body(NearBuffers b, uint32_t ordered, uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void site_place_body(
+    NearBuffers b, uint32_t ordered, uint32_t block, uint32_t grid) {
   const uint32_t S = min(b.info->sites, b.capacity);
   for (uint32_t s = block * blockDim.x + threadIdx.x; s < S; s += grid * blockDim.x) {
     const uint32_t c = b.ucell[s];
@@ -1063,7 +1081,9 @@ __device__ __attribute__((always_inline)) inline void site_place_body(NearBuffer
     }
   }
 }
-__global__ void k_site_place(NearBuffers b, uint32_t ordered) { site_place_body(b, ordered, blockIdx.x, gridDim.x); }
+__global__ void k_site_place(NearBuffers b, uint32_t ordered) {
+  site_place_body(b, ordered, blockIdx.x, gridDim.x);
+}
 
 // Moves unsorted site s to sorted position k.
 __device__ inline void move_site(const NearBuffers& b, uint32_t k, uint32_t s) {
@@ -1077,7 +1097,8 @@ __device__ inline void move_site(const NearBuffers& b, uint32_t k, uint32_t s) {
 // of its cell's ranks below its own (read from the ranks k_site_place laid out; the loads
 // are independent, no sorting chain); larger cells are listed for k_cell_sort_waves (by
 // the site that arrived first).
-__device__ __attribute__((always_inline)) inline void site_rank_place_body(NearBuffers b, uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void site_rank_place_body(
+    NearBuffers b, uint32_t block, uint32_t grid) {
   const uint32_t S = min(b.info->sites, b.capacity);
   for (uint32_t s = block * blockDim.x + threadIdx.x; s < S; s += grid * blockDim.x) {
     const uint32_t c = b.ucell[s];
@@ -1092,7 +1113,9 @@ __device__ __attribute__((always_inline)) inline void site_rank_place_body(NearB
     move_site(b, begin + below, s);
   }
 }
-__global__ void k_site_rank_place(NearBuffers b) { site_rank_place_body(b, blockIdx.x, gridDim.x); }
+__global__ void k_site_rank_place(NearBuffers b) {
+  site_rank_place_body(b, blockIdx.x, gridDim.x);
+}
 
 // Crowded cells (more than kThreadCell items: coarse cells, or a wide cutoff over a small
 // sample crowding the chart's centre), one wave each: up to 64 items placed by rank (the
@@ -1104,12 +1127,11 @@ struct SortLds {
   uint32_t k[kWaveChunk];
   uint32_t v[kWaveChunk];
 };
-__device__ __attribute__((always_inline)) inline void cell_sort_waves_body(const uint32_t* __restrict__ offsets,
-                                                         uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
-                                                         const uint32_t* __restrict__ crowded_list,
-                                                         const uint32_t* __restrict__ crowded, uint32_t list_capacity,
-                                                         uint32_t* __restrict__ huge_list, uint32_t* __restrict__ huge,
-                                                         NearBuffers sites, uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void cell_sort_waves_body(
+    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
+    const uint32_t* __restrict__ crowded_list, const uint32_t* __restrict__ crowded, uint32_t list_capacity,
+    uint32_t* __restrict__ huge_list, uint32_t* __restrict__ huge, NearBuffers sites, uint32_t block,
+    uint32_t grid) {
   __shared__ SortLds lds[kBlock / 64];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   SortLds& L = lds[wave];
@@ -1170,23 +1192,23 @@ __device__ __attribute__((always_inline)) inline void cell_sort_waves_body(const
     wave_sync();
   }
 }
-__global__ __launch_bounds__(256) void k_cell_sort_waves(const uint32_t* __restrict__ offsets,
-                                                         uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
-                                                         const uint32_t* __restrict__ crowded_list,
-                                                         const uint32_t* __restrict__ crowded, uint32_t list_capacity,
-                                                         uint32_t* __restrict__ huge_list, uint32_t* __restrict__ huge,
-                                                         NearBuffers sites) { cell_sort_waves_body(offsets, perm, key, crowded_list, crowded, list_capacity, huge_list, huge, sites, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_cell_sort_waves(
+    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
+    const uint32_t* __restrict__ crowded_list, const uint32_t* __restrict__ crowded, uint32_t list_capacity,
+    uint32_t* __restrict__ huge_list, uint32_t* __restrict__ huge, NearBuffers sites) {
+  cell_sort_waves_body(offsets, perm, key, crowded_list, crowded, list_capacity, huge_list, huge, sites, blockIdx.x,
+    gridDim.x);
+}
 
 // Cells of more than kWaveChunk items, one workgroup each: chunks of up to kBlockChunk
 // sorted in LDS (bitonic), then, for cells of more than one chunk, every item placed at
 // its rank (its index in its chunk plus, per other chunk, the count of smaller keys found
 // by binary search) through tmp.
 constexpr uint32_t kBlockChunk = 8192;
-__device__ __attribute__((always_inline)) inline void cell_sort_block_body(const uint32_t* __restrict__ offsets,
-                                                         uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
-                                                         const uint32_t* __restrict__ huge_list,
-                                                         const uint32_t* __restrict__ huge, uint32_t list_capacity,
-                                                         uint32_t* __restrict__ tmp, NearBuffers sites, uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void cell_sort_block_body(
+    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
+    const uint32_t* __restrict__ huge_list, const uint32_t* __restrict__ huge, uint32_t list_capacity,
+    uint32_t* __restrict__ tmp, NearBuffers sites, uint32_t block, uint32_t grid) {
   __shared__ uint32_t sk[kBlockChunk], sv[kBlockChunk];
   const uint32_t total = min(*huge, list_capacity);
   for (uint32_t b = block; b < total; b += grid) {
@@ -1253,11 +1275,12 @@ __device__ __attribute__((always_inline)) inline void cell_sort_block_body(const
     __syncthreads();
   }
 }
-__global__ __launch_bounds__(256) void k_cell_sort_block(const uint32_t* __restrict__ offsets,
-                                                         uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
-                                                         const uint32_t* __restrict__ huge_list,
-                                                         const uint32_t* __restrict__ huge, uint32_t list_capacity,
-                                                         uint32_t* __restrict__ tmp, NearBuffers sites) { cell_sort_block_body(offsets, perm, key, huge_list, huge, list_capacity, tmp, sites, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_cell_sort_block(
+    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ perm, const uint32_t* __restrict__ key,
+    const uint32_t* __restrict__ huge_list, const uint32_t* __restrict__ huge, uint32_t list_capacity,
+    uint32_t* __restrict__ tmp, NearBuffers sites) {
+  cell_sort_block_body(offsets, perm, key, huge_list, huge, list_capacity, tmp, sites, blockIdx.x, gridDim.x);
+}
 
 // The sources are visited tile by tile (8x8x8 cells, tiles in row-major order, cells
 // row-major within a tile), each cell's identity images in their sorted order: waves of
@@ -1277,8 +1300,8 @@ __device__ inline bool visit_cell(uint32_t v, uint32_t sdim, uint32_t& cell) {
 // ucell (free once the scatter has run) takes each visit's count of identity images: the
 // sites whose code has the core bit or, with two_order = 2|K| (unordered builds, which
 // keep no codes), whose rank src * 2|K| + image has image 0.
-__device__ __attribute__((always_inline)) inline void core_count_body(NearBuffers b, uint32_t two_order,
-                                                                      uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void core_count_body(
+    NearBuffers b, uint32_t two_order, uint32_t block, uint32_t grid) {
   const uint32_t visits = tile_visits(b.sdim);
   for (uint32_t v = block * blockDim.x + threadIdx.x; v < visits; v += grid * blockDim.x) {
     uint32_t cell, count = 0;
@@ -1288,12 +1311,14 @@ __device__ __attribute__((always_inline)) inline void core_count_body(NearBuffer
     b.ucell[v] = count;
   }
 }
-__global__ void k_core_count(NearBuffers b, uint32_t two_order) { core_count_body(b, two_order, blockIdx.x, gridDim.x); }
+__global__ void k_core_count(NearBuffers b, uint32_t two_order) {
+  core_count_body(b, two_order, blockIdx.x, gridDim.x);
+}
 
 // With ucell scanned: each visited cell writes its sources from its offset on (and, for
 // the spacing pass, two_order != 0, their positions).
-__device__ __attribute__((always_inline)) inline void core_order_body(NearBuffers b, uint32_t two_order,
-                                                                      uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void core_order_body(
+    NearBuffers b, uint32_t two_order, uint32_t block, uint32_t grid) {
   const uint32_t visits = tile_visits(b.sdim);
   for (uint32_t v = block * blockDim.x + threadIdx.x; v < visits; v += grid * blockDim.x) {
     uint32_t cell;
@@ -1314,7 +1339,9 @@ __device__ __attribute__((always_inline)) inline void core_order_body(NearBuffer
     }
   }
 }
-__global__ void k_core_order(NearBuffers b, uint32_t two_order) { core_order_body(b, two_order, blockIdx.x, gridDim.x); }
+__global__ void k_core_order(NearBuffers b, uint32_t two_order) {
+  core_order_body(b, two_order, blockIdx.x, gridDim.x);
+}
 
 static uint32_t stride_grid(size_t work) { return uint32_t(std::min<size_t>(8192, (work + kBlock - 1) / kBlock + 1)); }
 
@@ -1388,9 +1415,9 @@ __device__ inline float row_gap(float c, int idx, uint32_t dim) {
 template <bool OWN, int TERMS>
 __device__ __attribute__((always_inline)) inline void near_body(
     const float4* __restrict__ q_in, uint32_t n, const uint32_t* __restrict__ order_in,
-    const float4* __restrict__ sq_in, const uint32_t* __restrict__ scode_in, const uint32_t* __restrict__ offsets_in,
-    const NearInfo* __restrict__ info_in, uint32_t sdim, NearCut cut, Rows rows, uint32_t* __restrict__ counts,
-    uint32_t vb) {
+    const float4* __restrict__ sq_in, const uint32_t* __restrict__ scode_in,
+    const uint32_t* __restrict__ offsets_in, const NearInfo* __restrict__ info_in, uint32_t sdim, NearCut cut,
+    Rows rows, uint32_t* __restrict__ counts, uint32_t vb) {
   const auto* __restrict__ q = to_global(q_in);
   const auto* __restrict__ order = to_global(order_in);
   const auto* __restrict__ sq = to_global(sq_in);
@@ -1575,8 +1602,9 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
 }
 
 // Sums the per-source pair and candidate counts of a force pass into counters[0..1].
-__device__ __attribute__((always_inline)) inline void count_reduce_body(const uint32_t* __restrict__ counts, uint32_t n,
-                                                      unsigned long long* __restrict__ counters, uint32_t block, uint32_t grid) {
+__device__ __attribute__((always_inline)) inline void count_reduce_body(
+    const uint32_t* __restrict__ counts, uint32_t n, unsigned long long* __restrict__ counters, uint32_t block,
+    uint32_t grid) {
   unsigned long long a = 0, b = 0;
   for (uint32_t i = block * blockDim.x + threadIdx.x; i < n; i += grid * blockDim.x) {
     a += counts[2 * size_t(i)];
@@ -1589,8 +1617,10 @@ __device__ __attribute__((always_inline)) inline void count_reduce_body(const ui
     atomicAdd(&counters[1], b);
   }
 }
-__global__ __launch_bounds__(256) void k_count_reduce(const uint32_t* __restrict__ counts, uint32_t n,
-                                                      unsigned long long* __restrict__ counters) { count_reduce_body(counts, n, counters, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_count_reduce(
+    const uint32_t* __restrict__ counts, uint32_t n, unsigned long long* __restrict__ counters) {
+  count_reduce_body(counts, n, counters, blockIdx.x, gridDim.x);
+}
 
 void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, const NearBuffers& b, const NearCut& cut,
                        Rows out) {
