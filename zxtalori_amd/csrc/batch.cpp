@@ -40,8 +40,10 @@ Batch::~Batch() {
     if (t->host_desc) (void)hipHostFree(t->host_desc);
     if (t->host_start) (void)hipHostFree(t->host_start);
   }
-  if (near_table_.host_desc) (void)hipHostFree(near_table_.host_desc);
-  if (near_table_.host_start) (void)hipHostFree(near_table_.host_start);
+  for (NearTable& t : near_table_) {
+    if (t.host_desc) (void)hipHostFree(t.host_desc);
+    if (t.host_start) (void)hipHostFree(t.host_start);
+  }
   for (hipEvent_t e : inflight_) (void)hipEventDestroy(e);
   for (hipEvent_t e : spare_) (void)hipEventDestroy(e);
   for (hipStream_t s : aux_) (void)hipStreamDestroy(s);
@@ -145,8 +147,7 @@ JobSet Batch::upload(Table& t, const std::vector<JobDesc>& descs) {
 
 // The near-field job table (sorted by force variant) and its stage prefix sums, uploaded
 // only when it changed, as upload does for the other tables.
-NearSet Batch::upload_near(const std::vector<NearJob>& jobs) {
-  NearTable& t = near_table_;
+NearSet Batch::upload_near(NearTable& t, const std::vector<NearJob>& jobs) {
   const size_t count = jobs.size();
   if (count == 0) {
     t.last.clear();
@@ -280,22 +281,29 @@ uint32_t Batch::step() {
       }
     }
     for (Job* j : active) steps.push_back(j->engine->step_job());
-    // Jobs of one force variant share a launch.
+    // Jobs of one force variant share a launch.  With two extra streams the near-field
+    // jobs are dealt alternately (in that order) to two sets of similar work, each built and
+    // run on its own stream: one set's build overlaps the other's force launches.
     std::stable_sort(near.begin(), near.end(), [](const NearJob& a, const NearJob& b) { return a.key < b.key; });
+    // (Profiled steps stay on one stream: the profiler's events are on the batch stream.)
+    const uint32_t lanes = prof.on() ? 0u : std::min<uint32_t>(uint32_t(aux_.size()), near.size() >= 2 ? 2u : 1u);
+    std::vector<NearJob> part[2];
+    for (size_t k = 0; k < near.size(); ++k) part[lanes == 2 ? k % 2 : 0].push_back(near[k]);
     const JobSet ap_set = upload(ap_table_, ap);
     const JobSet step_set = upload(step_table_, steps);
-    const NearSet near_set = upload_near(near);
-    // (Profiled steps stay on one stream: the profiler's events are on the batch stream.)
-    const bool split = !aux_.empty() && !near.empty() && !ap.empty() && !prof.on();
+    const NearSet near_set[2] = {upload_near(near_table_[0], part[0]), upload_near(near_table_[1], part[1])};
+    const bool split = lanes > 0 && !near.empty() && (lanes == 2 || !ap.empty());
     if (split) {
       Engine::check(hipEventRecord(fork_, stream), "fork");
-      Engine::check(hipStreamWaitEvent(aux_[0], fork_, 0), "fork");
+      for (uint32_t k = 0; k < lanes; ++k) Engine::check(hipStreamWaitEvent(aux_[k], fork_, 0), "fork");
     }
-    launch_near_jobs(prof, split ? aux_[0] : stream, near_set);
+    for (uint32_t k = 0; k < 2; ++k) launch_near_jobs(prof, split ? aux_[k < lanes ? k : 0] : stream, near_set[k]);
     if (!ap.empty()) launch_allpairs(prof, stream, ap_set);
     if (split) {
-      Engine::check(hipEventRecord(join_[0], aux_[0]), "join");
-      Engine::check(hipStreamWaitEvent(stream, join_[0], 0), "join");
+      for (uint32_t k = 0; k < lanes; ++k) {
+        Engine::check(hipEventRecord(join_[k], aux_[k]), "join");
+        Engine::check(hipStreamWaitEvent(stream, join_[k], 0), "join");
+      }
     }
     launch_bb_step(prof, stream, step_set);  // move and fold included
   };

@@ -23,8 +23,8 @@ enum class JobKind : uint32_t { Relax = 0, Measure = 1, Spacing = 2 };
 class Batch {
  public:
   // use_graphs: replay steady-state steps from a captured graph.  streams > 0: the
-  // near-field launches of a step run on an extra stream, beside the all-pairs launch
-  // (0: all on the batch stream).
+  // near-field launches of a step run on an extra stream, beside the all-pairs launch, and
+  // with 2 or more on two (half the near-field jobs each); 0: all on the batch stream.
   Batch(bool use_graphs, uint32_t streams);
   ~Batch();
 
@@ -99,8 +99,10 @@ class Batch {
     std::vector<NearJob> last;
     NearSet set;
   };
-  NearTable near_table_;
-  NearSet upload_near(const std::vector<NearJob>& jobs);
+  // Near-field jobs go to up to two sets, each on its own extra stream, so that one set's
+  // build overlaps the other's force launches.
+  NearTable near_table_[2];
+  NearSet upload_near(NearTable& t, const std::vector<NearJob>& jobs);
   void finish(Job& j, bool recorded);
 
   std::vector<std::unique_ptr<Job>> jobs_;

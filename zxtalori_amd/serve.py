@@ -157,9 +157,9 @@ class Server:
         self.keep_results = keep_results
         self.bounds = tuple(float(b) for b in bounds)
         self.ahead = int(ahead)
-        # The large lane runs one job; the others run their near-field launches on an extra
-        # stream beside their all-pairs launch.
-        self.lanes = [_Lane(f"lane{k}", graphs, 1) for k in range(len(self.bounds))]
+        # The large lane runs one job; the others run their near-field launches on two extra
+        # streams (half the jobs each) beside their all-pairs launch.
+        self.lanes = [_Lane(f"lane{k}", graphs, 2) for k in range(len(self.bounds))]
         self.lanes.append(_Lane("large", graphs, 0, limit=1))
         self.peak_bytes = 0
         self.max_live = 0
