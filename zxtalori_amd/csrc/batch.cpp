@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 namespace zxt {
 
@@ -366,14 +367,86 @@ uint32_t Batch::step() {
     return completed + uint32_t(active.size());
   }
   mark_step();
+  std::vector<Job*> ending;
   for (Job* j : active) {
     j->engine->stepped();
-    if (--j->remaining == 0) {
-      finish(*j, true);
-      ++completed;
+    if (--j->remaining == 0) ending.push_back(j);
+  }
+  finish_all(ending);
+  return completed + uint32_t(ending.size());
+}
+
+// The jobs that reached their steps end as a relax call does (finish_relax), but their
+// final force passes run as multi-sample launches (one per near-field build stage and
+// force variant, one for all pairs) and one wait covers them all.  A job whose near field
+// overflowed redoes its pass alone, as finish_relax would.
+void Batch::finish_all(const std::vector<Job*>& ending) {
+  if (ending.empty()) return;
+  std::vector<Job*> ready;
+  std::vector<NearJob> near;
+  std::vector<JobDesc> ap;
+  for (Job* j : ending) {
+    Engine& e = *j->engine;
+    try {
+      e.finish_prepare();
+      if (e.relax_state_.has_near) {
+        NearJob nj;
+        e.near_job(nj);
+        near.push_back(nj);
+      } else {
+        ap.push_back(e.relax_state_.allpairs_desc(e, e.count(), e.order()));
+      }
+      ready.push_back(j);
+    } catch (const EngineError& x) {
+      j->state = 2;
+      j->error = x.message;
     }
   }
-  return completed;
+  try {
+    std::stable_sort(near.begin(), near.end(), [](const NearJob& a, const NearJob& b) { return a.key < b.key; });
+    const NearSet near_set = upload_near(near_table_[0], near);
+    const JobSet ap_set = upload(ap_table_, ap);
+    launch_near_jobs(prof, stream, near_set);
+    if (!ap.empty()) launch_allpairs(prof, stream, ap_set);
+    for (Job* j : ready) j->engine->finish_reads(true);
+    Engine::check(hipStreamSynchronize(stream), "wait");
+  } catch (const EngineError& x) {
+    for (Job* j : ready) {
+      j->state = 2;
+      j->error = x.message;
+    }
+    return;
+  }
+  // The host's share (the last step's result, per-point separations and measures) is the
+  // jobs' own: it runs on several host threads.  A pass to redo waits for the others.
+  std::vector<int> outcome(ready.size(), 0);  // 0 done, 1 redo, 2 failed
+  std::vector<std::string> message(ready.size());
+  const size_t threads = std::min<size_t>(ready.size(), std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  auto work = [&](size_t first) {
+    for (size_t k = first; k < ready.size(); k += threads) {
+      try {
+        outcome[k] = ready[k]->engine->finish_check(true) ? 0 : 1;
+      } catch (const EngineError& x) {
+        outcome[k] = 2;
+        message[k] = x.message;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < threads; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (std::thread& t : pool) t.join();
+  for (size_t k = 0; k < ready.size(); ++k) {
+    Job& j = *ready[k];
+    try {
+      if (outcome[k] == 2) throw EngineError{message[k]};
+      if (outcome[k] == 1) j.engine->finish_relax(false);
+      j.state = 1;
+    } catch (const EngineError& x) {
+      j.state = 2;
+      j.error = x.message;
+    }
+  }
 }
 
 int Batch::status(uint64_t id) const { return find(id)->state; }

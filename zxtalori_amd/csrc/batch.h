@@ -104,6 +104,7 @@ class Batch {
   NearTable near_table_[2];
   NearSet upload_near(NearTable& t, const std::vector<NearJob>& jobs);
   void finish(Job& j, bool recorded);
+  void finish_all(const std::vector<Job*>& ending);
 
   std::vector<std::unique_ptr<Job>> jobs_;
   uint64_t next_id_ = 1;

@@ -263,11 +263,19 @@ void NearField::build(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
 }
 
 bool NearField::overflowed(Engine& e) {
+  read_status(e);
+  e.sync();
+  return took_overflow(e);
+}
+
+void NearField::read_status(Engine& e) {
   Engine::check(hipMemcpyAsync(&status, info.p, sizeof(NearInfo), hipMemcpyDeviceToHost, e.stream), "info");
   Engine::check(hipMemcpyAsync(e.last_near_counts, counters.p, sizeof(e.last_near_counts), hipMemcpyDeviceToHost,
                                e.stream),
                 "counters");
-  e.sync();
+}
+
+bool NearField::took_overflow(Engine& e) {
   e.last_near = status;
   e.last_near_sdim = sdim;
   e.last_near_cut = cut;
@@ -818,41 +826,55 @@ void Engine::relax(uint32_t steps) {
 // One wait ends a batch of steps: last step, positions, energy and separation
 // (pairs.zig:377-399); these forces start the next step.
 void Engine::finish_relax(bool recorded) {
-  RelaxState& rs = relax_state_;
-  const uint32_t ord = order();
-  measured_ = false;
-  auto& sep = separation_out();
-  BBState st{};
   for (int attempt = 0; attempt < 4; ++attempt) {
-    rs.prepare(*this, n_);
-    rs.current = false;
-    rs.record_forces(*this, n_, ord);
-    if (recorded) check(hipMemcpyAsync(&st, rs.state.p, sizeof(BBState), hipMemcpyDeviceToHost, stream), "state");
-    host_.resize(size_t(n_) * 4);
-    check(hipMemcpyAsync(host_.data(), q_.p, size_t(n_) * 16, hipMemcpyDeviceToHost, stream), "positions");
-    download(rows_force_, rs.force, n_);
-    download(rows_energy_, rs.energy, n_);
-    download(rows_nearest_, rs.nearest, n_);
+    finish_prepare();
+    relax_state_.record_forces(*this, n_, order());
+    finish_reads(recorded);
     sync();
-    if (recorded) {
-      if (st.flags & 1u) fail("NonFiniteForce");
-      if (st.flags & 2u) fail("DuplicateOrientations");
-      float step_max;
-      std::memcpy(&step_max, &st.step_bits, 4);
-      step_deg_ = double(step_max) * 360.0 / kPi;
-      force_rms_ = std::sqrt(st.ff / double(n_));
-      recorded = false;
-    }
-    if (rs.has_near && rs.near.overflowed(*this)) continue;
-    measures_ = quick_measures(rows_force_, rows_energy_, rows_nearest_);
-    for (uint32_t i = 0; i < n_; ++i) sep[i] = float(nearest_degrees(rows_nearest_[i]));
-    full_measures_ = false;
-    measured_ = true;
-    measure_iteration_ = iteration_;
-    rs.current = true;
-    return;
+    if (finish_check(recorded)) return;
+    recorded = false;  // the last step's result was read; the redone pass reads none
   }
   fail("NearShellTooLarge");
+}
+
+void Engine::finish_prepare() {
+  measured_ = false;
+  relax_state_.prepare(*this, n_);
+  relax_state_.current = false;
+}
+
+void Engine::finish_reads(bool recorded) {
+  RelaxState& rs = relax_state_;
+  if (recorded)
+    check(hipMemcpyAsync(&finish_state_, rs.state.p, sizeof(BBState), hipMemcpyDeviceToHost, stream), "state");
+  host_.resize(size_t(n_) * 4);
+  check(hipMemcpyAsync(host_.data(), q_.p, size_t(n_) * 16, hipMemcpyDeviceToHost, stream), "positions");
+  download(rows_force_, rs.force, n_);
+  download(rows_energy_, rs.energy, n_);
+  download(rows_nearest_, rs.nearest, n_);
+  if (rs.has_near) rs.near.read_status(*this);
+}
+
+bool Engine::finish_check(bool recorded) {
+  RelaxState& rs = relax_state_;
+  if (recorded) {
+    const BBState& st = finish_state_;
+    if (st.flags & 1u) fail("NonFiniteForce");
+    if (st.flags & 2u) fail("DuplicateOrientations");
+    float step_max;
+    std::memcpy(&step_max, &st.step_bits, 4);
+    step_deg_ = double(step_max) * 360.0 / kPi;
+    force_rms_ = std::sqrt(st.ff / double(n_));
+  }
+  if (rs.has_near && rs.near.took_overflow(*this)) return false;
+  auto& sep = separation_out();
+  measures_ = quick_measures(rows_force_, rows_energy_, rows_nearest_);
+  for (uint32_t i = 0; i < n_; ++i) sep[i] = float(nearest_degrees(rows_nearest_[i]));
+  full_measures_ = false;
+  measured_ = true;
+  measure_iteration_ = iteration_;
+  rs.current = true;
+  return true;
 }
 
 size_t Engine::device_bytes() const {

@@ -135,6 +135,10 @@ struct NearField {
   void ready_for(Engine& e, const float4* q, uint32_t n, uint32_t order, uint32_t group, double mean_radius);
   // After a wait: whether the last pass overflowed (then the next pass sizes again).
   bool overflowed(Engine& e);
+  // overflowed() in two halves, for callers that wait once for many samples: the reads of
+  // the pass's status and counters (enqueued), then, after the wait, the check.
+  void read_status(Engine& e);
+  bool took_overflow(Engine& e);
 };
 
 // Relaxation state (pairs.zig:220-413): rows, BB history, step state, near field.
@@ -283,6 +287,14 @@ class Engine {
   JobDesc step_job();
   void stepped();
   void finish_relax(bool recorded);
+  // finish_relax in parts, so that a batch records the final force passes of many samples
+  // together and waits once: set-up before the pass, the reads after it (enqueued), and,
+  // after the wait, the last step's result and the measures (false: the near field
+  // overflowed, and finish_relax(false) redoes the pass).
+  void finish_prepare();
+  void finish_reads(bool recorded);
+  bool finish_check(bool recorded);
+  BBState finish_state_{};
   double relax_radius_ = 0;
   bool owns_stream_ = false;
 
