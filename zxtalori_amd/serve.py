@@ -98,7 +98,8 @@ def run_batched(jobs, *, graphs=True, keep_results=False, clock=time.perf_counte
     Lanes keep a cheap request from waiting on costlier ones' steps; with every job
     admitted at the start there is no such wait to spare, and one batch runs each build
     stage and force variant of all near-field jobs as one launch."""
-    with Server(hbm_budget=1 << 62, graphs=graphs, keep_results=keep_results, bounds=(math.inf,)) as server:
+    with Server(hbm_budget=1 << 62, graphs=graphs, keep_results=keep_results, bounds=(math.inf,),
+                near_streams=2) as server:
         outcomes, stats = server.run(jobs, realtime=False, clock=clock)
         stats["graph_captures"], stats["graph_replays"] = server.graphs()
     return outcomes, stats
@@ -152,14 +153,19 @@ class Server:
     waiting for all of its steps.
     """
 
-    def __init__(self, hbm_budget=64 << 30, *, graphs=True, keep_results=False, bounds=(6e8,), ahead=1):
+    def __init__(self, hbm_budget=64 << 30, *, graphs=True, keep_results=False, bounds=(6e8,), ahead=1,
+                 near_streams=1):
         self.hbm_budget = int(hbm_budget)
         self.keep_results = keep_results
         self.bounds = tuple(float(b) for b in bounds)
         self.ahead = int(ahead)
-        # The large lane runs one job; the others run their near-field launches on two extra
-        # streams (half the jobs each) beside their all-pairs launch.
-        self.lanes = [_Lane(f"lane{k}", graphs, 2) for k in range(len(self.bounds))]
+        # The large lane runs one job; the others run their near-field launches on
+        # `near_streams` extra streams beside their all-pairs launch (two: half the jobs
+        # each, one set's build overlapping the other's force pass).  A batch of all jobs at
+        # once gains from two (the 256-job mix: 151 against 159 ms per step); the serving
+        # lanes' small jobs do not (p50 68/53/432 ms with one against 87/57/443 with two,
+        # p99 alike, on three seeds of the 40-request stream).
+        self.lanes = [_Lane(f"lane{k}", graphs, near_streams) for k in range(len(self.bounds))]
         self.lanes.append(_Lane("large", graphs, 0, limit=1))
         self.peak_bytes = 0
         self.max_live = 0
