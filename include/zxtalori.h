@@ -214,6 +214,10 @@ const uint32_t* zo_batch_point_indices(zxt_batch* b, uint64_t id);
 uint64_t zo_batch_bytes(zxt_batch* b, uint64_t id);
 int32_t zo_batch_remove(zxt_batch* b, uint64_t id);
 uint32_t zo_batch_running(zxt_batch* b);
+/* Test hooks, as zo_near_squeeze and zo_near_resizes for a batch job's engine: the job's
+ * next near-field sizing leaves room for the site count it finds plus `sites`. */
+int32_t zo_batch_near_squeeze(zxt_batch* b, uint64_t id, int32_t sites);
+uint32_t zo_batch_near_resizes(zxt_batch* b, uint64_t id);
 /* Blocks until at most `most` steps of this batch are unfinished on the device (a serving
  * loop keeps each batch a step or two ahead of the device); in_flight receives how many are. */
 int32_t zo_batch_wait(zxt_batch* b, uint32_t most, uint32_t* in_flight);

@@ -474,3 +474,36 @@ def test_near_overflow_is_sized_again_and_redone(operator_fixture):
     ref = oracle.near_brute(q, ops, cut)
     check_rows(*rows, ref)
     assert energy == pytest.approx(ref[:, 4].sum() / (count * len(ops) * (count - 1)), rel=2e-6)
+
+
+def test_batch_end_of_relax_redoes_an_overflowed_pass(operator_fixture):
+    """The batch ends a job's relaxation in a pass shared with the other ending jobs; when the
+    job's near field overflowed there (room for half the shell, the test hook), the job's
+    pass is sized again and redone alone, as finish_relax would: one resize, and the energy
+    and separation it reports are the oracle's at its final positions."""
+    from zxtalori_amd.batch import Batch
+    group, count = "I", 5000
+    ops = ops32(operator_fixture, group)
+    cut, _ = dense.near_cut(count, len(ops), 20)
+    with zo.haar(group, count, 5) as plain:
+        plain.relax(0, neighbourhood=20)
+        plain.force_rows()
+        shell = plain.near_info()["shell"]
+    assert shell > 100
+    note("batch end of relax after a near overflow")
+    with Batch(graphs=False, streams=2) as batch:
+        other = batch.add("O", count=3000, seed=1, kind="relax", steps=4, neighbourhood=20)
+        job = batch.add(group, count=count, seed=5, kind="relax", steps=4, neighbourhood=20)
+        assert batch._lib.zo_batch_near_squeeze(batch._handle, job, -(shell // 2)) == 0
+        while batch.running:
+            batch.step()
+            batch.wait(0)
+        assert batch.status(job) == "done" and batch.status(other) == "done"
+        assert batch._lib.zo_batch_near_resizes(batch._handle, job) == 1
+        assert batch._lib.zo_batch_near_resizes(batch._handle, other) == 0
+        stats = batch.stats(job)
+        q = batch.quaternions(job).copy()
+    ref = oracle.near_brute(q, ops, cut)
+    assert stats["energy"] == pytest.approx(ref[:, 4].sum() / (count * len(ops) * (count - 1)), rel=2e-6)
+    least = 4 * math.degrees(math.asin(min(1.0, math.sqrt(ref[:, 5].min()) / 2)))
+    assert stats["separation_deg"] == pytest.approx(least, abs=1e-5)

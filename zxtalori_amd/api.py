@@ -45,7 +45,7 @@ zo_batch_create u e;  zo_batch_destroy e v;  zo_batch_error e s;  zo_batch_add e
 zo_batch_step eU i;  zo_batch_status eq i;  zo_batch_job_error eq s;  zo_batch_stats eq s
 zo_batch_quaternions eq F;  zo_batch_count eq u;  zo_batch_point_values equ F
 zo_batch_point_indices eq U;  zo_batch_bytes eq q;  zo_batch_remove eq i;  zo_batch_running e u
-zo_batch_wait euU i;  zo_batch_graphs eQ i
+zo_batch_wait euU i;  zo_batch_graphs eQ i;  zo_batch_near_squeeze eqi i;  zo_batch_near_resizes eq u
 """
 _CTYPES = {"e": C.c_void_p, "u": C.c_uint32, "i": C.c_int32, "q": C.c_uint64, "f": C.c_float, "b": C.c_bool,
            "z": C.c_size_t, "s": C.c_char_p, "p": C.c_void_p, "F": C.POINTER(C.c_float),

@@ -354,6 +354,20 @@ int32_t zo_batch_remove(zxt_batch* b, uint64_t id) {
 
 uint32_t zo_batch_running(zxt_batch* b) { return b->batch.running(); }
 
+int32_t zo_batch_near_squeeze(zxt_batch* b, uint64_t id, int32_t sites) {
+  return guarded_batch(b, "Batch near squeeze", [&] {
+    zxt::Engine& e = b->batch.engine(id);
+    e.near_squeeze = sites;
+    e.near_squeeze_pending = true;
+  });
+}
+
+uint32_t zo_batch_near_resizes(zxt_batch* b, uint64_t id) {
+  uint32_t v = 0;
+  guarded_batch(b, "Batch near resizes", [&] { v = b->batch.engine(id).near_resizes; });
+  return v;
+}
+
 int32_t zo_batch_wait(zxt_batch* b, uint32_t most, uint32_t* in_flight) {
   return guarded_batch(b, "Batch wait", [&] {
     b->batch.wait(most);
