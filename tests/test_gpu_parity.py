@@ -472,8 +472,19 @@ def test_near_overflow_is_sized_again_and_redone(operator_fixture):
         q = grid.quaternions
         assert grid._lib.zo_near_resizes(grid._handle) == 1
     ref = oracle.near_brute(q, ops, cut)
-    check_rows(*rows, ref)
+    force, energies, nearest = rows
+    np.testing.assert_array_equal(nearest, ref[:, 5].astype(np.float32))
+    np.testing.assert_allclose(energies, ref[:, 4], rtol=2e-6)
+    np.testing.assert_allclose(force[:, 3], ref[:, 3], rtol=1e-5)
     assert energy == pytest.approx(ref[:, 4].sum() / (count * len(ops) * (count - 1)), rel=2e-6)
+    # Relaxed rows: the net force is a small difference of large pair forces, so it is
+    # measured against FP64 pair terms on the pair-force scale (as for the O, N=1e5 rows).
+    _, r = dense.near_cut(count, len(ops), 20)
+    ref64 = oracle.near_grid64(q, ops, np.asarray(operator_fixture[group], np.float64), cut, r)
+    scaled = np.linalg.norm(force[:, :3] - ref64[:, :3]) / np.linalg.norm(ref64[:, 6])
+    ref_err = np.linalg.norm(ref[:, :3] - ref64[:, :3]) / np.linalg.norm(ref64[:, 6])
+    note("relaxed force err vs pair-force scale", scaled, "oracle fp32", ref_err)
+    assert scaled <= 2e-6 and scaled <= 2 * ref_err
 
 
 def test_batch_end_of_relax_redoes_an_overflowed_pass(operator_fixture):

@@ -30,6 +30,17 @@ constexpr double kFullMeasurePairs = 5e10;    // engine.zig:1123
 constexpr uint32_t kSpacingAllPairs = 20000;  // engine.zig:1126
 constexpr double kSpacingDelta = 4;           // engine.zig:1127
 constexpr double kSpacingCellSites = 0.75;    // sites per cell of the spacing pass's grid
+// Sites of one near field: the force kernel addresses them by 32-bit byte offsets
+// (16 B each, 64 entries of slack past the capacity).
+constexpr uint64_t kMaxSites = (uint64_t(1) << 28) - 64;
+// Cell shape of the force passes' grid: y/z cells kForceEdge times the edge of a cube
+// holding the chosen sites per cell, split into kForceXSplit x-cells each.  A cutoff ball
+// then spans fewer rows, trimmed to finer x-cells: fewer, fuller 64-site chunks per
+// accepted pair.  Measured on MI355X at I, N = 1e6, reach 20 (force kernel, same box):
+// cubic cells 14.53 ms; edge 1.35 with 16 x-cells 13.32-13.40; 1.5/16 13.46; 1.2/16 13.53;
+// 1.0/16 13.78; 2.0/8 14.14; 2.5/16 14.60; 32 x-cells no better than 16.
+constexpr uint32_t kForceXSplit = 16;
+constexpr double kForceEdge = 1.35;
 const char* kVersion = "0.3.0-mi355x";
 
 [[noreturn]] void fail(const std::string& what) { throw EngineError{what}; }
@@ -107,6 +118,7 @@ NearBuffers NearField::view() const {
   b.reach = float(radius + 1e-4);
   b.capacity = capacity;
   b.sdim = sdim;
+  b.xdiv = xdiv;
   b.info = info.as<NearInfo>();
   b.rim = rim.as<uint32_t>();
   b.uq = uq.as<float4>();
@@ -195,7 +207,7 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   uint64_t cap = std::max<uint64_t>(uint64_t(n) + 1024, uint64_t(n) * 4);
   bool sized = false;
   for (int attempt = 0; attempt < 5; ++attempt) {
-    capacity = uint32_t(std::min<uint64_t>(cap, 0xfff00000ull));
+    capacity = uint32_t(std::min<uint64_t>(cap, kMaxSites));
     uq.ensure(size_t(capacity) * 16);
     ucode.ensure(size_t(capacity) * 4);
     urank.ensure(size_t(capacity) * 4);
@@ -218,32 +230,37 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
     capacity = uint32_t(std::min<int64_t>(want, int64_t(capacity)));
   } else {
     // Room for the shell to change as the sources move.
-    capacity = uint32_t(std::min<uint64_t>(std::max<uint64_t>(capacity, count + count / 4 + 1024), 0xfff00000ull));
+    capacity = uint32_t(std::min<uint64_t>(std::max<uint64_t>(capacity, count + count / 4 + 1024), kMaxSites));
   }
   // Sites per cell of the box.  A ball of `reach` mean radii holds about reach^3 sites:
   // thousands at reach 20, where coarse cells (8 sites) win by giving fewer, fuller chunks
-  // of 64 sites per row at the cost of more candidates.  Measured on MI355X at I, N = 1e6:
-  // the force pass at reach 20 takes 15.5 ms with 8 sites per cell against 17.3 ms with 2,
-  // at reach 8 2.2 ms with 4 against 2.6 ms with 2.  The spacing pass sets its own
+  // of 64 sites per row at the cost of more candidates.  Measured on MI355X at I, N = 1e6
+  // with cubic cells: the force pass at reach 20 takes 15.5 ms with 8 sites per cell
+  // against 17.3 ms with 2, at reach 8 2.2 ms with 4 against 2.6 ms with 2.  Force passes
+  // then reshape the cells (kForceEdge, kForceXSplit).  The spacing pass sets its own
   // (cell_sites): its first search covers half its reach, a few sites, best found in
   // fine cells (I, N = 1e7, the whole pass: 4.24 ms with 0.75 sites per cell, 4.33 with
   // 0.5, 4.40 with 1.0, 4.53 with 1.6, 4.83 with 3).
   const double reach = radius / mean_radius_;
   const double per_cell = cell_sites > 0 ? cell_sites : std::min(8.0, std::max(1.5, reach / 2.5));
-  sdim = uint32_t(std::min(320.0, std::max(1.0, std::round(std::cbrt(double(count) / per_cell)))));
-  const size_t cells = size_t(sdim) * sdim * sdim;
+  xdiv = cell_sites > 0 ? 1u : x_split;
+  const double edge = xdiv > 1 ? kForceEdge : 1.0;
+  const double max_dim = std::floor(320.0 / std::cbrt(double(xdiv)));  // at most 320^3 cells
+  sdim = uint32_t(std::min(max_dim, std::max(1.0, std::round(std::cbrt(double(count) / per_cell) / edge))));
+  const size_t cells = cell_count(sdim, xdiv);
   uq.ensure(size_t(capacity) * 16);
   ucode.ensure(size_t(capacity) * 4);
   urank.ensure(size_t(capacity) * 4);
-  ucell.ensure((std::max<size_t>(capacity, tile_visits(sdim)) + 1) * 4);  // then the visits' scan
+  ucell.ensure((std::max<size_t>(capacity, tile_visits(sdim, xdiv)) + 1) * 4);  // then the visits' scan
   perm.ensure(size_t(capacity) * 4);
-  sq.ensure(size_t(capacity) * 16);
-  scode.ensure(size_t(capacity) * 4);
+  // The force kernel loads whole 64-site chunks from a row's start: 64 entries of slack.
+  sq.ensure((size_t(capacity) + 64) * 16);
+  scode.ensure((size_t(capacity) + 64) * 4);
   srank.ensure(size_t(capacity) * 4);
   offsets.ensure((cells + 1) * 4);
   uslot.ensure(size_t(capacity) * 4);
   order_buf.ensure(size_t(n) * 4);
-  scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, tile_visits(sdim))) * 4);
+  scan_tmp.ensure(scan_tmp_words(std::max<size_t>(cells, tile_visits(sdim, xdiv))) * 4);
   counters.ensure(4 * sizeof(unsigned long long));
   counts.ensure(size_t(n) * 8);
   crowded.ensure(size_t(crowded_capacity(capacity)) * 4);
@@ -572,6 +589,7 @@ void Engine::set_neighbourhood(double delta) {
 
 void Engine::sync_neighbourhood() {
   relax_state_.set_near(group_, neighbourhood_ > 0 ? &neighbourhood_ : nullptr);
+  relax_state_.near.x_split = kForceXSplit;
   measure_neighbourhood_ = neighbourhood_;
 }
 
@@ -700,6 +718,7 @@ double Engine::nearest_pass(RelaxState& rs) {
       // The nearest orbit point is a minimum over keys: the sites' order within a cell
       // does not matter.
       rs.near.build(*this, q, n_, order(), group_, mean_radius(uint64_t(n_) * order()), false);
+      if (rs.near.xdiv != 1) fail("GpuError: the spacing walk takes cubic cells");
       launch_near_nearest(prof, stream, n_, order(), rs.near.view(), rs.near.cut, degrees, rs.nn_rank.as<uint32_t>(),
                           least_bits);
     } else {
@@ -1005,6 +1024,7 @@ void Engine::cover_measure(bool vertices, bool tets, bool codes) {
       }
       cover_near_.build(*this, q, n_, ord, group_, mr);
       view = cover_near_.view();
+      if (view.xdiv != 1) fail("GpuError: the covering walk takes cubic cells");
       grid = &view;
       cut = cover_near_.cut;
     }

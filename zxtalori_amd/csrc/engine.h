@@ -120,6 +120,9 @@ struct NearField {
   double radius = 0;
   NearCut cut{};
   uint32_t capacity = 0, sdim = 1, nplanes = 0;
+  // x-cells per y/z cell: x_split asked for (force passes), xdiv the grid's (1 when
+  // cell_sites is set: the spacing and covering walks assume cubic cells).
+  uint32_t x_split = 1, xdiv = 1;
   float band = 0;
   // The last pass's sites, shell and overflow flag (valid after a wait).
   NearInfo status{};

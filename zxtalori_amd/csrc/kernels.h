@@ -195,8 +195,8 @@ struct NearBuffers {
   float band = 0;        // sin(r + 1e-4): rim and shell margin (shell.wgsl:19-22)
   float reach = 0;       // r + 1e-4: the angle within which a site can matter
   uint32_t capacity = 0;
-  uint32_t sdim = 1;
-  uint32_t pad0 = 0;             // (no padding bytes: job tables compare bytewise)
+  uint32_t sdim = 1;             // cells along y and z
+  uint32_t xdiv = 1;             // cells along x per y/z cell (x-cells xdiv times finer)
   NearInfo* info = nullptr;
   uint32_t* rim = nullptr;       // n
   float4* uq = nullptr;          // capacity, unsorted sites
@@ -207,7 +207,7 @@ struct NearBuffers {
   float4* sq = nullptr;          // capacity, sorted by cell
   uint32_t* scode = nullptr;     // capacity
   uint32_t* srank = nullptr;     // capacity
-  uint32_t* offsets = nullptr;   // sdim^3 + 1
+  uint32_t* offsets = nullptr;   // cell_count(sdim, xdiv) + 1
   uint32_t* uslot = nullptr;     // capacity: each site's arrival slot in its cell
   uint32_t* order = nullptr;     // n, sources in the tile-by-tile visit order of their identity images
   float4* opos = nullptr;        // n, their positions in that order, unordered builds only (shares uq)
@@ -222,12 +222,18 @@ struct NearBuffers {
   uint32_t pad2 = 0;
 };
 static_assert(sizeof(NearBuffers) == 200, "NearBuffers has no padding bytes");
-// The sources are visited by tiles of kTile^3 cells; the padded grid of tiles has this many
-// visits (at least sdim^3).
+// Cells of a near-field grid: sdim * xdiv along x, sdim along y and z.
+__host__ __device__ inline size_t cell_count(uint32_t sdim, uint32_t xdiv) {
+  return size_t(sdim) * xdiv * sdim * sdim;
+}
+// The sources are visited by tiles of cells: kTile^3 cubic cells, or with x-cells xdiv times
+// finer, (kTile / 2 * xdiv) x (kTile / 2)^2 cells.  The padded grid of tiles has this many
+// visits (at least cell_count).
 constexpr uint32_t kTile = 8;
-__host__ __device__ inline uint32_t tile_visits(uint32_t sdim) {
-  const uint32_t nt = (sdim + kTile - 1) / kTile;
-  return nt * nt * nt * kTile * kTile * kTile;
+__host__ __device__ inline uint32_t tile_edge(uint32_t xdiv) { return xdiv > 1 ? kTile / 2 : kTile; }
+__host__ __device__ inline uint32_t tile_visits(uint32_t sdim, uint32_t xdiv) {
+  const uint32_t e = tile_edge(xdiv), nt = (sdim + e - 1) / e;
+  return nt * nt * nt * e * e * e * xdiv;
 }
 // capacity / 17 + 1 bounds the cells holding more than 16 of `capacity` sites, and
 // capacity / 1025 + 1 those holding more than 1024.

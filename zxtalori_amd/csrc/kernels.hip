@@ -99,15 +99,6 @@ __device__ inline T wave_sum(T v) {
   return v;
 }
 
-// acc + (sum of v over the wave) in FP64, returned as a wave-uniform value (scalar
-// registers).
-__device__ inline double uniform_add(double acc, float v) {
-  const uint64_t bits = uint64_t(__double_as_longlong(wave_sum_to_last(double(v))));
-  const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(bits), 63);
-  const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(bits >> 32), 63);
-  return acc + __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
-}
-
 __device__ inline uint32_t wave_min_u32(uint32_t v) {
   for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
   return v;
@@ -1047,10 +1038,11 @@ __device__ inline uint32_t cell_axis(float v, float lower, float cpu, uint32_t d
 __device__ __attribute__((always_inline)) inline void site_count_body(NearBuffers b, uint32_t block, uint32_t grid) {
   const uint32_t S = min(b.info->sites, b.capacity);
   const float lx = b.info->lower[0], ly = b.info->lower[1], lz = b.info->lower[2], cpu = b.info->cpu;
-  const uint32_t d = b.sdim;
+  const uint32_t d = b.sdim, dx = b.sdim * b.xdiv;
+  const float cpx = cpu * float(b.xdiv);  // x-cells per chart unit
   for (uint32_t s = block * blockDim.x + threadIdx.x; s < S; s += grid * blockDim.x) {
     const float3 p = chart(b.uq[s]);
-    const uint32_t c = cell_axis(p.x, lx, cpu, d) + d * (cell_axis(p.y, ly, cpu, d) + d * cell_axis(p.z, lz, cpu, d));
+    const uint32_t c = cell_axis(p.x, lx, cpx, dx) + dx * (cell_axis(p.y, ly, cpu, d) + d * cell_axis(p.z, lz, cpu, d));
     b.ucell[s] = c;
     b.uslot[s] = atomicAdd(&b.offsets[c], 1u);
   }
@@ -1282,18 +1274,18 @@ __global__ __launch_bounds__(256) void k_cell_sort_block(
   cell_sort_block_body(offsets, perm, key, huge_list, huge, list_capacity, tmp, sites, blockIdx.x, gridDim.x);
 }
 
-// The sources are visited tile by tile (8x8x8 cells, tiles in row-major order, cells
-// row-major within a tile), each cell's identity images in their sorted order: waves of
-// neighbouring sources read overlapping site runs, and each XCD takes a contiguous range
-// of them (xcd_remap) whose neighbourhood its L2 holds.  Visit v of the padded tile grid
-// (tile_visits(sdim) entries) names the cell, or none past the grid's edge.
-__device__ inline bool visit_cell(uint32_t v, uint32_t sdim, uint32_t& cell) {
-  const uint32_t nt = (sdim + kTile - 1) / kTile;
-  const uint32_t w = v % (kTile * kTile * kTile), t = v / (kTile * kTile * kTile);
-  const uint32_t x = (t % nt) * kTile + w % kTile, y = (t / nt % nt) * kTile + w / kTile % kTile,
-                 z = (t / (nt * nt)) * kTile + w / (kTile * kTile);
-  if (x >= sdim || y >= sdim || z >= sdim) return false;
-  cell = x + sdim * (y + sdim * z);
+// The sources are visited tile by tile (tiles of tile_edge y/z cells a side, tiles in
+// row-major order, cells row-major within a tile), each cell's identity images in their
+// sorted order: waves of neighbouring sources read overlapping site runs, and each XCD
+// takes a contiguous range of them (xcd_remap) whose neighbourhood its L2 holds.  Visit v
+// of the padded tile grid (tile_visits entries) names the cell, or none past the grid's
+// edge.
+__device__ inline bool visit_cell(uint32_t v, uint32_t sdim, uint32_t xdiv, uint32_t& cell) {
+  const uint32_t e = tile_edge(xdiv), ex = e * xdiv, nt = (sdim + e - 1) / e, per = ex * e * e;
+  const uint32_t w = v % per, t = v / per;
+  const uint32_t x = (t % nt) * ex + w % ex, y = (t / nt % nt) * e + w / ex % e, z = (t / (nt * nt)) * e + w / (ex * e);
+  if (x >= sdim * xdiv || y >= sdim || z >= sdim) return false;
+  cell = x + sdim * xdiv * (y + sdim * z);
   return true;
 }
 
@@ -1302,10 +1294,10 @@ __device__ inline bool visit_cell(uint32_t v, uint32_t sdim, uint32_t& cell) {
 // keep no codes), whose rank src * 2|K| + image has image 0.
 __device__ __attribute__((always_inline)) inline void core_count_body(
     NearBuffers b, uint32_t two_order, uint32_t block, uint32_t grid) {
-  const uint32_t visits = tile_visits(b.sdim);
+  const uint32_t visits = tile_visits(b.sdim, b.xdiv);
   for (uint32_t v = block * blockDim.x + threadIdx.x; v < visits; v += grid * blockDim.x) {
     uint32_t cell, count = 0;
-    if (visit_cell(v, b.sdim, cell))
+    if (visit_cell(v, b.sdim, b.xdiv, cell))
       for (uint32_t k = b.offsets[cell]; k < b.offsets[cell + 1]; ++k)
         count += two_order ? uint32_t(b.srank[k] % two_order == 0) : b.scode[k] >> 31;
     b.ucell[v] = count;
@@ -1319,10 +1311,10 @@ __global__ void k_core_count(NearBuffers b, uint32_t two_order) {
 // the spacing pass, two_order != 0, their positions).
 __device__ __attribute__((always_inline)) inline void core_order_body(
     NearBuffers b, uint32_t two_order, uint32_t block, uint32_t grid) {
-  const uint32_t visits = tile_visits(b.sdim);
+  const uint32_t visits = tile_visits(b.sdim, b.xdiv);
   for (uint32_t v = block * blockDim.x + threadIdx.x; v < visits; v += grid * blockDim.x) {
     uint32_t cell;
-    if (!visit_cell(v, b.sdim, cell)) continue;
+    if (!visit_cell(v, b.sdim, b.xdiv, cell)) continue;
     uint32_t at = b.ucell[v];
     for (uint32_t k = b.offsets[cell]; k < b.offsets[cell + 1]; ++k) {
       if (two_order) {
@@ -1360,7 +1352,7 @@ void launch_near_sites(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 
 void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, const float4* ops, uint32_t order,
                        const NearBuffers& b, bool ordered) {
-  const size_t cells = size_t(b.sdim) * b.sdim * b.sdim;
+  const size_t cells = cell_count(b.sdim, b.xdiv);
   launch_fill_u32(p, s, b.offsets, 0u, cells + 1);
   launch_near_sites(p, s, q, n, ops, order, b);
   const uint32_t g = stride_grid(b.capacity);
@@ -1374,7 +1366,7 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
     hipLaunchKernelGGL(k_cell_sort_block, dim3(kHugeBlocks), dim3(kBlock), 0, s, b.offsets, b.perm, b.urank, b.huge,
                        &b.info->huge, b.huge_capacity, b.ucell, b);
   });
-  const uint32_t visits = tile_visits(b.sdim), gv = stride_grid(visits);
+  const uint32_t visits = tile_visits(b.sdim, b.xdiv), gv = stride_grid(visits);
   const uint32_t two_order = ordered ? 0u : 2 * order;
   p.run("core_order", [&] { hipLaunchKernelGGL(k_core_count, dim3(gv), dim3(kBlock), 0, s, b, two_order); });
   launch_scan(p, s, b.ucell, visits, b.scan_tmp);
@@ -1383,8 +1375,13 @@ void launch_near_build(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
 }
 
 // ---- near-field rows (pairs.wgsl:363-427) -------------------------------------------
+// The queue of accepted sites of one wave: x - y and |x - y|^2 exactly as the candidate
+// test computed them, one array per component (4-byte strides: a wave's 64 consecutive
+// entries fall in distinct banks).  It is a stack: a chunk of candidates pushes its
+// accepted sites on top, and the pair stage pops the top 64, so at most 63 + 64 entries
+// are ever held and no index wraps.
 struct WaveLds {
-  float4 qe[128];  // x - y of each queued site; |x - y|^2 is recomputed from it bit for bit
+  float stack[5 * 128];  // entry e: x - y at e, e + 128, e + 256, e + 384; |x - y|^2 at e + 512
 };
 
 __device__ inline float row_gap(float c, int idx, uint32_t dim) {
@@ -1395,35 +1392,66 @@ __device__ inline float row_gap(float c, int idx, uint32_t dim) {
   return fmaxf(0.0f, fmaxf(lo - c, c - hi));
 }
 
+// The wave's FP64 total of a per-lane value, as a wave-uniform value (the same order every
+// run).
+__device__ inline double wave_total(double v) {
+  const uint64_t bits = uint64_t(__double_as_longlong(wave_sum_to_last(v)));
+  const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(bits), 63);
+  const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(bits >> 32), 63);
+  return __longlong_as_double((long long)((uint64_t(hi) << 32) | lo));
+}
+
+// Raw buffer view of a device array for wave loads: a uniform 32-bit byte offset in a
+// scalar register plus a fixed per-lane offset (the last descriptor word is the gfx9
+// format's; no range check beyond what the offsets cover).
+__device__ inline __amdgpu_buffer_rsrc_t buffer_view(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, int(0xffffffffu), 0x00020000);
+}
+__device__ inline float4 buffer_load4(__amdgpu_buffer_rsrc_t r, uint32_t lane_bytes, uint32_t wave_bytes) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, lane_bytes, wave_bytes, 0);
+  return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+__device__ inline uint32_t buffer_load1(__amdgpu_buffer_rsrc_t r, uint32_t lane_bytes, uint32_t wave_bytes) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, lane_bytes, wave_bytes, 0);
+}
+
 // Near-field force rows (pairs.wgsl:363-427): per source (one wave) the sum of the
 // windowed Quey pair over every orbit point within the cutoff, outside its own orbit.
 //
 // The cutoff cap is a ball in the chart; up to 64 of the cell rows it crosses are set up
 // at once, one per lane, each trimmed in x to the ball.  The wave then walks them row by
-// row (uniform): a row's sites are contiguous, so lane l tests the site row start + c + l.
-// Candidates within the cutoff enter an LDS ring (ballot + mbcnt) and each full 64 runs
-// the pair arithmetic on a full wave; only a row's last chunk runs partly empty.  Per-lane
-// FP32 sums go into the wave's FP64 sums every 64 rounds.
+// row (uniform): a row's sites are contiguous, so lane l tests the site row start + c + l,
+// loaded at a uniform byte offset (a scalar register, advanced by 1 KiB per chunk) plus the
+// lane's fixed offset.  A row's full 64-site chunks need no lane mask; its last, partial
+// chunk loads past the row's end (the site arrays hold 64 entries beyond their capacity)
+// and masks those lanes out of the test.  Accepted candidates are pushed on the wave's LDS
+// stack (ballot + mbcnt) and each full 64 runs the pair arithmetic on a full wave; only a
+// row's last chunk runs partly empty.  Per-lane FP32 sums go into per-lane FP64 sums every
+// 64 rounds, and the wave's totals are taken once at the end.  Scalar work per chunk is
+// kept to the loop, the push and the stack count: the scalar unit is shared by the CU's
+// four SIMDs and runs near its issue limit here.
 //
 // OWN: whether the source's own orbit can lie within the cutoff.  x.g is at S3 angle
 // acos|g.w| from x for every x, so below the smallest such angle (0.63 rad for I) no own
 // image is ever a candidate: then the candidate stage needs no site codes and no test at
-// all beyond the cutoff, the source's own identity image (chord exactly 0) being dropped
-// when its pair is evaluated, and the nearest chord is taken over the compacted pairs.
+// all beyond the cutoff, the source's own identity image (chord exactly 0: the site
+// bit-identical to x, rebuilt from q before every pass) being dropped when its pair is
+// evaluated.  A zero chord enters the pair arithmetic as chord^2 2 (S3 angle pi/2, beyond
+// every window, so its terms vanish exactly), stays out of the nearest chord, and is
+// counted; a second one (an exact duplicate of x) is restored at the end.
 // TERMS: the arctangent series (atan2x_small).
 // vb: the workgroup's index in the source order (4 sources each).
 template <bool OWN, int TERMS>
 __device__ __attribute__((always_inline)) inline void near_body(
     const float4* __restrict__ q_in, uint32_t n, const uint32_t* __restrict__ order_in,
     const float4* __restrict__ sq_in, const uint32_t* __restrict__ scode_in,
-    const uint32_t* __restrict__ offsets_in, const NearInfo* __restrict__ info_in, uint32_t sdim, NearCut cut,
-    Rows rows, uint32_t* __restrict__ counts, uint32_t vb) {
+    const uint32_t* __restrict__ offsets_in, const NearInfo* __restrict__ info_in, uint32_t sdim, uint32_t xdiv,
+    NearCut cut, Rows rows, uint32_t* __restrict__ counts, uint32_t vb) {
   const auto* __restrict__ q = to_global(q_in);
   const auto* __restrict__ order = to_global(order_in);
-  const auto* __restrict__ sq = to_global(sq_in);
-  const auto* __restrict__ scode = to_global(scode_in);
   const auto* __restrict__ offsets = to_global(offsets_in);
   const auto* __restrict__ info = to_global(info_in);
+  const __amdgpu_buffer_rsrc_t sites = buffer_view(sq_in), codes = buffer_view(scode_in);
   __shared__ WaveLds lds[kBlock / 64];
   // The wave index is uniform: the source, its position and every per-source constant
   // below then live in scalar registers, leaving the VGPRs to the candidate and pair work.
@@ -1434,63 +1462,97 @@ __device__ __attribute__((always_inline)) inline void near_body(
   const uint32_t src = order[slot];
   const float4 x = load4(q + src);
   const float cpu = info->cpu;
-  const float fdim = float(sdim);
+  const float fdim = float(sdim), xd = float(xdiv), fdx = float(sdim * xdiv);
   const float denom = cut.c0 + x.x;
   const bool full = !(denom > 1e-3f);
-  float cx = 0.5f * fdim, cy = 0.5f * fdim, cz = 0.5f * fdim, rc = 3e18f;
+  // Chart position in cell units: y and z in y/z cells, x in x-cells (xdiv per y/z cell).
+  float cx = 0.5f * fdx, cy = 0.5f * fdim, cz = 0.5f * fdim, rc = 3e18f;
   if (!full) {
     const float inv = 1.0f / denom;
-    cx = (x.y * inv - info->lower[0]) * cpu;
+    cx = (x.y * inv - info->lower[0]) * (cpu * xd);
     cy = (x.z * inv - info->lower[1]) * cpu;
     cz = (x.w * inv - info->lower[2]) * cpu;
     rc = (cut.s0 * inv * 1.001f + 1e-6f) * cpu + 1e-3f;
   }
   auto clampc = [&](float v) -> int { return int(fminf(fmaxf(floorf(v), 0.0f), fdim - 1.0f)); };
+  auto clampx = [&](float v) -> int { return int(fminf(fmaxf(floorf(v), 0.0f), fdx - 1.0f)); };
   const int zlo = clampc(cz - rc), zhi = clampc(cz + rc), ylo = clampc(cy - rc), yhi = clampc(cy + rc);
   const uint32_t ny = uint32_t(yhi - ylo + 1), nrows = ny * uint32_t(zhi - zlo + 1);
   const float rc2 = rc * rc;
   const uint32_t self_code = src | 0x80000000u;
+  const uint32_t lane16 = lane * 16u, lane4 = lane * 4u;
 
   NearSum ps{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // per lane
   // Chords are sums of squares (>= 0), so their bits order as unsigned integers: one
   // integer min per update, no float canonicalisation.
   uint32_t nearest_bits = __float_as_uint(4.0f);
-  // Without own images in reach (!OWN), the source's identity image is accepted like any
-  // site and dropped when its pair is evaluated: it is the site bit-identical to x (sites
-  // are rebuilt from q before every pass), so its chord is exactly 0.  A zero chord enters
-  // the pair arithmetic as chord^2 2 (S3 angle pi/2, beyond every window, so its terms
-  // vanish exactly), stays out of the nearest chord, and is counted; a second one (an
-  // exact duplicate of x) is restored at the end.  The count is a ballot over the active
-  // lanes, so every active lane holds it; the partial tail runs on lanes 0..qcount-1 only,
-  // and the count is read from lane 0 after it.
-  uint32_t zeros = 0;
+  uint32_t zeros = 0;  // zero chords met (!OWN): counted from ballots in uniform control flow
   const float u_offset = -cut.taper_start * cut.inv_width;
   const float slope_pi = -12.0f * kPiF * cut.inv_width;
-  uint32_t qhead = 0, qcount = 0, rounds = 0;
+  float* top = &L.stack[0];  // the stack's top (uniform): 0 to 63 entries between chunks
+  uint32_t rounds = 0;
   uint32_t candidates = 0;  // sites tested (uniform); accepted pairs are 64 rounds + the tail
 
-  auto process = [&](uint32_t e) {
-    const float4 d = L.qe[e];
-    float dm2 = dot4(d, d);  // the candidate test's operations: the same bits
+  // The pair stage for the 64 stack entries from `top` (a uniform address).
+  auto process = [&](const float* top) {
+    float dm2 = top[lane + 512];
     if (!OWN) {
-      const bool zero = dm2 == 0.0f;
-      zeros += uint32_t(__popcll(__builtin_amdgcn_ballot_w64(zero)));
-      dm2 = zero ? 2.0f : dm2;
+      const uint64_t zero = __builtin_amdgcn_ballot_w64(dm2 == 0.0f);
+      zeros += uint32_t(__popcll(zero));
       // 2 exceeds every accepted chord (< 2), so it stands for "no pair" in the minimum.
+      dm2 = __builtin_amdgcn_inverse_ballot_w64(zero) ? 2.0f : dm2;
       nearest_bits = min(nearest_bits, __float_as_uint(dm2));
     }
+    const float4 d = make_float4(top[lane], top[lane + 128], top[lane + 256], top[lane + 384]);
     near_pair_terms<TERMS>(d, dm2, cut.inv_width, u_offset, slope_pi, ps);
   };
-  // Every 64 rounds the lanes' FP32 sums (64 pairs each) go into the wave's FP64 sums.
+  // Every 64 rounds the lanes' FP32 sums (64 pairs each) go into their FP64 sums.
   auto flush = [&]() {
-    acc[0] = uniform_add(acc[0], ps.gx);
-    acc[1] = uniform_add(acc[1], ps.gy);
-    acc[2] = uniform_add(acc[2], ps.gz);
-    acc[3] = uniform_add(acc[3], ps.gw);
-    acc[4] = uniform_add(acc[4], ps.curvature);
-    acc[5] = uniform_add(acc[5], ps.energy);
+    acc[0] += double(ps.gx);
+    acc[1] += double(ps.gy);
+    acc[2] += double(ps.gz);
+    acc[3] += double(ps.gw);
+    acc[4] += double(ps.curvature);
+    acc[5] += double(ps.energy);
     ps = NearSum{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  };
+  // One chunk of 64 sites at byte offset `at` of the site array; `in_row`: its lanes that
+  // lie in the row.
+  auto chunk = [&](uint32_t at, uint64_t in_row) {
+    const float4 y = buffer_load4(sites, lane16, at);
+    const float4 d = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
+    const float dm2 = dot4(d, d);  // chord2(x, y), the same operations
+    uint64_t mask = __builtin_amdgcn_ballot_w64(dm2 <= cut.chord2) & in_row;
+    if (OWN) {
+      // Every orbit point but the source itself bounds the nearest chord; the source's own
+      // images exert no force (pairs.wgsl:404-408).
+      const uint32_t code = buffer_load1(codes, lane4, at >> 2);
+      if (__builtin_amdgcn_inverse_ballot_w64(mask) && code != self_code)
+        nearest_bits = min(nearest_bits, __float_as_uint(dm2));
+      mask &= ~__builtin_amdgcn_ballot_w64((code & 0x7fffffffu) == src);
+    }
+    if (mask) {
+      if (__builtin_amdgcn_inverse_ballot_w64(mask)) {
+        const uint32_t e =
+            __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+        top[e] = d.x;
+        top[e + 128] = d.y;
+        top[e + 256] = d.z;
+        top[e + 384] = d.w;
+        top[e + 512] = dm2;
+      }
+      top += __popcll(mask);
+      wave_sync();
+      if (top >= &L.stack[64]) {
+        // The top 64 entries: the ones below stay where they are.
+        top -= 64;
+        process(top);
+        rounds = __builtin_amdgcn_readfirstlane(rounds + 1);
+        if ((rounds & 63u) == 0) flush();
+        wave_sync();
+      }
+    }
   };
 
   for (uint32_t rb = 0; rb < nrows; rb += 64) {
@@ -1501,9 +1563,9 @@ __device__ __attribute__((always_inline)) inline void near_body(
       const float gy = row_gap(cy, yy, sdim), gz = row_gap(cz, zz, sdim);
       const float rest = rc2 - gy * gy - gz * gz;
       if (rest >= 0.0f) {
-        const float span = full ? 3e18f : sqrtf(rest);
-        const int x0 = clampc(cx - span), x1 = clampc(cx + span);
-        const uint32_t row = sdim * (uint32_t(yy) + sdim * uint32_t(zz));
+        const float span = full ? 3e18f : sqrtf(rest) * xd;  // in x-cells
+        const int x0 = clampx(cx - span), x1 = clampx(cx + span);
+        const uint32_t row = sdim * xdiv * (uint32_t(yy) + sdim * uint32_t(zz));
         start = offsets[row + x0];
         len = offsets[row + x1 + 1] - start;
       }
@@ -1512,68 +1574,49 @@ __device__ __attribute__((always_inline)) inline void near_body(
     for (uint32_t rr = 0; rr < 64; ++rr) {
       const uint32_t n_row = uint32_t(__builtin_amdgcn_readlane(int(len), int(rr)));
       if (n_row == 0) continue;
-      const uint32_t s_row = uint32_t(__builtin_amdgcn_readlane(int(start), int(rr)));
-      // The row's sites and codes from a uniform base: lane offsets stay 32-bit.
-      const auto* __restrict__ row_sites = sq + s_row;
-      const auto* __restrict__ row_codes = scode + s_row;
-      for (uint32_t c = 0; c < n_row; c += 64) {
-        const uint32_t k = c + lane;
-        float4 d;
-        float dm2 = 8.0f;  // beyond every cutoff: the lanes past the row's end
-        bool own = false;  // the source's own images exert no force (pairs.wgsl:404-408)
-        if (k < n_row) {   // the lanes past the row's end load nothing
-          const float4 y = load4(row_sites + k);
-          d = make_float4(x.x - y.x, x.y - y.y, x.z - y.z, x.w - y.w);
-          dm2 = dot4(d, d);  // chord2(x, y), the same operations
-          if (OWN) {
-            // Every orbit point but the source itself bounds the nearest chord.
-            const uint32_t code = row_codes[k];
-            if (dm2 <= cut.chord2 && code != self_code) nearest_bits = min(nearest_bits, __float_as_uint(dm2));
-            own = (code & 0x7fffffffu) == src;
-          }
-        }
-        const uint64_t mask = __builtin_amdgcn_ballot_w64(dm2 <= cut.chord2 && !own);
-        const bool take = __builtin_amdgcn_inverse_ballot_w64(mask);
-        if (mask) {
-          if (take) {
-            const uint32_t e = (qhead + qcount + mbcnt(mask)) & 127u;
-            L.qe[e] = d;
-          }
-          qcount = __builtin_amdgcn_readfirstlane(qcount + uint32_t(__popcll(mask)));
-          wave_sync();
-          if (qcount >= 64) {
-            // qhead is 0 or 64, so qhead + lane indexes the ring without a wrap.
-            process(qhead + lane);
-            qhead = __builtin_amdgcn_readfirstlane(qhead ^ 64u);
-            qcount -= 64;
-            rounds = __builtin_amdgcn_readfirstlane(rounds + 1);
-            if ((rounds & 63u) == 0) flush();
-            wave_sync();
-          }
-        }
-      }
+      uint32_t at = uint32_t(__builtin_amdgcn_readlane(int(start), int(rr))) * 16u;
+      const uint32_t full_end = at + (n_row & ~63u) * 16u;
+      for (; at < full_end; at += 1024u) chunk(at, ~0ull);
+      if (n_row & 63u) chunk(at, (1ull << (n_row & 63u)) - 1ull);
     }
   }
-  if (lane < qcount) process(qhead + lane);
-  zeros = __builtin_amdgcn_readfirstlane(zeros);
+  // The tail (fewer than 64 entries), its zero chords counted in uniform control flow.
+  const uint32_t qcount = uint32_t(top - &L.stack[0]);
+  {
+    const uint64_t active = (1ull << qcount) - 1ull;
+    float dm2 = L.stack[lane + 512];
+    if (!OWN) {
+      const uint64_t zero = __builtin_amdgcn_ballot_w64(dm2 == 0.0f) & active;
+      zeros += uint32_t(__popcll(zero));
+      dm2 = __builtin_amdgcn_inverse_ballot_w64(zero) ? 2.0f : dm2;
+    }
+    if (lane < qcount) {
+      if (!OWN) nearest_bits = min(nearest_bits, __float_as_uint(dm2));
+      const float4 d = make_float4(L.stack[lane], L.stack[lane + 128], L.stack[lane + 256], L.stack[lane + 384]);
+      near_pair_terms<TERMS>(d, dm2, cut.inv_width, u_offset, slope_pi, ps);
+    }
+  }
   flush();
+  double total[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) total[i] = wave_total(acc[i]);
   uint32_t pairs = 64 * rounds + qcount;
   uint32_t nearest_key = wave_min_u32(nearest_bits);
   if (!OWN) {
     if (zeros > 1) {
-      // Each duplicate of x is a pair at chord 0 (pairs.wgsl:404-408 keeps it): its
-      // terms, clamped by the pair's regularisers, with no force (x - y = 0).
+      // Each duplicate of x is a pair at chord 0 (pairs.wgsl:404-408 keeps it): its terms,
+      // clamped by the pair's regularisers, with no force (x - y = 0).
       NearSum z{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
       near_pair_terms<TERMS>(make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, cut.inv_width, u_offset, slope_pi, z);
-      acc[4] += double(zeros - 1) * double(z.curvature);
-      acc[5] += double(zeros - 1) * double(z.energy);
+      total[4] += double(zeros - 1) * double(z.curvature);
+      total[5] += double(zeros - 1) * double(z.energy);
       nearest_key = 0u;
     }
     if (nearest_key == __float_as_uint(2.0f)) nearest_key = __float_as_uint(4.0f);  // no pair at all
     pairs -= min(zeros, 1u);  // the identity image is no pair
   }
-  const double g[4] = {acc[0], acc[1], acc[2], acc[3]};
-  const double curvature = acc[4], e = acc[5] * kPi64;  // the pairs' energies were summed without pi
+  const double g[4] = {total[0], total[1], total[2], total[3]};
+  const double curvature = total[4], e = total[5] * kPi64;  // the pairs' energies were summed without pi
   const float nearest = __uint_as_float(nearest_key);
   if (lane == 0) {
     double f[3];
@@ -1591,14 +1634,14 @@ __global__ __launch_bounds__(256) void k_near(const float4* __restrict__ q, uint
                                               const uint32_t* __restrict__ order, const float4* __restrict__ sq,
                                               const uint32_t* __restrict__ scode,
                                               const uint32_t* __restrict__ offsets, const NearInfo* __restrict__ info,
-                                              uint32_t sdim, NearCut cut,
+                                              uint32_t sdim, uint32_t xdiv, NearCut cut,
                                               uint32_t real_blocks, Rows rows, uint32_t* __restrict__ counts) {
   // Each XCD takes a contiguous range of the tile-by-tile source order, so its L2 holds
   // that range's sites: 420 MB of HBM traffic per pass at I, N = 1e6, against 1061 MB
   // with the workgroups dealt round-robin over the XCDs (at the same kernel time).
   const uint32_t vb = xcd_remap(blockIdx.x, gridDim.x);
   if (vb >= real_blocks) return;
-  near_body<OWN, TERMS>(q, n, order, sq, scode, offsets, info, sdim, cut, rows, counts, vb);
+  near_body<OWN, TERMS>(q, n, order, sq, scode, offsets, info, sdim, xdiv, cut, rows, counts, vb);
 }
 
 // Sums the per-source pair and candidate counts of a force pass into counters[0..1].
@@ -1628,7 +1671,7 @@ void launch_near_force(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
   p.run("near_force", [&] {
     auto go = [&](auto kernel) {
       hipLaunchKernelGGL(kernel, dim3(xcd_grid(real)), dim3(kBlock), 0, s, q, n, b.order, b.sq, b.scode, b.offsets,
-                         b.info, b.sdim, cut, real, out, b.counts);
+                         b.info, b.sdim, b.xdiv, cut, real, out, b.counts);
     };
     const int terms = atan_terms(cut.chord2);
     if (cut.own) {
@@ -1665,7 +1708,7 @@ uint32_t near_force_key(const NearCut& cut) { return (cut.own ? 8u : 0u) + uint3
 // the 256-job mix).  The visit stages keep one visit per thread: each walks its cell's
 // sites, and fewer threads leave too little parallelism (0.8 -> 1.3 ms per step).
 uint32_t near_stage_blocks(const NearJob& j, uint32_t stage) {
-  const size_t cells = size_t(j.b.sdim) * j.b.sdim * j.b.sdim, visits = tile_visits(j.b.sdim);
+  const size_t cells = cell_count(j.b.sdim, j.b.xdiv), visits = tile_visits(j.b.sdim, j.b.xdiv);
   const uint32_t per = kBlock * 8;
   switch (stage) {
     case kNsFill: return std::min<uint32_t>(4096, blocks_for(cells + 1, per));
@@ -1717,7 +1760,7 @@ __global__ void k_jobs_reset(NearSet set) {
 }
 __global__ void k_jobs_fill_offsets(NearSet set) {
   const JobPart w = near_part(set, kNsFill, blockIdx.x);
-  const size_t count = size_t(w.job->b.sdim) * w.job->b.sdim * w.job->b.sdim + 1;
+  const size_t count = cell_count(w.job->b.sdim, w.job->b.xdiv) + 1;
   for (size_t i = size_t(w.block) * blockDim.x + threadIdx.x; i < count; i += size_t(w.grid) * blockDim.x)
     w.job->b.offsets[i] = 0u;
 }
@@ -1745,12 +1788,12 @@ __global__ void k_jobs_sites(NearSet set, uint32_t stage) {
   else
     site_rank_place_body(w.job->b, w.block, w.grid);
 }
-// Scans of the cell counts (visits 0: offsets over sdim^3 cells) or of the visit counts
-// (visits 1: ucell over tile_visits(sdim)), each in its job's scan_tmp.
+// Scans of the cell counts (visits 0: offsets over the grid's cells) or of the visit counts
+// (visits 1: ucell over tile_visits), each in its job's scan_tmp.
 __device__ inline void scan_of(const NearJob& j, uint32_t visits, uint32_t*& data, size_t& count) {
   const uint32_t d = j.b.sdim;
   data = visits ? j.b.ucell : j.b.offsets;
-  count = visits ? size_t(tile_visits(d)) : size_t(d) * d * d;
+  count = visits ? size_t(tile_visits(d, j.b.xdiv)) : cell_count(d, j.b.xdiv);
 }
 __global__ __launch_bounds__(256) void k_jobs_scan_tiles(NearSet set, uint32_t visits) {
   const JobPart w = near_part(set, visits ? kNsVisitTiles : kNsCellTiles, blockIdx.x);
@@ -1800,7 +1843,7 @@ __global__ __launch_bounds__(256) void k_jobs_near(NearSet set, uint32_t first, 
   if (vb >= blocks) return;
   const JobPart w = near_part(set, kNsForce, first + vb);
   const NearJob& j = *w.job;
-  near_body<OWN, TERMS>(j.q, j.n, j.b.order, j.b.sq, j.b.scode, j.b.offsets, j.b.info, j.b.sdim, j.cut, j.rows,
+  near_body<OWN, TERMS>(j.q, j.n, j.b.order, j.b.sq, j.b.scode, j.b.offsets, j.b.info, j.b.sdim, j.b.xdiv, j.cut, j.rows,
                         j.b.counts, w.block);
 }
 __global__ __launch_bounds__(256) void k_jobs_count_reduce(NearSet set) {
