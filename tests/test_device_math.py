@@ -246,17 +246,18 @@ def kernel_near_pair(x, y, taper_start, inv_width, terms):
     aa, bb = a * a, b * b
     a2, b2 = aa + 1e-9, bb
     a3, b3 = aa * a + 1e-9, bb * b + 1e-9
-    sine = 0.5 * t * fr + 1e-9
-    P1, P2, P3 = a2 * b2 * sine, a3 * b3, a * b
+    sine2 = t * fr + 2e-9  # twice the regularised sine
+    P1, P2, P3 = a2 * b2 * sine2, a3 * b3, a * b
     D = 1 / (P1 * P2 * P3)
-    iP1, iP2, iP3 = D * P2 * P3, D * P1 * P3, D * P1 * P2  # the pair energy is pi iP3
+    E12 = D * P3
+    iP1, iP2, iP3 = E12 * P2, E12 * P1, D * P1 * P2  # the pair energy is pi iP3
     u = min(max((a - taper_start) * inv_width, 0.0), 1.0)
     uu = u * u
     w = uu * (2 * u - 3) + 1
-    slope_pi = -12 * math.pi * inv_width  # per source: pi times twice the slope's factor
-    coef = w * (a2 - b2) * iP1 + iP3 * (slope_pi * (u - uu)) * R
-    # the kernel sums iP3 w and multiplies the source's sum by pi once
-    return direction_M(x, coef * e), (a3 + b3) * iP2 * w, math.pi * (iP3 * w)
+    slope_pi = -6 * math.pi * inv_width  # per source: pi times the slope's factor
+    coef = w * (a2 - b2) * iP1 + iP3 * (slope_pi * (u - uu)) * R  # half the pair's coefficient
+    # the kernel sums iP3 w and multiplies the source's sum by pi once, and doubles g
+    return direction_M(x, 2 * coef * e), (a3 + b3) * iP2 * w, math.pi * (iP3 * w)
 
 
 def test_near_pair_form_is_the_reference_pair():

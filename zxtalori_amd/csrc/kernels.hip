@@ -1489,7 +1489,7 @@ __device__ __attribute__((always_inline)) inline void near_body(
   uint32_t nearest_bits = __float_as_uint(4.0f);
   uint32_t zeros = 0;  // zero chords met (!OWN): counted from ballots in uniform control flow
   const float u_offset = -cut.taper_start * cut.inv_width;
-  const float slope_pi = -12.0f * kPiF * cut.inv_width;
+  const float slope_pi = -6.0f * kPiF * cut.inv_width;  // pi times the slope's factor (g is halved)
   float* top = &L.stack[0];  // the stack's top (uniform): 0 to 63 entries between chunks
   uint32_t rounds = 0;
   uint32_t candidates = 0;  // sites tested (uniform); accepted pairs are 64 rounds + the tail
@@ -1615,8 +1615,9 @@ __device__ __attribute__((always_inline)) inline void near_body(
     if (nearest_key == __float_as_uint(2.0f)) nearest_key = __float_as_uint(4.0f);  // no pair at all
     pairs -= min(zeros, 1u);  // the identity image is no pair
   }
-  const double g[4] = {total[0], total[1], total[2], total[3]};
-  const double curvature = total[4], e = total[5] * kPi64;  // the pairs' energies were summed without pi
+  // The pairs' force coefficients were summed halved and their energies without pi.
+  const double g[4] = {2.0 * total[0], 2.0 * total[1], 2.0 * total[2], 2.0 * total[3]};
+  const double curvature = total[4], e = total[5] * kPi64;
   const float nearest = __uint_as_float(nearest_key);
   if (lane == 0) {
     double f[3];

@@ -148,8 +148,8 @@ __device__ __forceinline__ float atan2x_small(float t) {
 
 // Per-lane sums of near-field pairs.  The force direction of a pair is linear in the
 // site, M(x) y = x0 y_v - y0 x_v + x_v x y_v with M(x) x = 0, so it is also -M(x) (x - y):
-// the kernel sums g += coef (x - y) over the pairs and applies -M(x) once per source
-// (near_force_of), 4 FMAs per pair instead of 15.
+// the kernel sums g += coef (x - y) / 2 over the pairs and applies -2 M(x) once per source
+// (near_force_of of 2 g), 4 FMAs per pair instead of 15.
 struct NearSum {
   float gx, gy, gz, gw, curvature, energy;
 };
@@ -161,10 +161,15 @@ struct NearSum {
 // the window coordinate (a - taper_start) / width is one fused multiply-add,
 // a * inv_width + u_offset with u_offset = -taper_start * inv_width (one rounding instead
 // of two); the pair energy pi / (a b) is summed without its factor pi, which the caller
-// applies once per source in FP64, and rides with the window's slope factor -12 on the
-// per-source constant slope_pi = -12 pi inv_width (twice the slope, times pi); and the
-// slope's u (1 - u) is u - u^2 from the u^2 the window already has.  Each saves one
+// applies once per source in FP64, and rides with the window's slope factor -6 on the
+// per-source constant slope_pi = -6 pi inv_width (the slope, times pi); the slope's
+// u (1 - u) is u - u^2 from the u^2 the window already has; the regularised sine is taken
+// at twice its value, t far + 2e-9 = 2 (sin a + 1e-9), which halves the force coefficient
+// exactly (a power of two scales every rounding alike), and the caller doubles the force
+// sum once per source; and the reciprocals of P1 = a2 b2 sine and P2 = a3 b3 come from the
+// one reciprocal of P1 P2 P3 through 1 / (P1 P2) = P3 / (P1 P2 P3).  Each saves one
 // instruction per pair; the rounding differs from the reference's form in the last bit.
+// g accumulates coef (x - y) / 2.
 template <int TERMS>
 __device__ __forceinline__ void near_pair_terms(const float4 e, float dm2, float inv_width, float u_offset,
                                                 float slope_pi, NearSum& s) {
@@ -179,16 +184,17 @@ __device__ __forceinline__ void near_pair_terms(const float4 e, float dm2, float
   // to b^2 in FP32 (half an ulp of 2.6 is 1.2e-7): the regulariser is dropped exactly.
   const float a2 = aa + 1e-9f, b2 = bb;
   const float a3 = fmaf(aa, a, 1e-9f), b3 = fmaf(bb, b, 1e-9f);
-  const float sine = fmaf(0.5f * t, fr, 1e-9f);
-  const float P1 = a2 * b2 * sine, P2 = a3 * b3, P3 = a * b;
+  const float sine2 = fmaf(t, fr, 2e-9f);  // 2 (sin a + 1e-9)
+  const float P1 = a2 * b2 * sine2, P2 = a3 * b3, P3 = a * b;
   const float P12 = P1 * P2;
   const float D = __builtin_amdgcn_rcpf(P12 * P3);
-  const float iP1 = D * (P2 * P3), iP2 = D * (P1 * P3), iP3 = D * P12;  // energy / pi = iP3
+  const float E12 = D * P3;                                 // 1 / (P1 P2)
+  const float iP1 = E12 * P2, iP2 = E12 * P1, iP3 = D * P12;  // energy / pi = iP3
   const float u = fminf(fmaxf(fmaf(a, inv_width, u_offset), 0.0f), 1.0f);
   const float uu = u * u;
   const float w = fmaf(uu, fmaf(2.0f, u, -3.0f), 1.0f);
-  const float slope2pi = slope_pi * (u - uu);  // pi times twice the window's slope
-  const float coef = fmaf(w, (a2 - b2) * iP1, iP3 * slope2pi * R);
+  const float slope_half = slope_pi * (u - uu);  // pi times the window's slope
+  const float coef = fmaf(w, (a2 - b2) * iP1, iP3 * slope_half * R);
   s.gx = fmaf(coef, e.x, s.gx);
   s.gy = fmaf(coef, e.y, s.gy);
   s.gz = fmaf(coef, e.z, s.gz);
