@@ -370,6 +370,10 @@ def main():
                    "group": args.group, "n": args.n, "neighbourhood": args.delta, "seed": 193},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_from_profiles(),
+                     "traffic_source": "HBM bytes per launch of this kernel at this workload, read from "
+                                       "profiles/near_force_i1e6_traffic.md (rocprofv3 FETCH_SIZE x 2.0 + "
+                                       "WRITE_SIZE x 1.0, factors from a copy of known size), not measured "
+                                       "in this run",
                      "kernel": "near_force", "kernel_ms": kernel_s * 1e3, "pairs_per_launch": pairs,
                      "alg_valu_per_pair": valu, "alg_trans_per_pair": trans,
                      "basis": "frac: speed on the reference's per-pair arithmetic (its VALU and "
