@@ -200,6 +200,10 @@ int32_t zo_batch_add(zxt_batch* b, uint32_t group, const float* q, uint32_t n, u
 /* One step of every running job (pending measure and spacing jobs run first); completed
  * receives how many jobs ended (done or failed) in it. */
 int32_t zo_batch_step(zxt_batch* b, uint32_t* completed);
+/* As zo_batch_step, but steps that end no job may run as one launch of a graph of up to
+ * `most` steps (graph replay on, the same jobs running; at most 64): taken receives how
+ * many steps every running job advanced (1 when no such span applies). */
+int32_t zo_batch_advance(zxt_batch* b, uint32_t most, uint32_t* completed, uint32_t* taken);
 /* 0 running, 1 done, 2 failed (zo_batch_job_error), -1 unknown id. */
 int32_t zo_batch_status(zxt_batch* b, uint64_t id);
 const char* zo_batch_job_error(zxt_batch* b, uint64_t id);

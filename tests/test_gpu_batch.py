@@ -77,6 +77,36 @@ def expected():
     return out
 
 
+# Jobs with long steady runs between their ends: steps that end no job run as one launch
+# of a graph of several steps (Batch.advance).
+SPAN_MIX = [
+    ("O", 1500, 3, "relax", 20, 0.0),
+    ("I", 12000, 7, "relax", 20, 20.0),
+    ("D3", 8000, 1, "relax", 10, 20.0),
+    ("C4", 900, 2, "relax", 14, 0.0),
+]
+
+
+@pytest.mark.parametrize("streams", [0, 2])
+def test_spans_of_steps_equal_single_steps(streams):
+    """Spans of steps replayed as one graph launch give the positions, stats and per-point
+    values of each job relaxed alone (bit for bit), as single steps do."""
+    note("batch spans, streams", streams)
+    with Batch(graphs=True, streams=streams) as batch:
+        jobs = [batch.add(g, count=n, seed=s, kind=k, steps=t, neighbourhood=d) for g, n, s, k, t, d in SPAN_MIX]
+        taken, calls = [], 0
+        while batch.running:
+            _, steps = batch.advance(64)
+            taken.append(steps)
+            calls += 1
+            assert calls <= 40
+        assert sum(taken) == 20 and max(taken) > 1, taken
+        for spec, job in zip(SPAN_MIX, jobs):
+            check_same(batch, job, alone(*spec), spec)
+        captured, replayed = batch.graphs
+        assert replayed >= max(taken), (captured, replayed, taken)
+
+
 @pytest.mark.parametrize("graphs,streams", [(False, 0), (True, 0), (False, 3), (True, 3)])
 def test_alone_equals_in_mix(expected, graphs, streams):
     note("batch mix, graphs", graphs, "streams", streams)

@@ -42,7 +42,7 @@ zo_point_indices e U;  zo_force_passes eu i;  zo_near_info eU i;  zo_force_rows 
 zo_near_pairs eQ i;  zo_near_cut eF i;  zo_near_squeeze ei i;  zo_near_resizes e u
 zo_hopf_points uupu u;  zo_hopf_frequency uu u;  zo_issue_probe uup i
 zo_batch_create u e;  zo_batch_destroy e v;  zo_batch_error e s;  zo_batch_add eupuufuuQ i
-zo_batch_step eU i;  zo_batch_status eq i;  zo_batch_job_error eq s;  zo_batch_stats eq s
+zo_batch_step eU i;  zo_batch_advance euUU i;  zo_batch_status eq i;  zo_batch_job_error eq s;  zo_batch_stats eq s
 zo_batch_quaternions eq F;  zo_batch_count eq u;  zo_batch_point_values equ F
 zo_batch_point_indices eq U;  zo_batch_bytes eq q;  zo_batch_remove eq i;  zo_batch_running e u
 zo_batch_wait euU i;  zo_batch_graphs eQ i;  zo_batch_near_squeeze eqi i;  zo_batch_near_resizes eq u

@@ -88,10 +88,20 @@ class Batch:
         self._check(self._lib.zo_batch_step(self._handle, C.byref(done)))
         return int(done.value)
 
-    def run(self):
-        """Steps until no job is running."""
+    def advance(self, most=64):
+        """Steps every running job: up to `most` steps that end no job go as one launch of a
+        captured graph (with graph replay on and the same jobs running).  Returns (jobs
+        ended, steps taken)."""
+        self._open()
+        done, taken = C.c_uint32(0), C.c_uint32(0)
+        self._check(self._lib.zo_batch_advance(self._handle, int(most), C.byref(done), C.byref(taken)))
+        return int(done.value), int(taken.value)
+
+    def run(self, most=64):
+        """Steps until no job is running, spans of steps that end no job as single graph
+        launches."""
         while self.running:
-            self.step()
+            self.advance(most)
         return self
 
     def wait(self, most=1):

@@ -1,5 +1,6 @@
 // The zo_* C ABI (include/zxtalori.h): the reference's export table
 // (src/exports.zig:4-120, src/native_api.zig:9-55) over the HIP engine.
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -299,6 +300,15 @@ int32_t zo_batch_step(zxt_batch* b, uint32_t* completed) {
   return guarded_batch(b, "Batch step", [&] {
     const uint32_t done = b->batch.step();
     if (completed) *completed = done;
+  });
+}
+
+int32_t zo_batch_advance(zxt_batch* b, uint32_t most, uint32_t* completed, uint32_t* taken) {
+  return guarded_batch(b, "Batch advance", [&] {
+    uint32_t steps = 0;
+    const uint32_t done = b->batch.step(std::max<uint32_t>(most, 1), &steps);
+    if (completed) *completed = done;
+    if (taken) *taken = steps;
   });
 }
 

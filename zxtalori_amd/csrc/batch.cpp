@@ -33,6 +33,7 @@ Batch::~Batch() {
   jobs_.clear();
   CaptureLock lock(false);
   if (graph_) (void)hipGraphExecDestroy(graph_);
+  if (span_) (void)hipGraphExecDestroy(span_);
   for (const Retired& r : retired_) {
     (void)hipGraphExecDestroy(r.exec);
     (void)hipEventDestroy(r.done);
@@ -214,7 +215,8 @@ void Batch::finish(Job& j, bool recorded) {
   }
 }
 
-uint32_t Batch::step() {
+uint32_t Batch::step(uint32_t most, uint32_t* taken_out) {
+  if (taken_out) *taken_out = 0;
   reap_graphs();
   uint32_t completed = 0;
   // Measure and spacing jobs run as they come (each ends with a wait).
@@ -308,52 +310,87 @@ uint32_t Batch::step() {
     }
     launch_bb_step(prof, stream, step_set);  // move and fold included
   };
+  // Captures `steps` recorded steps into one executable graph; null when a job table
+  // changed during the capture (nothing captured has run: the caller records eagerly).
+  auto capture = [&](uint32_t steps) -> hipGraphExec_t {
+    hipGraph_t g = nullptr;
+    bool changed = false;
+    hipError_t ended;
+    {
+      CaptureLock lock(true);
+      Engine::check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
+      capturing_ = true;
+      try {
+        for (uint32_t k = 0; k < steps; ++k) record();
+      } catch (const GraphTableChanged&) {
+        changed = true;
+      } catch (...) {
+        capturing_ = false;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      capturing_ = false;
+      ended = hipStreamEndCapture(stream, &g);
+    }
+    if (changed) {
+      if (g) (void)hipGraphDestroy(g);
+      Engine::check(ended, "capture");
+      return nullptr;
+    }
+    Engine::check(ended, "capture");
+    hipGraphExec_t exec = nullptr;
+    hipError_t made;
+    {
+      // Instantiation allocates the graph's argument memory and the streams of its
+      // parallel branches: like any allocation, not while another thread captures.
+      CaptureLock lock(false);
+      made = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+    }
+    Engine::check(made, "graph");
+    ++captures_;
+    return exec;
+  };
+  uint32_t taken = 1;
   try {
     if (steady && graph_ && graph_key_ == key) {
-      Engine::check(hipGraphLaunch(graph_, stream), "graph");
-      ++replays_;
-    } else if (steady) {
-      retire_graph();
-      graph_key_.clear();
-      hipGraph_t g = nullptr;
-      bool changed = false;
-      hipError_t ended;
-      {
-        CaptureLock lock(true);
-        Engine::check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
-        capturing_ = true;
-        try {
+      // Steps that end no job (each job's last step finishes it on the host) can run as one
+      // launch of a graph of `span` steps: the same launches, recorded back to back.
+      uint32_t span = std::min(most, kMaxSpan);
+      for (Job* j : active) span = std::min(span, j->remaining - 1);
+      if (span >= 2) {
+        if (!span_ || span_key_ != key || span_steps_ != span) {
+          retire_graph(span_);
+          span_key_.clear();
+          span_ = capture(span);
+          if (span_) {
+            span_key_ = key;
+            span_steps_ = span;
+          }
+        }
+        if (span_) {
+          Engine::check(hipGraphLaunch(span_, stream), "graph");
+          replays_ += span;
+          taken = span;
+        } else {
           record();
-        } catch (const GraphTableChanged&) {
-          changed = true;
-        } catch (...) {
-          capturing_ = false;
-          (void)hipStreamEndCapture(stream, &g);
-          if (g) (void)hipGraphDestroy(g);
-          throw;
         }
-        capturing_ = false;
-        ended = hipStreamEndCapture(stream, &g);
-      }
-      if (changed) {
-        // Nothing captured has run: record this step eagerly, capture the next one.
-        if (g) (void)hipGraphDestroy(g);
-        Engine::check(ended, "capture");
-        record();
       } else {
-        Engine::check(ended, "capture");
-        hipError_t made;
-        {
-          // Instantiation allocates the graph's argument memory and the streams of its
-          // parallel branches: like any allocation, not while another thread captures.
-          CaptureLock lock(false);
-          made = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
-          (void)hipGraphDestroy(g);
-        }
-        Engine::check(made, "graph");
+        Engine::check(hipGraphLaunch(graph_, stream), "graph");
+        ++replays_;
+      }
+    } else if (steady) {
+      retire_graph(graph_);
+      retire_graph(span_);
+      graph_key_.clear();
+      span_key_.clear();
+      graph_ = capture(1);
+      if (graph_) {
         graph_key_ = key;
         Engine::check(hipGraphLaunch(graph_, stream), "graph");
-        ++captures_;
+      } else {
+        record();  // capture the next step
       }
     } else {
       record();
@@ -367,10 +404,12 @@ uint32_t Batch::step() {
     return completed + uint32_t(active.size());
   }
   mark_step();
+  if (taken_out) *taken_out = taken;
   std::vector<Job*> ending;
   for (Job* j : active) {
-    j->engine->stepped();
-    if (--j->remaining == 0) ending.push_back(j);
+    for (uint32_t k = 0; k < taken; ++k) j->engine->stepped();
+    j->remaining -= taken;
+    if (j->remaining == 0) ending.push_back(j);
   }
   finish_all(ending);
   return completed + uint32_t(ending.size());
@@ -487,12 +526,12 @@ void Batch::mark_step() {
 
 // Called outside capture only: the event marks the end of everything the stream holds,
 // the graph's last launch included.
-void Batch::retire_graph() {
-  if (!graph_) return;
+void Batch::retire_graph(hipGraphExec_t& g) {
+  if (!g) return;
   hipEvent_t ev = take_event();
   Engine::check(hipEventRecord(ev, stream), "event");
-  retired_.push_back({graph_, ev});
-  graph_ = nullptr;
+  retired_.push_back({g, ev});
+  g = nullptr;
 }
 
 void Batch::reap_graphs() {

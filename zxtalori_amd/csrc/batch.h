@@ -34,8 +34,10 @@ class Batch {
                uint32_t steps);
   // One BB step for every running relax job, then the jobs that reached their steps are
   // finished (one wait each) and pending measure/spacing jobs run.  Returns how many jobs
-  // completed.
-  uint32_t step();
+  // completed.  With most > 1, steps that end no job may go as one launch of a graph of up
+  // to `most` steps (at most kMaxSpan); *taken receives how many steps were taken.
+  uint32_t step(uint32_t most = 1, uint32_t* taken = nullptr);
+  static constexpr uint32_t kMaxSpan = 64;
   // 0 running, 1 done, 2 failed (message in error(id)); throws for an unknown id.
   int status(uint64_t id) const;
   std::string stats(uint64_t id);
@@ -50,7 +52,7 @@ class Batch {
   // `most` are left (a serving loop keeps each batch at most a step or two ahead).
   uint32_t in_flight();
   void wait(uint32_t most);
-  // Steps captured into a graph, and steps replayed from one.
+  // Graphs captured, and steps replayed from one.
   uint64_t captures() const { return captures_; }
   uint64_t replays() const { return replays_; }
 
@@ -81,7 +83,7 @@ class Batch {
   hipEvent_t take_event();
   // A replaced graph is destroyed only after the stream has passed its last launch: HIP
   // does not promise to keep an executable graph alive while a launch of it still runs.
-  void retire_graph();
+  void retire_graph(hipGraphExec_t& g);
   void reap_graphs();
   struct Retired {
     hipGraphExec_t exec;
@@ -116,8 +118,11 @@ class Batch {
   std::vector<hipStream_t> aux_;
   std::vector<hipEvent_t> join_;
   hipEvent_t fork_ = nullptr;
-  hipGraphExec_t graph_ = nullptr;
+  hipGraphExec_t graph_ = nullptr;  // one step
   std::vector<uint64_t> graph_key_;
+  hipGraphExec_t span_ = nullptr;  // span_steps_ steps, for steps that end no job
+  std::vector<uint64_t> span_key_;
+  uint32_t span_steps_ = 0;
   std::vector<uint64_t> last_key_;
 };
 

@@ -92,16 +92,47 @@ def mixed_relax_jobs(total=256, seed=193, count_range=(500, 200_000), steps=50, 
     return out
 
 
-def run_batched(jobs, *, graphs=True, keep_results=False, clock=time.perf_counter):
-    """All jobs admitted at once and stepped to the end in one batch (no memory budget).
+def run_batched(jobs, *, graphs=True, keep_results=False, clock=time.perf_counter, span=64):
+    """All jobs admitted at once and stepped to the end in one batch (no memory budget, no
+    lanes: with every job admitted at the start, nothing waits on a costlier job's steps).
 
-    Lanes keep a cheap request from waiting on costlier ones' steps; with every job
-    admitted at the start there is no such wait to spare, and one batch runs each build
-    stage and force variant of all near-field jobs as one launch."""
-    with Server(hbm_budget=1 << 62, graphs=graphs, keep_results=keep_results, bounds=(math.inf,),
-                near_streams=2) as server:
-        outcomes, stats = server.run(jobs, realtime=False, clock=clock)
-        stats["graph_captures"], stats["graph_replays"] = server.graphs()
+    Each build stage and force variant of all near-field jobs is one launch, and the steps
+    that end no job run as one launch of a captured graph of up to `span` steps
+    (Batch.advance).  A job's latency runs from the start to the end of the step that
+    finished it."""
+    outcomes = [None] * len(jobs)
+    with Batch(graphs=graphs, streams=2) as batch:
+        start = clock()
+        ids = [batch.add(req.group, count=req.count, seed=req.seed, kind=req.kind, steps=req.steps,
+                         neighbourhood=req.neighbourhood) for req in jobs]
+        open_ = set(range(len(jobs)))
+
+        def collect():
+            t = clock() - start
+            for index in sorted(open_):
+                state = batch.status(ids[index])
+                if state == "running":
+                    continue
+                req = jobs[index]
+                out = Outcome(req, state, batch.error(ids[index]) if state == "failed" else "", 0.0, t)
+                if state == "done":
+                    out.stats = batch.stats(ids[index])
+                    if keep_results:
+                        out.quaternions = batch.quaternions(ids[index])
+                        out.separation = batch.point_values(ids[index], "separation")
+                        if req.kind == "spacing":
+                            out.indices = batch.point_indices(ids[index])
+                outcomes[index] = out
+                open_.discard(index)
+
+        while batch.running:
+            ended, _ = batch.advance(span)
+            if ended:
+                collect()
+        batch.wait(0)
+        collect()
+        stats = summary(outcomes, clock() - start)
+        stats["graph_captures"], stats["graph_replays"] = batch.graphs
     return outcomes, stats
 
 
