@@ -30,6 +30,12 @@ constexpr double kFullMeasurePairs = 5e10;    // engine.zig:1123
 constexpr uint32_t kSpacingAllPairs = 20000;  // engine.zig:1126
 constexpr double kSpacingDelta = 4;           // engine.zig:1127
 constexpr double kSpacingCellSites = 0.75;    // sites per cell of the spacing pass's grid
+// and its shape: y/z cells 1.75 times that edge, 4 x-cells each (fewer rows per cap, trimmed
+// closer; about as many cells).  Measured on MI355X, I, N = 1e7, whole pass, same box:
+// cubic 3.47 ms; 1.75 x 4 3.32; 2.0 x 8 3.34; 1.5 x 4 3.37; 2.0 x 16 3.53; 2.5 x 16 3.62;
+// 1.0 x 4 3.65 (more cells: the search is faster, the scans and source order slower).
+constexpr uint32_t kSpacingXSplit = 4;
+constexpr double kSpacingEdge = 1.75;
 // Sites of one near field: the force kernel addresses them by 32-bit byte offsets
 // (16 B each, 64 entries of slack past the capacity).
 constexpr uint64_t kMaxSites = (uint64_t(1) << 28) - 64;
@@ -243,10 +249,9 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   // 0.5, 4.40 with 1.0, 4.53 with 1.6, 4.83 with 3).
   const double reach = radius / mean_radius_;
   const double per_cell = cell_sites > 0 ? cell_sites : std::min(8.0, std::max(1.5, reach / 2.5));
-  xdiv = cell_sites > 0 ? 1u : x_split;
-  const double edge = xdiv > 1 ? kForceEdge : 1.0;
+  xdiv = x_split;
   const double max_dim = std::floor(320.0 / std::cbrt(double(xdiv)));  // at most 320^3 cells
-  sdim = uint32_t(std::min(max_dim, std::max(1.0, std::round(std::cbrt(double(count) / per_cell) / edge))));
+  sdim = uint32_t(std::min(max_dim, std::max(1.0, std::round(std::cbrt(double(count) / per_cell) / cell_edge))));
   const size_t cells = cell_count(sdim, xdiv);
   uq.ensure(size_t(capacity) * 16);
   ucode.ensure(size_t(capacity) * 4);
@@ -590,6 +595,7 @@ void Engine::set_neighbourhood(double delta) {
 void Engine::sync_neighbourhood() {
   relax_state_.set_near(group_, neighbourhood_ > 0 ? &neighbourhood_ : nullptr);
   relax_state_.near.x_split = kForceXSplit;
+  relax_state_.near.cell_edge = kForceEdge;
   measure_neighbourhood_ = neighbourhood_;
 }
 
@@ -718,7 +724,6 @@ double Engine::nearest_pass(RelaxState& rs) {
       // The nearest orbit point is a minimum over keys: the sites' order within a cell
       // does not matter.
       rs.near.build(*this, q, n_, order(), group_, mean_radius(uint64_t(n_) * order()), false);
-      if (rs.near.xdiv != 1) fail("GpuError: the spacing walk takes cubic cells");
       launch_near_nearest(prof, stream, n_, order(), rs.near.view(), rs.near.cut, degrees, rs.nn_rank.as<uint32_t>(),
                           least_bits);
     } else {
@@ -749,6 +754,8 @@ void Engine::spacing() {
   const double delta = kSpacingDelta;
   spacing_state_.set_near(group_, n_ > kSpacingAllPairs ? &delta : nullptr);
   spacing_state_.near.cell_sites = kSpacingCellSites;
+  spacing_state_.near.x_split = kSpacingXSplit;
+  spacing_state_.near.cell_edge = kSpacingEdge;
   const double least = nearest_pass(spacing_state_);
   if (measured_ && measure_iteration_ == iteration_) {
     measures_.separation_deg = least;
@@ -1438,6 +1445,8 @@ void Engine::optimize_covering(uint32_t steps, uint32_t mode, bool finish, float
     const double delta = kSpacingDelta;
     spacing_state_.set_near(group_, n_ > kSpacingAllPairs ? &delta : nullptr);
     spacing_state_.near.cell_sites = kSpacingCellSites;
+    spacing_state_.near.x_split = kSpacingXSplit;
+    spacing_state_.near.cell_edge = kSpacingEdge;
     return nearest_pass(spacing_state_) >= double(floor_deg);
   };
   uint64_t rng = 0x9E3779B97F4A7C15ull ^ (uint64_t(opt_.kicks + 1) << 17);

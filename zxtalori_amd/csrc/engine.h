@@ -120,9 +120,11 @@ struct NearField {
   double radius = 0;
   NearCut cut{};
   uint32_t capacity = 0, sdim = 1, nplanes = 0;
-  // x-cells per y/z cell: x_split asked for (force passes), xdiv the grid's (1 when
-  // cell_sites is set: the spacing and covering walks assume cubic cells).
+  // Cell shape the owner asks for: y/z cells cell_edge times the edge of a cube holding
+  // the chosen sites per cell, split into x_split x-cells each (1 and 1: cubic cells, as
+  // the covering walk takes them); xdiv is the grid's x-cells per y/z cell.
   uint32_t x_split = 1, xdiv = 1;
+  double cell_edge = 1.0;
   float band = 0;
   // The last pass's sites, shell and overflow flag (valid after a wait).
   NearInfo status{};

@@ -1917,8 +1917,8 @@ __global__ __launch_bounds__(256) void k_spacing(uint32_t n, const uint32_t* __r
                                                  const float4* __restrict__ sq,
                                                  const uint32_t* __restrict__ srank,
                                                  const uint32_t* __restrict__ offsets,
-                                                 const NearInfo* __restrict__ info, uint32_t sdim, NearCut cut,
-                                                 uint32_t two_order, uint32_t real_blocks,
+                                                 const NearInfo* __restrict__ info, uint32_t sdim, uint32_t xdiv,
+                                                 NearCut cut, uint32_t two_order, uint32_t real_blocks,
                                                  const float4* __restrict__ opos, uint64_t* __restrict__ key_out) {
   constexpr uint32_t G = kSpacingLanes;
   // Each XCD takes a contiguous range of the tile-by-tile source order (as in k_near), so
@@ -1931,9 +1931,10 @@ __global__ __launch_bounds__(256) void k_spacing(uint32_t n, const uint32_t* __r
   const uint32_t src = order[slot];
   const float4 x = opos[slot];
   const float cpu = info->cpu;
-  const float fdim = float(sdim);
+  const float fdim = float(sdim), xd = float(xdiv), fdx = float(sdim * xdiv);
   const float lower0 = info->lower[0], lower1 = info->lower[1], lower2 = info->lower[2];
   auto clampc = [&](float v) -> int { return int(fminf(fmaxf(floorf(v), 0.0f), fdim - 1.0f)); };
+  auto clampx = [&](float v) -> int { return int(fminf(fmaxf(floorf(v), 0.0f), fdx - 1.0f)); };
   const uint32_t self_rank = src * two_order;
   uint64_t best = ~uint64_t(0);
   // Every site in the cells the chart ball of the cap {y : x.y >= c0} (radius s0 / (c0 + w))
@@ -1941,10 +1942,11 @@ __global__ __launch_bounds__(256) void k_spacing(uint32_t n, const uint32_t* __r
   auto scan = [&](float c0, float s0) {
     const float denom = c0 + x.x;
     const bool full = !(denom > 1e-3f);
-    float cx = 0.5f * fdim, cy = 0.5f * fdim, cz = 0.5f * fdim, rc = 3e18f;
+    // Chart position in cell units: y and z in y/z cells, x in x-cells (xdiv per y/z cell).
+    float cx = 0.5f * fdx, cy = 0.5f * fdim, cz = 0.5f * fdim, rc = 3e18f;
     if (!full) {
       const float inv = 1.0f / denom;
-      cx = (x.y * inv - lower0) * cpu;
+      cx = (x.y * inv - lower0) * (cpu * xd);
       cy = (x.z * inv - lower1) * cpu;
       cz = (x.w * inv - lower2) * cpu;
       rc = (s0 * inv * 1.001f + 1e-6f) * cpu + 1e-3f;
@@ -1960,9 +1962,9 @@ __global__ __launch_bounds__(256) void k_spacing(uint32_t n, const uint32_t* __r
         const float gy = row_gap(cy, yy, sdim), gz = row_gap(cz, zz, sdim);
         const float rest = rc2 - gy * gy - gz * gz;
         if (rest >= 0.0f) {
-          const float span = full ? 3e18f : sqrtf(rest);
-          const int x0 = clampc(cx - span), x1 = clampc(cx + span);
-          const uint32_t row = sdim * (uint32_t(yy) + sdim * uint32_t(zz));
+          const float span = full ? 3e18f : sqrtf(rest) * xd;  // in x-cells
+          const int x0 = clampx(cx - span), x1 = clampx(cx + span);
+          const uint32_t row = sdim * xdiv * (uint32_t(yy) + sdim * uint32_t(zz));
           start = offsets[row + x0];
           len = offsets[row + x1 + 1] - start;
         }
@@ -2046,7 +2048,7 @@ void launch_near_nearest(Prof& p, hipStream_t s, uint32_t n, uint32_t order, con
   uint64_t* keys = reinterpret_cast<uint64_t*>(b.counts);  // 2n words: one key per source
   p.run("near_nearest", [&] {
     hipLaunchKernelGGL(k_spacing, dim3(xcd_grid(real)), dim3(kBlock), 0, s, n, b.order, b.sq, b.srank, b.offsets, b.info,
-                       b.sdim, cut, 2 * order, real, b.opos, keys);
+                       b.sdim, b.xdiv, cut, 2 * order, real, b.opos, keys);
   });
   launch_fill_u32(p, s, least_bits, 0xffffffffu, 1);
   p.run("degrees", [&] {
