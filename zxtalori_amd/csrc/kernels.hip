@@ -1419,7 +1419,8 @@ __device__ inline uint32_t buffer_load1(__amdgpu_buffer_rsrc_t r, uint32_t lane_
 // windowed Quey pair over every orbit point within the cutoff, outside its own orbit.
 //
 // The cutoff cap is a ball in the chart; up to 64 of the cell rows it crosses are set up
-// at once, one per lane, each trimmed in x to the ball.  The wave then walks them row by
+// at once, one per lane, each trimmed in x to the ball (to x-cells xdiv times finer than
+// the y/z cells, so a row ends close to the ball).  The wave then walks them row by
 // row (uniform): a row's sites are contiguous, so lane l tests the site row start + c + l,
 // loaded at a uniform byte offset (a scalar register, advanced by 1 KiB per chunk) plus the
 // lane's fixed offset.  A row's full 64-site chunks need no lane mask; its last, partial
@@ -1429,7 +1430,8 @@ __device__ inline uint32_t buffer_load1(__amdgpu_buffer_rsrc_t r, uint32_t lane_
 // row's last chunk runs partly empty.  Per-lane FP32 sums go into per-lane FP64 sums every
 // 64 rounds, and the wave's totals are taken once at the end.  Scalar work per chunk is
 // kept to the loop, the push and the stack count: the scalar unit is shared by the CU's
-// four SIMDs and runs near its issue limit here.
+// four SIMDs, and scalar row masks built for every chunk took it to 79 % of its issue
+// rate, cancelling the vector instructions they saved.
 //
 // OWN: whether the source's own orbit can lie within the cutoff.  x.g is at S3 angle
 // acos|g.w| from x for every x, so below the smallest such angle (0.63 rad for I) no own
