@@ -201,7 +201,7 @@ void NearField::setup(Engine& e, const float4* q, uint32_t n_, uint32_t order_, 
   }
   // Nearest planes first (by their distance from the identity, the normal's scalar part):
   // the rim and shell tests then stop as soon as the remaining planes are provably out of
-  // reach (k_core_rim, shell_image).  The tests are "any" and "all", so order only
+  // reach (k_core_rim, shell_planes).  The tests are "any" and "all", so order only
   // changes when they stop.
   std::stable_sort(pl.begin(), pl.end(), [](const float4& a, const float4& b) { return a.x < b.x; });
   nplanes = uint32_t(pl.size());

@@ -824,6 +824,31 @@ __device__ inline void box_reduce_commit(NearInfo* info, uint32_t lo[3], uint32_
   }
 }
 
+// Slots for a workgroup's items from one atomic on counter: returns each thread's first
+// slot for its mine items (in thread order).  Every thread of the workgroup calls it.
+// Atomics on one address from every workgroup of the chip serialise (the counter is
+// shared by all XCDs), so one per workgroup rather than one per wave.
+__device__ inline uint32_t block_reserve(uint32_t* counter, uint32_t mine) {
+  __shared__ uint32_t part[kBlock / 64 + 1];
+  const uint32_t wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  const uint32_t incl = wave_incl_scan(mine);
+  if ((threadIdx.x & 63) == 63) part[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < waves; ++w) {
+      const uint32_t t = part[w];
+      part[w] = total;
+      total += t;
+    }
+    part[kBlock / 64] = total ? atomicAdd(counter, total) : 0u;
+  }
+  __syncthreads();
+  const uint32_t at = part[kBlock / 64] + part[wave] + incl - mine;
+  __syncthreads();  // part is reused by the next call
+  return at;
+}
+
 // |q - e|, the chord from the identity, without the cancellation of sqrt(2 - 2 q.w).
 __device__ inline float identity_chord(const float4 q) {
   const float d0 = q.x - 1.0f;
@@ -888,17 +913,8 @@ __device__ __attribute__((always_inline)) inline void core_rim_body(
     }
     rim_mask |= uint32_t(rim) << r;
   }
-  // One append per wave for all its rounds (a counter shared by every wave would
-  // otherwise serialise one atomic per wave per round).
-  const uint32_t mine = uint32_t(__popc(rim_mask));
-  const uint32_t incl = wave_incl_scan(mine);
-  const uint32_t total = uint32_t(__builtin_amdgcn_readlane(int(incl), 63));
-  uint32_t base = 0;
-  if (total) {
-    if ((threadIdx.x & 63) == 63) base = atomicAdd(&b.info->rim, total);
-    base = uint32_t(__builtin_amdgcn_readlane(int(base), 63));
-  }
-  uint32_t at = base + incl - mine;
+  // One append per workgroup for all its rounds.
+  uint32_t at = block_reserve(&b.info->rim, uint32_t(__popc(rim_mask)));
   for (uint32_t m = rim_mask; m; m &= m - 1) b.rim[at++] = uint32_t(__ffs(m) - 1) * stride + first;
   box_reduce_commit(b.info, lo, hi);
   block_min_commit(&b.info->wmin_key, wmin);
@@ -916,19 +932,21 @@ static uint32_t core_rim_blocks(uint32_t n) {
   return std::max<uint32_t>(512, blocks_for(n, kBlock * 32));
 }
 
-// Whether image img (1..2|K|-1: operator img/2, negated when odd) of x is a shell site;
-// the image in y.  A site matters only within angle r of some source, so within
-// acos(min source w) + r of the identity: its scalar part is at least w_floor (a cheap,
-// exact necessary condition that spares most images the normalisation and plane tests).
-// Then the reference's test: within the band of every zone halfspace (shell.wgsl:19-22).
-__device__ inline bool shell_image(const float4 x, uint32_t img, const float4* __restrict__ ops,
-                                   const float4* __restrict__ planes, uint32_t nplanes, float band, float w_floor,
-                                   float4& y) {
+// The scalar part of x op (the first component of image(x, op, k)): the images x op_k and
+// -(x op_k) (images 2k and 2k + 1) have scalar parts w and -w.
+__device__ inline float image_scalar(const float4 x, const float4 op) {
+  return fmaf(-x.w, op.w, fmaf(-x.z, op.z, fmaf(-x.y, op.y, x.x * op.x)));
+}
+
+// Whether image img (1..2|K|-1: operator img/2, negated when odd) of x, whose scalar part
+// already clears w_floor, is a shell site: the reference's test, within the band of every
+// zone halfspace (shell.wgsl:19-22).  A site matters only within angle r of some source,
+// so within acos(min source w) + r of the identity: its scalar part is at least w_floor, a
+// cheap, exact necessary condition that spares most images this test.
+__device__ inline bool shell_planes(const float4 x, uint32_t img, const float4* __restrict__ ops,
+                                    const float4* __restrict__ planes, uint32_t nplanes, float band) {
   const uint32_t k = img >> 1;
-  const float4 op = ops[k];
-  const float w = fmaf(-x.w, op.w, fmaf(-x.z, op.z, fmaf(-x.y, op.y, x.x * op.x)));
-  if (((img & 1u) ? -w : w) < w_floor) return false;
-  y = image(x, op, k);
+  float4 y = image(x, ops[k], k);
   if (img & 1u) y = neg4(y);
   if (!(y.x > -0.9999999f)) return false;
   // Planes nearest first: once y.p >= p.w - |y - e| clears -band (with 1e-5 to spare),
@@ -942,47 +960,68 @@ __device__ inline bool shell_image(const float4 x, uint32_t img, const float4* _
   return true;
 }
 
+// Of the images first + b for the set bits b of cand, those that are shell sites, as bits.
+// Each lane walks its own candidates: the loop runs as often as the lane with the most.
+__device__ inline uint32_t shell_keep(const float4 x, uint32_t first, uint32_t cand, const float4* __restrict__ ops,
+                                      const float4* __restrict__ planes, uint32_t nplanes, float band) {
+  uint32_t keep = 0;
+  for (; cand; cand &= cand - 1) {
+    const uint32_t bit = uint32_t(__builtin_ctz(cand));
+    if (shell_planes(x, first + bit, ops, planes, nplanes, band)) keep |= 1u << bit;
+  }
+  return keep;
+}
+
 // Images (k, sign) != (identity, +) of the rim sources that are shell sites, appended
-// after the core (shell.wgsl:97-130).  One thread per rim source: it counts its sites, the
-// wave reserves room for all of its lanes' sites with one atomic, then each thread writes
-// its own.  The append order varies; the cell sort's in-cell key order makes the result
-// deterministic.
+// after the core (shell.wgsl:97-130).  One thread per rim source, in two passes over its
+// images: the scalar-part test for all of them at once (no divergence: a rim source has
+// few images that pass it, but some lane of the wave almost always does, so testing the
+// planes image by image ran the plane test for nearly every image), then the plane test
+// for each lane's own candidates.  It counts its sites, the workgroup reserves room for
+// all of its threads' sites with one atomic, then each thread writes its own.  The append
+// order varies; the cell sort's in-cell key order makes the result deterministic.
 __device__ __attribute__((always_inline)) inline void shell_body(
     const float4* __restrict__ q, uint32_t n, const float4* __restrict__ ops, uint32_t order,
     const float4* __restrict__ planes, uint32_t nplanes, float band, NearBuffers b, uint32_t block, uint32_t grid) {
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   const uint32_t m = b.info->rim;
-  const uint32_t per = 2 * order - 1;
   const float w_floor =
       cosf(fminf(3.14159265f, acosf(fminf(1.0f, unordered_key(b.info->wmin_key))) + b.reach)) - 1e-6f;
   const uint32_t stride = grid * blockDim.x;
   const uint32_t rounds = (m + stride - 1) / stride;
-  const uint32_t lane = lane_id();
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint32_t t = r * stride + block * blockDim.x + threadIdx.x;
     const bool live = t < m;
     const uint32_t src = live ? b.rim[t] : 0u;
     const float4 x = live ? q[src] : make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-    uint32_t mine = 0;
-    float4 y;
-    // The passing images, as bits (per <= 119 < 128): the writing pass recomputes only
-    // their positions, in the same order, without the tests.
+    // The passing images, as bits of four words (image img is bit img & 31 of word
+    // img >> 5; per <= 119 < 128): the writing pass recomputes only their positions, in
+    // the same order, without the tests.
     uint32_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
-    if (live)
-      for (uint32_t img = 1; img <= per; ++img)
-        if (shell_image(x, img, ops, planes, nplanes, band, w_floor, y)) {
-          ++mine;
-          const uint32_t bit = 1u << (img & 31u), word = img >> 5;
-          m0 |= word == 0 ? bit : 0u;
-          m1 |= word == 1 ? bit : 0u;
-          m2 |= word == 2 ? bit : 0u;
-          m3 |= word == 3 ? bit : 0u;
-        }
-    const uint32_t incl = wave_incl_scan(mine);
-    const uint32_t total = __shfl(incl, 63);
-    uint32_t base = 0;
-    if (lane == 63 && total) base = atomicAdd(&b.info->sites, total);
-    base = __shfl(base, 63) + incl - mine;
+    if (live) {
+      // Operators 16 at a time, one word each: images 2k and 2k + 1 are bits 2 (k & 15)
+      // and 2 (k & 15) + 1.  The operators are uniform (scalar loads).
+      for (uint32_t kb = 0; kb < order; kb += 16) {
+        uint32_t word = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 16; ++j)
+          if (kb + j < order) {
+            const float w = image_scalar(x, ops[kb + j]);
+            word |= (!(w < w_floor) ? 1u : 0u) << (2 * j);
+            word |= (!(-w < w_floor) ? 2u : 0u) << (2 * j);
+          }
+        if (kb == 0) m0 = word & ~1u;  // image 0 is x itself: the core site
+        else if (kb == 16) m1 = word;
+        else if (kb == 32) m2 = word;
+        else m3 = word;
+      }
+      m0 = shell_keep(x, 0, m0, ops, planes, nplanes, band);
+      m1 = shell_keep(x, 32, m1, ops, planes, nplanes, band);
+      m2 = shell_keep(x, 64, m2, ops, planes, nplanes, band);
+      m3 = shell_keep(x, 96, m3, ops, planes, nplanes, band);
+    }
+    const uint32_t mine = uint32_t(__popc(m0) + __popc(m1) + __popc(m2) + __popc(m3));
+    const uint32_t base = block_reserve(&b.info->sites, mine);
     if (live && mine) {
       uint32_t slot = base;
       for (uint32_t word = 0; word < 4; ++word) {
@@ -990,7 +1029,7 @@ __device__ __attribute__((always_inline)) inline void shell_body(
         for (; bits; bits &= bits - 1) {
           const uint32_t img = word * 32 + uint32_t(__builtin_ctz(bits));
           const uint32_t k = img >> 1;
-          y = image(x, ops[k], k);
+          float4 y = image(x, ops[k], k);
           if (img & 1u) y = neg4(y);
           if (slot < b.capacity) {
             b.uq[slot] = y;
