@@ -868,6 +868,8 @@ __device__ inline void box_add(uint32_t lo[3], uint32_t hi[3], const float4 y) {
 
 // Core sites (every source's identity image, at its own index) and the rim list: the
 // sources within the band of some zone plane (shell.wgsl:68-96).
+// The workgroup's minimum (maximum, sum) of v into *dst with one atomic, as every
+// workgroup's atomics on one address serialise.  Every thread of the workgroup calls.
 __device__ inline void block_min_commit(uint32_t* dst, uint32_t v) {
   __shared__ uint32_t sm[kBlock / 64];
   for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
@@ -876,6 +878,26 @@ __device__ inline void block_min_commit(uint32_t* dst, uint32_t v) {
   if (threadIdx.x == 0) {
     for (uint32_t w = 1; w < blockDim.x / 64; ++w) v = min(v, sm[w]);
     if (v != 0xffffffffu) atomicMin(dst, v);
+  }
+}
+__device__ inline void block_max_commit(uint32_t* dst, uint32_t v) {
+  __shared__ uint32_t sm[kBlock / 64];
+  for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) v = max(v, sm[w]);
+    if (v != 0u) atomicMax(dst, v);
+  }
+}
+__device__ inline void block_sum_commit(unsigned long long* dst, unsigned long long v) {
+  __shared__ unsigned long long sm[kBlock / 64];
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < blockDim.x / 64; ++w) v += sm[w];
+    if (v != 0u) atomicAdd(dst, v);
   }
 }
 
@@ -1695,12 +1717,8 @@ __device__ __attribute__((always_inline)) inline void count_reduce_body(
     a += counts[2 * size_t(i)];
     b += counts[2 * size_t(i) + 1];
   }
-  a = wave_sum(a);
-  b = wave_sum(b);
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&counters[0], a);
-    atomicAdd(&counters[1], b);
-  }
+  block_sum_commit(&counters[0], a);
+  block_sum_commit(&counters[1], b);
 }
 __global__ __launch_bounds__(256) void k_count_reduce(
     const uint32_t* __restrict__ counts, uint32_t n, unsigned long long* __restrict__ counters) {
@@ -2079,8 +2097,7 @@ __global__ void k_nearest_unpack(const uint64_t* __restrict__ keys, uint32_t n, 
     rank_out[i] = key == ~uint64_t(0) ? 0xffffffffu : uint32_t(key);
     lo = min(lo, __float_as_uint(d));
   }
-  lo = wave_min_u32(lo);
-  if ((threadIdx.x & 63) == 0) atomicMin(least, lo);
+  block_min_commit(least, lo);
 }
 
 void launch_near_nearest(Prof& p, hipStream_t s, uint32_t n, uint32_t order, const NearBuffers& b, const NearCut& cut,
@@ -2597,8 +2614,7 @@ __global__ void k_degrees(const float* __restrict__ chord, uint32_t n, float* __
     out[i] = d;
     lo = min(lo, __float_as_uint(d));
   }
-  lo = wave_min_u32(lo);
-  if ((threadIdx.x & 63) == 0) atomicMin(least, lo);
+  block_min_commit(least, lo);
 }
 
 void launch_degrees(Prof& p, hipStream_t s, const float* chord, uint32_t n, float* out, uint32_t* least) {
@@ -2896,8 +2912,7 @@ __global__ __launch_bounds__(256) void k_bb_move(JobSet s) {
     // Right-action folding keeps the left tangent frame of the BB history (engine.zig:914-916).
     d.q[i] = fold_point(m, d.group, d.ops, d.order);
   }
-  const uint32_t mx = wave_max_u32(__float_as_uint(moved_size));
-  if ((threadIdx.x & 63) == 0 && mx) atomicMax(&st->step_bits, mx);
+  block_max_commit(&st->step_bits, __float_as_uint(moved_size));
 }
 
 // One BB step of every job in the set, resident, in four launches: stats with the first
