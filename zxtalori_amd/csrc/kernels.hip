@@ -904,36 +904,62 @@ __device__ inline void block_sum_commit(unsigned long long* dst, unsigned long l
 __device__ __attribute__((always_inline)) inline void core_rim_body(
     const float4* __restrict__ q, uint32_t n, const float4* __restrict__ planes, uint32_t nplanes, float band,
     uint32_t two_order, NearBuffers b, uint32_t block, uint32_t grid) {
+  if (n == 0) return;  // uniform: no thread reaches the workgroup's barriers
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   uint32_t wmin = 0xffffffffu;
   const uint32_t stride = grid * blockDim.x;
   const uint32_t rounds = (n + stride - 1) / stride;  // at most 32 (core_rim_blocks)
   const uint32_t first = block * blockDim.x + threadIdx.x;
   uint32_t rim_mask = 0;  // bit r: source r * stride + first is a rim source
-  for (uint32_t r = 0; r < rounds; ++r) {
-    const uint32_t i = r * stride + first;
-    bool rim = false;
-    if (i < n) {
-      const float4 x = q[i];
-      b.uq[i] = x;
-      b.ucode[i] = i | 0x80000000u;
-      b.urank[i] = i * two_order;
+  // Rounds four at a time, with no branch around a round's loads and stores: a lane past n
+  // works on source n - 1 (rewriting its bytes unchanged; the box and the minimum are
+  // idempotent) and its rim bit is dropped.  Each batch's positions are loaded while the
+  // batch before is processed.  (Stores count in the same in-order counter as loads, so a
+  // branch around them made every round wait for the round before's stores.)
+  const uint32_t last = n - 1;
+  float4 cur[4], nxt[4];
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j) cur[j] = q[min(j * stride + first, last)];
+  for (uint32_t r0 = 0; r0 < rounds; r0 += 4) {
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) nxt[j] = q[min((r0 + 4 + j) * stride + first, last)];
+    float de[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      const uint32_t i = (r0 + j) * stride + first, ic = min(i, last);
+      const float4 x = cur[j];
+      b.uq[ic] = x;
+      b.ucode[ic] = ic | 0x80000000u;
+      b.urank[ic] = ic * two_order;
       box_add(lo, hi, x);
       wmin = min(wmin, ordered_key(x.x));
-      // x.p >= p.w - |x - e| (e the identity, |p| = 1) and the planes come nearest first:
-      // once that bound clears the band (with 1e-5 to spare for FP32 rounding), no later
-      // plane can hold x within it.
-      const float de = identity_chord(x);
-      for (uint32_t k = 0; k < nplanes; ++k) {
-        const float4 pl = planes[k];
-        if (pl.x - de > band + 2e-6f + 1e-5f) break;
-        if (dot4(x, pl) <= band + 2e-6f) {
-          rim = true;
-          break;
-        }
+      de[j] = identity_chord(x);
+    }
+    // The rim test of the four sources in one walk over the planes (the walk's scalar loads
+    // and loop control were most of the kernel when each source walked alone).  For each
+    // source: x.p >= p.w - |x - e| (e the identity, |p| = 1) and the planes come nearest
+    // first, so once that bound clears the band (with 1e-5 to spare for FP32 rounding), no
+    // later plane can hold x within it; a source leaves the walk there or at the first
+    // plane that holds it.
+    uint32_t open = 0xfu, rim = 0u;  // bit j: source j still walking / a rim source
+    for (uint32_t k = 0; k < nplanes && open; ++k) {
+      const float4 pl = planes[k];
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const bool past = pl.x - de[j] > band + 2e-6f + 1e-5f;
+        const bool held = dot4(cur[j], pl) <= band + 2e-6f;
+        const uint32_t bit = 1u << j;
+        if ((open & bit) && !past && held) rim |= bit;
+        if (past || held) open &= ~bit;
       }
     }
-    rim_mask |= uint32_t(rim) << r;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      const uint32_t r = r0 + j;
+      rim_mask |= uint32_t((rim >> j) & 1u && r * stride + first < n) << r;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) cur[j] = nxt[j];
   }
   // One append per workgroup for all its rounds.
   uint32_t at = block_reserve(&b.info->rim, uint32_t(__popc(rim_mask)));
