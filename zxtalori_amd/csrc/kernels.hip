@@ -980,6 +980,11 @@ static uint32_t core_rim_blocks(uint32_t n) {
   return std::max<uint32_t>(512, blocks_for(n, kBlock * 32));
 }
 
+// Workgroups of k_shell (each takes rim sources round by round): few enough that the
+// workgroups' atomics on the site counter and the box stay cheap, as every workgroup's
+// atomics on one address serialise.
+constexpr uint32_t kShellBlocks = 512;
+
 // The scalar part of x op (the first component of image(x, op, k)): the images x op_k and
 // -(x op_k) (images 2k and 2k + 1) have scalar parts w and -w.
 __device__ inline float image_scalar(const float4 x, const float4 op) {
@@ -1432,7 +1437,7 @@ void launch_near_sites(Prof& p, hipStream_t s, const float4* q, uint32_t n, cons
                        b.planes, b.nplanes, b.band, 2 * order, b);
   });
   p.run("shell", [&] {
-    hipLaunchKernelGGL(k_shell, dim3(2048), dim3(kBlock), 0, s, q, n, ops, order, b.planes, b.nplanes, b.band, b);
+    hipLaunchKernelGGL(k_shell, dim3(kShellBlocks), dim3(kBlock), 0, s, q, n, ops, order, b.planes, b.nplanes, b.band, b);
   });
   p.run("box", [&] { hipLaunchKernelGGL(k_box_finalize, dim3(1), dim3(1), 0, s, b.info, b.sdim); });
 }
@@ -1799,7 +1804,7 @@ uint32_t near_stage_blocks(const NearJob& j, uint32_t stage) {
   switch (stage) {
     case kNsFill: return std::min<uint32_t>(4096, blocks_for(cells + 1, per));
     case kNsRim: return std::min(core_rim_blocks(j.n), blocks_for(j.n, kBlock));
-    case kNsShell: return std::min<uint32_t>(2048, blocks_for(j.n, kBlock));  // rim sources <= n
+    case kNsShell: return std::min<uint32_t>(kShellBlocks, blocks_for(j.n, kBlock));  // rim sources <= n
     case kNsSites: return std::min<uint32_t>(8192, blocks_for(j.b.capacity, per));
     case kNsCellTiles: return blocks_for(cells, kScanTile);
     case kNsCellAdd: return std::min<uint32_t>(8192, blocks_for(cells, per));
