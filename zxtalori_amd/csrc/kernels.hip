@@ -103,10 +103,6 @@ __device__ inline uint32_t wave_min_u32(uint32_t v) {
   for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
   return v;
 }
-__device__ inline uint32_t wave_max_u32(uint32_t v) {
-  for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
-  return v;
-}
 
 // Loads through a pointer of the global address space are global loads with a scalar
 // base; through a generic pointer read from memory (a job table's), the compiler cannot
