@@ -451,23 +451,49 @@ __global__ __launch_bounds__(256) void k_allpairs(JobSet s) {
   }
 }
 
-__global__ void k_allpairs_reduce(JobSet s) {
-  const uint32_t jb = find_job(s.point_start, s.count, blockIdx.x);
+// The sum of a source's split partials.  Each workgroup takes kReduceSources sources of one
+// source block (kReduceParts workgroups per block): lane l of group g adds splits g,
+// g + kReduceParts, ... of source l (consecutive lanes read consecutive sources), then one
+// lane per source adds the groups' sums in group order.  The order is fixed, so the rows
+// are the same bits every run and in any mix; a thread per source summing every split in
+// turn left a small sample (N = 1000, 96 splits) with 16 waves on the chip, each a long
+// chain of dependent loads and adds.
+constexpr uint32_t kReduceSources = 16, kReduceParts = kBlock / kReduceSources;
+__global__ __launch_bounds__(256) void k_allpairs_reduce(JobSet s) {
+  __shared__ double part[kReduceParts][5][kReduceSources];
+  __shared__ float part_near[kReduceParts][kReduceSources];
+  const uint32_t block = blockIdx.x / kReduceParts, sub = blockIdx.x % kReduceParts;
+  const uint32_t jb = find_job(s.point_start, s.count, block);
   const JobDesc d = load_job(s, jb);
-  const uint32_t i = (blockIdx.x - job_start(s.point_start, jb)) * blockDim.x + threadIdx.x;
-  if (i >= d.n) return;
+  const uint32_t l = threadIdx.x % kReduceSources, g = threadIdx.x / kReduceSources;
+  const uint32_t i = (block - job_start(s.point_start, jb)) * kBlock + sub * kReduceSources + l;
+  const bool live = i < d.n;
   double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   float nearest = 4.0f;
   const uint32_t splits = d.plan.splits();
-  for (uint32_t sp = 0; sp < splits; ++sp) {
-    const size_t o = size_t(sp) * d.n + i;
+  if (live)
+    for (uint32_t sp = g; sp < splits; sp += kReduceParts) {
+      const size_t o = size_t(sp) * d.n + i;
 #pragma unroll
-    for (int c = 0; c < 5; ++c) a[c] += d.ap_partial[5 * o + c];
-    nearest = fminf(nearest, d.ap_pnear[o]);
+      for (int c = 0; c < 5; ++c) a[c] += d.ap_partial[5 * o + c];
+      nearest = fminf(nearest, d.ap_pnear[o]);
+    }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) part[g][c][l] = a[c];
+  part_near[g][l] = nearest;
+  __syncthreads();
+  if (g == 0 && live) {
+    double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    float tn = 4.0f;
+    for (uint32_t h = 0; h < kReduceParts; ++h) {
+#pragma unroll
+      for (int c = 0; c < 5; ++c) t[c] += part[h][c][l];
+      tn = fminf(tn, part_near[h][l]);
+    }
+    d.rows.force[i] = make_float4(float(t[0]), float(t[1]), float(t[2]), float(t[3]));
+    d.rows.energy[i] = t[4];
+    d.rows.nearest[i] = tn;
   }
-  d.rows.force[i] = make_float4(float(a[0]), float(a[1]), float(a[2]), float(a[3]));
-  d.rows.energy[i] = a[4];
-  d.rows.nearest[i] = nearest;
 }
 
 namespace {
@@ -521,7 +547,7 @@ void launch_allpairs(Prof& p, hipStream_t s, const JobSet& jobs) {
   if (!jobs.ap_total) return;
   p.run("allpairs", [&] { hipLaunchKernelGGL(k_allpairs, dim3(jobs.ap_total), dim3(kBlock), 0, s, jobs); });
   p.run("allpairs_reduce",
-        [&] { hipLaunchKernelGGL(k_allpairs_reduce, dim3(jobs.point_total), dim3(kBlock), 0, s, jobs); });
+        [&] { hipLaunchKernelGGL(k_allpairs_reduce, dim3(jobs.point_total * kReduceParts), dim3(kBlock), 0, s, jobs); });
 }
 
 // Nearest orbit point over all pairs with its rank j*2*order + 2*k + sign; the minimum
