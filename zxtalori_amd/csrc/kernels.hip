@@ -533,8 +533,16 @@ AllPairsPlan plan_allpairs(uint32_t n, uint32_t order, double budget_scale, AllP
   if (uint64_t(sb) * p.op_splits < target) chunks = uint32_t((target + sb * p.op_splits - 1) / (sb * p.op_splits));
   // A smaller budget (zo_pair_budget) splits the sites finer: only the rounding changes.
   if (budget_scale > 0.0 && budget_scale < 1.0) chunks = uint32_t(std::min<double>(1e6, chunks / budget_scale));
-  chunks = std::max(1u, std::min(chunks, blocks_for(n, kBlock)));
-  p.site_chunk = blocks_for(n, chunks * kBlock) * kBlock;
+  // Site chunks are whole tiles of kBlock sites, or, when a small sample has fewer tiles
+  // than the target asks for, whole waves of 64: a workgroup's threads each walk a chunk
+  // of sites in turn, and with few workgroups that walk is the pass's critical path.
+  chunks = std::max(1u, chunks);
+  if (chunks <= blocks_for(n, kBlock)) {
+    p.site_chunk = blocks_for(n, chunks * kBlock) * kBlock;
+  } else {
+    chunks = std::min(chunks, blocks_for(n, 64));
+    p.site_chunk = blocks_for(n, chunks * 64) * 64;
+  }
   p.site_splits = blocks_for(n, p.site_chunk);
   if (budget_scale > 0.0 && budget_scale < 1.0) {
     p.ops_per_split = 1;
