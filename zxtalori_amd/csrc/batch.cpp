@@ -38,7 +38,7 @@ Batch::~Batch() {
     (void)hipGraphExecDestroy(r.exec);
     (void)hipEventDestroy(r.done);
   }
-  for (Table* t : {&ap_table_, &step_table_}) {
+  for (Table* t : {&ap_table_, &step_table_, &part_step_table_[0], &part_step_table_[1]}) {
     if (t->host_desc) (void)hipHostFree(t->host_desc);
     if (t->host_start) (void)hipHostFree(t->host_start);
   }
@@ -266,49 +266,76 @@ uint32_t Batch::step(uint32_t most, uint32_t* taken_out) {
   // starts now (a first step sizes buffers and may wait): then a captured graph replays it.
   const bool steady = use_graphs_ && !prof.on() && !fresh && key == last_key_;
   last_key_ = key;
-  auto record = [&]() {
+  // Whether the extra streams are forked from the batch stream and not yet joined back.
+  bool forked = false;
+  auto join_aux = [&](uint32_t lanes) {
+    for (uint32_t k = 0; k < lanes; ++k) {
+      Engine::check(hipEventRecord(join_[k], aux_[k]), "join");
+      Engine::check(hipStreamWaitEvent(stream, join_[k], 0), "join");
+    }
+    forked = false;
+  };
+  // Records one step; `first` and `last`: whether it starts and ends a recorded run of
+  // steps (a single step is both).
+  auto record = [&](bool first, bool last) {
     // Tables first (a captured step cannot upload), then the near-field launches, forked
-    // onto an extra stream after the last step's moves, beside the all-pairs launch, and
-    // joined back before this step's BB step.
-    std::vector<JobDesc> ap, steps;
-    std::vector<NearJob> near;
+    // onto extra streams after the last step's moves, beside the all-pairs launch.
+    std::vector<JobDesc> ap;
+    std::vector<std::pair<NearJob, Job*>> near;
     for (Job* j : active) {
       Engine& e = *j->engine;
       e.prepare_step();
       if (e.relax_state_.has_near) {
         NearJob nj;
-        if (e.near_job(nj)) near.push_back(nj);
+        if (e.near_job(nj)) near.push_back({nj, j});
       } else {
         JobDesc d;
         if (e.step_forces(d)) ap.push_back(d);
       }
     }
-    for (Job* j : active) steps.push_back(j->engine->step_job());
     // Jobs of one force variant share a launch.  With two extra streams the near-field
     // jobs are dealt alternately (in that order) to two sets of similar work, each built and
     // run on its own stream: one set's build overlaps the other's force launches.
-    std::stable_sort(near.begin(), near.end(), [](const NearJob& a, const NearJob& b) { return a.key < b.key; });
+    std::stable_sort(near.begin(), near.end(), [](const auto& a, const auto& b) { return a.first.key < b.first.key; });
     // (Profiled steps stay on one stream: the profiler's events are on the batch stream.)
     const uint32_t lanes = prof.on() ? 0u : std::min<uint32_t>(uint32_t(aux_.size()), near.size() >= 2 ? 2u : 1u);
     std::vector<NearJob> part[2];
-    for (size_t k = 0; k < near.size(); ++k) part[lanes == 2 ? k % 2 : 0].push_back(near[k]);
+    std::vector<Job*> owner[2];
+    for (size_t k = 0; k < near.size(); ++k) {
+      part[lanes == 2 ? k % 2 : 0].push_back(near[k].first);
+      owner[lanes == 2 ? k % 2 : 0].push_back(near[k].second);
+    }
+    // With two sets, each set's BB step follows its force launches on its own stream, so a
+    // run of steps joins the streams only at its end: a set goes on to its next build while
+    // the other set's force launches still run.  The sets' jobs are disjoint and a job's
+    // step reads only its own buffers.  Between runs the streams are joined, so the next
+    // upload (which waits for the batch stream) cannot overwrite a table still in use.
+    const bool independent = lanes == 2;
+    std::vector<JobDesc> steps, part_steps[2];
+    for (Job* j : active) {
+      int in = -1;
+      for (int p = 0; independent && p < 2 && in < 0; ++p)
+        if (std::find(owner[p].begin(), owner[p].end(), j) != owner[p].end()) in = p;
+      (in < 0 ? steps : part_steps[in]).push_back(j->engine->step_job());
+    }
     const JobSet ap_set = upload(ap_table_, ap);
     const JobSet step_set = upload(step_table_, steps);
+    const JobSet part_set[2] = {upload(part_step_table_[0], part_steps[0]), upload(part_step_table_[1], part_steps[1])};
     const NearSet near_set[2] = {upload_near(near_table_[0], part[0]), upload_near(near_table_[1], part[1])};
     const bool split = lanes > 0 && !near.empty() && (lanes == 2 || !ap.empty());
-    if (split) {
+    if (split && (first || !independent)) {
       Engine::check(hipEventRecord(fork_, stream), "fork");
       for (uint32_t k = 0; k < lanes; ++k) Engine::check(hipStreamWaitEvent(aux_[k], fork_, 0), "fork");
+      forked = true;
     }
-    for (uint32_t k = 0; k < 2; ++k) launch_near_jobs(prof, split ? aux_[k < lanes ? k : 0] : stream, near_set[k]);
+    for (uint32_t k = 0; k < 2; ++k) {
+      const hipStream_t on = split ? aux_[k < lanes ? k : 0] : stream;
+      launch_near_jobs(prof, on, near_set[k]);
+      if (independent) launch_bb_step(prof, on, part_set[k]);  // move and fold included
+    }
     if (!ap.empty()) launch_allpairs(prof, stream, ap_set);
-    if (split) {
-      for (uint32_t k = 0; k < lanes; ++k) {
-        Engine::check(hipEventRecord(join_[k], aux_[k]), "join");
-        Engine::check(hipStreamWaitEvent(stream, join_[k], 0), "join");
-      }
-    }
-    launch_bb_step(prof, stream, step_set);  // move and fold included
+    if (split && (last || !independent)) join_aux(lanes);
+    launch_bb_step(prof, stream, step_set);  // the all-pairs jobs' (or every job's)
   };
   // Captures `steps` recorded steps into one executable graph; null when a job table
   // changed during the capture (nothing captured has run: the caller records eagerly).
@@ -321,8 +348,11 @@ uint32_t Batch::step(uint32_t most, uint32_t* taken_out) {
       Engine::check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "capture");
       capturing_ = true;
       try {
-        for (uint32_t k = 0; k < steps; ++k) record();
+        for (uint32_t k = 0; k < steps; ++k) record(k == 0, k + 1 == steps);
       } catch (const GraphTableChanged&) {
+        // A later step of a run found a table changed: the streams forked by its first step
+        // join back, so that the capture ends whole (and is then dropped).
+        if (forked) join_aux(std::min<uint32_t>(uint32_t(aux_.size()), 2u));
         changed = true;
       } catch (...) {
         capturing_ = false;
@@ -374,7 +404,7 @@ uint32_t Batch::step(uint32_t most, uint32_t* taken_out) {
           replays_ += span;
           taken = span;
         } else {
-          record();
+          record(true, true);
         }
       } else {
         Engine::check(hipGraphLaunch(graph_, stream), "graph");
@@ -390,10 +420,10 @@ uint32_t Batch::step(uint32_t most, uint32_t* taken_out) {
         graph_key_ = key;
         Engine::check(hipGraphLaunch(graph_, stream), "graph");
       } else {
-        record();  // capture the next step
+        record(true, true);  // capture the next step
       }
     } else {
-      record();
+      record(true, true);
     }
   } catch (const EngineError& e) {
     for (Job* j : active) {

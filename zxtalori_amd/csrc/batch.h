@@ -111,6 +111,9 @@ class Batch {
   std::vector<std::unique_ptr<Job>> jobs_;
   uint64_t next_id_ = 1;
   Table ap_table_, step_table_;
+  // With two near-field sets on their own streams, each set's BB step runs on its stream
+  // (its own table), so within a recorded run of steps the sets never wait for each other.
+  Table part_step_table_[2];
   bool use_graphs_ = false;
   bool capturing_ = false;
   struct GraphTableChanged {};
