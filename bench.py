@@ -243,6 +243,29 @@ def counters_from_profile(name):
     return out if len(out) == 3 else None
 
 
+def mix_pairs(total=256, seed=193):
+    """Accepted pairs and sites tested of one force pass of each near-field job of the
+    batched mix (serve.mixed_relax_jobs), summed by arctangent variant."""
+    import zxtalori_amd as zo
+    from zxtalori_amd import serve
+    out = {}
+    for job in serve.mixed_relax_jobs(total, seed=seed):
+        if job.neighbourhood <= 0:
+            continue
+        with zo.haar(job.group, job.count, job.seed) as grid:
+            grid.relax(1, neighbourhood=job.neighbourhood)
+            info = grid.near_info()
+        row = out.setdefault(info["atan_terms"], {"jobs": 0, "points": 0, "pairs": 0, "candidates": 0, "sites": 0})
+        row["jobs"] += 1
+        row["points"] += job.count
+        row["pairs"] += info["pairs"]
+        row["candidates"] += info["candidates"]
+        row["sites"] += info["sites"]
+    for row in out.values():
+        row["candidates_per_pair"] = row["candidates"] / max(1, row["pairs"])
+    return out
+
+
 def issue_rates():
     """The chip's measured issue rates (zo_issue_probe): non-packed FP32 FMA,
     transcendental and v_mfma_f32_16x16x4f32 wave64 instructions per second, each the best
@@ -300,9 +323,15 @@ def main():
                     help="skip the generation, covering and exact-count measurements")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (the HBM counter calibration kernel)")
+    ap.add_argument("--mix-pairs", action="store_true",
+                    help="run only one force pass of each near-field job of the batched mix and print its "
+                         "accepted pairs and sites tested by arctangent variant")
     ap.add_argument("--calibrate-issue", action="store_true",
                     help="run only the VALU issue-rate probe (FMA and transcendental chains)")
     args = ap.parse_args()
+    if args.mix_pairs:
+        print(json.dumps({"mix_pairs_by_atan_terms": mix_pairs()}))
+        return
     if args.calibrate_issue:
         print(json.dumps({"issue_rates_wave64_per_s": issue_rates()}))
         return
