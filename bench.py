@@ -525,11 +525,16 @@ def spacing_section(n):
         info = grid.near_info()
     kernels = {k: round(ms / c, 4) for k, (c, ms) in sorted(report.items(), key=lambda kv: -kv[1][1])}
     device_ms = sum(ms for _, ms in report.values())
-    return {"workload": f"zo.haar('I', {n}, 193).spacing(): nearest orbit point of every point, "
-                        "neighbourhood 4 mean radii", "n": n, "ms": dt * 1e3, "device_ms": device_ms,
-            "points_per_s": n / dt, "alg_bytes_per_point": 76,
-            "alg_gbps_device": 76 * n / (device_ms * 1e-3) / 1e9, "kernels_ms": kernels,
-            "sites": info["sites"], "separation_deg": stats["separation_deg"]}
+    out = {"workload": f"zo.haar('I', {n}, 193).spacing(): nearest orbit point of every point, "
+                       "neighbourhood 4 mean radii", "n": n, "ms": dt * 1e3, "device_ms": device_ms,
+           "points_per_s": n / dt, "alg_bytes_per_point": 76,
+           "alg_gbps_device": 76 * n / (device_ms * 1e-3) / 1e9, "kernels_ms": kernels,
+           "sites": info["sites"], "separation_deg": stats["separation_deg"]}
+    if n == 10_000_000 and "near_nearest" in kernels:
+        # The search kernel's executed VALU and transcendental instructions at this workload
+        # (profiles/spacing_i1e7_counters.md) over its measured time, as roofline.valu_busy.
+        out["search_valu_busy"] = valu_busy(counters_from_profile("spacing_i1e7"), kernels["near_nearest"] * 1e-3)
+    return out
 
 
 def _median_of(run, repeats):

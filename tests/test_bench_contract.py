@@ -40,3 +40,9 @@ def test_counter_summaries_parse():
     expected = ((9506401256 - 253788840) * 2 + 253788840 * 8) / (1024 * 2.4e9 * 12.99e-3)
     assert busy == pytest.approx(expected)
     assert busy == pytest.approx(0.643, abs=0.001)
+
+
+def test_spacing_counter_summary_parses():
+    counts = bench.counters_from_profile("spacing_i1e7")
+    assert counts == {"valu_per_launch": 778408599.0, "trans_per_launch": 3769623.0, "salu_per_launch": 125594259.0}
+    assert bench.valu_busy(counts, 1.26e-3) == pytest.approx(0.51, abs=0.005)
